@@ -1,0 +1,212 @@
+/*
+ * robtop_amd — C ABI of the MI355X-native multigrid FEM / SIMP hot path.
+ *
+ * Every entry point returns 0 on success and a negative code on failure
+ * (RT_E*); rt_last_error() gives the message of the calling thread's last
+ * failure. Handles are not thread-safe; each owns one HIP stream and all of its
+ * device memory. Host buffers are owned by the caller.
+ *
+ * Layouts at the boundary:
+ *   - solid bits: uint32 words, LSB-first, bit id x + y*R + z*R^2 (what the
+ *     reference holds after wordReverse_g, grid/Grid.cu:2662-2687);
+ *   - "box" arrays: a level's dense node box (nx, ny, nz) or element box
+ *     (nx-1, ny-1, nz-1), x fastest; rt_level_info gives origin and sizes;
+ *   - "active" arrays: only the solid elements / existing nodes, in
+ *     lexicographic (z, y, x) order — the reference's lexicographic ids before
+ *     its colour permutation (eidmap/vidmap, grid/Grid.h:220-221);
+ *   - 3-vectors: three consecutive blocks [x..., y..., z...] (SoA) in FP64.
+ *
+ * Each declaration names the reference interface it replaces (file:line in
+ * lavenklau/robtop).
+ */
+#ifndef ROBTOP_AMD_H
+#define ROBTOP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    RT_OK = 0,
+    RT_EARG = -1,     /* invalid argument / size */
+    RT_EHIP = -2,     /* HIP runtime failure */
+    RT_ESTATE = -3,   /* call out of order (e.g. solve before update_stencil) */
+    RT_ENUM = -4,     /* numerical failure (non-SPD coarsest system, breakdown) */
+    RT_EINTERNAL = -5
+};
+
+/* solver modes for rt_solve */
+enum {
+    RT_SOLVE_FP64 = 0,      /* FP64 PCG, FP64 V-cycle preconditioner */
+    RT_SOLVE_FP32 = 1,      /* FP32 PCG vectors (FP64 dots), FP32 V-cycle */
+    RT_SOLVE_MIXED = 2      /* FP64 PCG vectors and dots, FP32 V-cycle preconditioner */
+};
+
+/* vector selectors for the per-level accessors */
+enum { RT_VEC_U = 0, RT_VEC_F = 1, RT_VEC_R = 2 };
+
+typedef struct rt_hier rt_hier;
+
+typedef struct rt_solve_stats {
+    int iterations;         /* PCG iterations */
+    double rel_res;         /* ||f - K u|| / ||f|| of the returned u, recomputed in FP64 */
+    double rel_res_pcg;     /* the PCG recurrence residual at exit */
+    double seconds;         /* wall time of the solve */
+} rt_solve_stats;
+
+const char* rt_last_error(void);
+const char* rt_version(void);
+
+/* --- hierarchy --------------------------------------------------------- */
+
+/* Build the multigrid hierarchy from a solid voxel set of a cube of side R
+ * (R a power of two). Replaces HierarchyGrid::genFromMesh(solid_bits, reso)
+ * (grid/Grid.cpp:455-712): levels coarsen by OR-pooling until a level has at
+ * most 400 solid elements (Grid.cpp:229), layer 1 is skipped (x4 first
+ * coarsening, Grid.cpp:390-393, optimization.cpp:17). Mode is "with support"
+ * (wsff/wscf, grid/Grid.h:146-151). */
+int rt_create_from_voxels(int R, const uint32_t* solid_bits, size_t nwords, rt_hier** out);
+int rt_destroy(rt_hier* h);
+
+/* number of stored levels (layer 1 excluded) and of reference layers */
+int rt_num_levels(const rt_hier* h);
+int rt_num_layers(const rt_hier* h);
+int rt_layer_active_elements(const rt_hier* h, int layer);
+/* info[11] = {layer, scale, ox, oy, oz, ex, ey, ez, nx, ny, nz}; origin in the
+ * level's absolute element coordinates */
+int rt_level_info(const rt_hier* h, int level, int* info);
+/* masks of the level's element box and node box (1 = solid / existing) */
+int rt_level_masks(const rt_hier* h, int level, uint8_t* eact_box, uint8_t* vact_box);
+int rt_num_active(const rt_hier* h, int level, size_t* n_elements, size_t* n_nodes);
+
+/* --- material and boundary conditions ---------------------------------- */
+
+/* Young's modulus, Poisson ratio, SIMP power. Replaces setParameters
+ * (optimization/optimization.cpp:32-50) + initTemplateMatrix
+ * (element/templateMatrix.cpp:31-124) + uploadTemplateMatrix
+ * (optimization/optimization.cu:330-343). Element size 1 (Grid.cpp:457-461). */
+int rt_set_material(rt_hier* h, double youngs, double poisson, float power_penalty);
+/* element stiffness KE (576 doubles, row-major) as used on the device */
+int rt_get_template_matrix(const rt_hier* h, double* ke576);
+
+/* Support (fixed) node flags on the finest node box. Replaces the support
+ * flag set in Grid::computeProjectionMatrix (grid/Grid.cpp:1291-1403). */
+int rt_set_support_box(rt_hier* h, const uint8_t* fixed_box);
+int rt_set_support_active(rt_hier* h, const uint8_t* fixed_active);
+/* Nodal force (FP64, 3 x n) on the finest level; zeroed at supports.
+ * Replaces setForceSupport (optimization/projection.cu:344-369). */
+int rt_set_force_box(rt_hier* h, const double* f_box);
+int rt_set_force_active(rt_hier* h, const double* f_active);
+int rt_get_force_box(const rt_hier* h, double* f_box);
+/* shell elements (density forced to 1 by the OC update, optimization.cu:159) */
+int rt_set_shell_box(rt_hier* h, const uint8_t* shell_box);
+
+/* --- densities ----------------------------------------------------------- */
+
+/* Replaces initDensities / Grid::init_rho (optimization.cpp:687-690) and the
+ * device density array rho_e (Grid.h:196). */
+int rt_init_density(rt_hier* h, float rho);
+int rt_set_density_box(rt_hier* h, const float* rho_box);
+int rt_get_density_box(const rt_hier* h, float* rho_box);
+int rt_set_density_active(rt_hier* h, const float* rho_active);
+int rt_get_density_active(const rt_hier* h, float* rho_active);
+/* rho^p as evaluated on the device (FP32 powf, grid/Grid.cu:834) */
+int rt_get_rhop_box(const rt_hier* h, float* rhop_box);
+
+/* --- multigrid ----------------------------------------------------------- */
+
+/* Galerkin coarse operators for the current densities. Replaces
+ * HierarchyGrid::update_stencil (grid/Grid.cpp:2108-2119): x4 restriction of
+ * the penalised fine operator (Grid.cu:392-506), x2 restrictions
+ * (Grid.cu:120-177) and the coarsest dense factorisation (Grid.cpp:1884-1939). */
+int rt_update_stencil(rt_hier* h);
+/* stencil [27][9][nodes] of a coarse level on its node box */
+int rt_get_stencil_box(const rt_hier* h, int level, double* st_box);
+
+/* Precision of the V-cycle used by rt_vcycle and the per-level operations:
+ * 64 (the reference's FP64) or 32. */
+int rt_set_vcycle_precision(rt_hier* h, int bits);
+
+/* Per-level vectors (U, F, R) of the V-cycle, box layout, FP64 at the boundary. */
+int rt_set_level_vec(rt_hier* h, int level, int which, const double* v_box);
+int rt_get_level_vec(const rt_hier* h, int level, int which, double* v_box);
+
+/* Grid::gs_relax (grid/Grid.cu:1068-1111); reverse=1 runs colours 7..0 and
+ * components z,y,x (the symmetric counterpart, no reference equivalent). */
+int rt_gs_relax(rt_hier* h, int level, int nsweeps, int reverse);
+/* Grid::update_residual (grid/Grid.cu:1644-1678) */
+int rt_update_residual(rt_hier* h, int level);
+/* Grid::restrict_residual (grid/Grid.cu:1796-1816): level = the coarse level */
+int rt_restrict_residual(rt_hier* h, int level);
+/* Grid::prolongate_correction (grid/Grid.cu:1898-1915): level = the fine level */
+int rt_prolongate_correction(rt_hier* h, int level);
+/* Grid::reset_displacement (grid/Grid.cu:1917-1922) */
+int rt_reset_displacement(rt_hier* h, int level);
+/* Grid::solve_fem_host (grid/Grid.cpp:1981-2043) on the coarsest level */
+int rt_solve_coarsest(rt_hier* h);
+/* Grid::relative_residual (grid/Grid.cu:1946-1951) of the finest level */
+int rt_relative_residual(rt_hier* h, double* rel);
+
+/* One V(pre, post) cycle on the finest displacement/force. Replaces
+ * HierarchyGrid::v_cycle (grid/Grid.cpp:1068-1113); returns ||r||/||f||. */
+int rt_vcycle(rt_hier* h, int pre, int post, double* rel_res);
+
+/* --- solve and SIMP step ------------------------------------------------- */
+
+/* MG-preconditioned CG to ||r||/||f|| <= tol from the current displacement
+ * (new; the reference iterates V-cycles: solveFEM, optimization.cpp:85-91,
+ * and test_utils.cpp:633-637). Preconditioner: V(pre, post) with symmetric
+ * (reversed) post-smoothing. */
+int rt_solve(rt_hier* h, double tol, int maxit, int mode, int pre, int post, rt_solve_stats* stats);
+
+int rt_set_displacement_box(rt_hier* h, const double* u_box);
+int rt_get_displacement_box(const rt_hier* h, double* u_box);
+int rt_get_displacement_active(const rt_hier* h, double* u_active);
+
+/* Grid::compliance = F.U (grid/Grid.cpp:1976-1979) */
+int rt_compliance(rt_hier* h, double* c);
+
+/* computeSensitivity (optimization/optimization.cu:109-133) without the
+ * filter: g_e = -p rho^(p-1) u_e^T KE u_e */
+int rt_compute_sensitivity(rt_hier* h);
+/* Grid::filterSensitivity (grid/Grid.cu:2411-2437) */
+int rt_filter_sensitivity(rt_hier* h, double radius);
+int rt_get_sensitivity_box(const rt_hier* h, float* g_box);
+int rt_set_sensitivity_box(rt_hier* h, const float* g_box);
+
+/* updateDensities (optimization/optimization.cu:164-226): OC bisection on the
+ * multiplier; volume averaged over solid elements. Returns the multiplier and
+ * the number of trial evaluations. */
+int rt_update_densities(rt_hier* h, float vgoal, float step, float damp, float rho_min, float* multiplier,
+                        int* trials);
+/* mean density over solid elements */
+int rt_volume(rt_hier* h, double* v);
+
+/* --- voxelizer ----------------------------------------------------------- */
+
+/* voxelize_mesh (voxelizer/voxelize_main.cpp:344-404): padded cube of side
+ * R = smallest power of two > prefer_reso, parity fill on the GPU
+ * (voxelize_solid.cu:78-150). bits must hold R^3/32 words; out_box = {min[3],
+ * max[3]}. */
+int rt_voxelize_mesh(const float* coords, int nverts, const int* faces, int nfaces, int prefer_reso,
+                     uint32_t* bits, size_t maxwords, int* out_reso, float* out_box);
+
+/* --- timing hooks (benchmarking) ----------------------------------------- */
+
+int rt_synchronize(rt_hier* h);
+/* Runs n preconditioner V-cycles (precision `bits`) on the current force and
+ * returns the mean milliseconds per cycle measured with HIP events on the
+ * handle's stream. */
+int rt_time_vcycle(rt_hier* h, int bits, int pre, int post, int n, double* ms_per_cycle);
+/* Times n full fine-level smoothing sweeps (8 colour passes) of precision
+ * `bits`; returns mean ms per sweep. */
+int rt_time_fine_sweep(rt_hier* h, int bits, int n, double* ms_per_sweep);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ROBTOP_AMD_H */

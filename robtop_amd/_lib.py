@@ -1,0 +1,120 @@
+"""ctypes binding of librobtop_amd.so (the C ABI in include/robtop_amd.h).
+
+The shared library is loaded from this package directory only. There is no
+fallback: if it is missing or fails to load, every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librobtop_amd.so")
+_LIB = None
+
+RT_OK, RT_EARG, RT_EHIP, RT_ESTATE, RT_ENUM, RT_EINTERNAL = 0, -1, -2, -3, -4, -5
+RT_SOLVE_FP64, RT_SOLVE_FP32, RT_SOLVE_MIXED = 0, 1, 2
+RT_VEC_U, RT_VEC_F, RT_VEC_R = 0, 1, 2
+
+
+class RobtopError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"robtop_amd error {code}: {msg}")
+        self.code = code
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("rel_res", ctypes.c_double),
+                ("rel_res_pcg", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+D = ctypes.c_double
+F = ctypes.c_float
+SZ = ctypes.c_size_t
+
+# name: (argtypes, restype)
+SIGNATURES = {
+    "rt_last_error": ([], ctypes.c_char_p),
+    "rt_version": ([], ctypes.c_char_p),
+    "rt_create_from_voxels": ([I, P, SZ, P], I),
+    "rt_destroy": ([P], I),
+    "rt_num_levels": ([P], I),
+    "rt_num_layers": ([P], I),
+    "rt_layer_active_elements": ([P, I], I),
+    "rt_level_info": ([P, I, P], I),
+    "rt_level_masks": ([P, I, P, P], I),
+    "rt_num_active": ([P, I, P, P], I),
+    "rt_set_material": ([P, D, D, F], I),
+    "rt_get_template_matrix": ([P, P], I),
+    "rt_set_support_box": ([P, P], I),
+    "rt_set_support_active": ([P, P], I),
+    "rt_set_force_box": ([P, P], I),
+    "rt_set_force_active": ([P, P], I),
+    "rt_get_force_box": ([P, P], I),
+    "rt_set_shell_box": ([P, P], I),
+    "rt_init_density": ([P, F], I),
+    "rt_set_density_box": ([P, P], I),
+    "rt_get_density_box": ([P, P], I),
+    "rt_set_density_active": ([P, P], I),
+    "rt_get_density_active": ([P, P], I),
+    "rt_get_rhop_box": ([P, P], I),
+    "rt_update_stencil": ([P], I),
+    "rt_get_stencil_box": ([P, I, P], I),
+    "rt_set_vcycle_precision": ([P, I], I),
+    "rt_set_level_vec": ([P, I, I, P], I),
+    "rt_get_level_vec": ([P, I, I, P], I),
+    "rt_gs_relax": ([P, I, I, I], I),
+    "rt_update_residual": ([P, I], I),
+    "rt_restrict_residual": ([P, I], I),
+    "rt_prolongate_correction": ([P, I], I),
+    "rt_reset_displacement": ([P, I], I),
+    "rt_solve_coarsest": ([P], I),
+    "rt_relative_residual": ([P, P], I),
+    "rt_vcycle": ([P, I, I, P], I),
+    "rt_solve": ([P, D, I, I, I, I, P], I),
+    "rt_set_displacement_box": ([P, P], I),
+    "rt_get_displacement_box": ([P, P], I),
+    "rt_get_displacement_active": ([P, P], I),
+    "rt_compliance": ([P, P], I),
+    "rt_compute_sensitivity": ([P], I),
+    "rt_filter_sensitivity": ([P, D], I),
+    "rt_get_sensitivity_box": ([P, P], I),
+    "rt_set_sensitivity_box": ([P, P], I),
+    "rt_update_densities": ([P, F, F, F, F, P, P], I),
+    "rt_volume": ([P, P], I),
+    "rt_voxelize_mesh": ([P, I, P, I, I, P, SZ, P, P], I),
+    "rt_synchronize": ([P], I),
+    "rt_time_vcycle": ([P, I, I, I, I, P], I),
+    "rt_time_fine_sweep": ([P, I, I, P], I),
+}
+
+
+def lib():
+    """Load librobtop_amd.so (raises if it is not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m robtop_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _LIB = L
+    return _LIB
+
+
+def check(code):
+    if code < 0:
+        raise RobtopError(code, lib().rt_last_error().decode())
+    return code
+
+
+def ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def as_c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)

@@ -1,0 +1,511 @@
+// Coarse-level stencil smoother/residual, grid transfers, coarsest solve,
+// deterministic FP64 reductions and the PCG vector updates.
+#include "rt_internal.h"
+
+namespace rt {
+
+// ---------------------------------------------------------------------------
+// helpers shared with k_fine.hip (duplicated as inline templates)
+// ---------------------------------------------------------------------------
+template <int C>
+__device__ __forceinline__ bool c_colour_thread(const Geom& g, int t, int& base) {
+    constexpr int cx = C & 1, cy = (C >> 1) & 1, cz = C >> 2;
+    const int nhx = (g.n[0] - cx + 1) >> 1, nhy = (g.n[1] - cy + 1) >> 1, nhz = (g.n[2] - cz + 1) >> 1;
+    if (t >= nhx * nhy * nhz) return false;
+    const int i = t % nhx;
+    const int r = t / nhx;
+    base = ((r / nhy + 1) * g.h[1] + (r % nhy + 1)) * g.h[0] + (i + 1);
+    return true;
+}
+
+template <int C>
+__device__ __forceinline__ int c_nb_slot(const Geom& g, int base, int nb) {
+    constexpr int cx = C & 1, cy = (C >> 1) & 1, cz = C >> 2;
+    const int dx = nb % 3 - 1, dy = nb / 3 % 3 - 1, dz = nb / 9 - 1;
+    const int col = C ^ ((dx & 1) | ((dy & 1) << 1) | ((dz & 1) << 2));
+    const int ox = (cx + dx) >> 1, oy = (cy + dy) >> 1, oz = (cz + dz) >> 1;
+    return col * g.cs + base + (oz * g.h[1] + oy) * g.h[0] + ox;
+}
+
+static inline unsigned colour_blocks(const Geom& g, int colour, int bs) {
+    const int cx = colour & 1, cy = (colour >> 1) & 1, cz = colour >> 2;
+    long long tot = (long long)((g.n[0] - cx + 1) >> 1) * ((g.n[1] - cy + 1) >> 1) * ((g.n[2] - cz + 1) >> 1);
+    return (unsigned)((tot + bs - 1) / bs);
+}
+
+template <typename TS> __device__ __forceinline__ double ld_st(const TS* p) { return (double)*p; }
+
+// ---------------------------------------------------------------------------
+// coarse smoother: Grid.cu:678-790. Au over the 26 off-centre stencil blocks,
+// then component-wise GS with the centre block (stencil index (nb*9 + 3r + q)).
+// ---------------------------------------------------------------------------
+template <typename T, typename TS, int C>
+__device__ __forceinline__ void coarse_gs_body(const Geom& g, Vec3<T> U, Vec3<const T> F, const TS* __restrict__ st,
+                                               const uint8_t* __restrict__ vflag, bool reverse) {
+    int base;
+    if (!c_colour_thread<C>(g, blockIdx.x * blockDim.x + threadIdx.x, base)) return;
+    const int v = C * g.cs + base;
+    if (!(vflag[v] & NF_ACTIVE)) return;
+    const size_t N = (size_t)g.N;
+    T Au[3] = {T(0), T(0), T(0)};
+#pragma unroll
+    for (int nb = 0; nb < 27; nb++) {
+        if (nb == 13) continue;
+        const int s = c_nb_slot<C>(g, base, nb);
+        const T u0 = U.c[0][s], u1 = U.c[1][s], u2 = U.c[2][s];
+        const TS* p = st + (size_t)nb * 9 * N + v;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            Au[r] += (T)p[(3 * r + 0) * N] * u0;
+            Au[r] += (T)p[(3 * r + 1) * N] * u1;
+            Au[r] += (T)p[(3 * r + 2) * N] * u2;
+        }
+    }
+    const TS* sc = st + (size_t)13 * 9 * N + v;
+    T S[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) S[i] = (T)sc[i * N];
+    T u[3] = {U.c[0][v], U.c[1][v], U.c[2][v]};
+    const T f[3] = {F.c[0][v], F.c[1][v], F.c[2][v]};
+    if (!reverse) {
+        u[0] = (f[0] - Au[0] - (S[1] * u[1] + S[2] * u[2])) / S[0];
+        u[1] = (f[1] - Au[1] - (S[3] * u[0] + S[5] * u[2])) / S[4];
+        u[2] = (f[2] - Au[2] - (S[6] * u[0] + S[7] * u[1])) / S[8];
+    } else {
+        u[2] = (f[2] - Au[2] - (S[6] * u[0] + S[7] * u[1])) / S[8];
+        u[1] = (f[1] - Au[1] - (S[3] * u[0] + S[5] * u[2])) / S[4];
+        u[0] = (f[0] - Au[0] - (S[1] * u[1] + S[2] * u[2])) / S[0];
+    }
+    U.c[0][v] = u[0]; U.c[1][v] = u[1]; U.c[2][v] = u[2];
+}
+
+template <typename T, typename TS, int C>
+__global__ void __launch_bounds__(256) k_coarse_gs(Geom g, Vec3<T> U, Vec3<const T> F, const TS* __restrict__ st,
+                                                   const uint8_t* __restrict__ vflag, int reverse) {
+    coarse_gs_body<T, TS, C>(g, U, F, st, vflag, reverse != 0);
+}
+
+template <typename T, typename TS>
+void launch_coarse_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const TS* st, const uint8_t* vflag, int colour,
+                      bool reverse, hipStream_t s) {
+    unsigned nb = colour_blocks(g, colour, 256);
+    if (!nb) return;
+    switch (colour) {
+#define RT_CASE(c) case c: hipLaunchKernelGGL((k_coarse_gs<T, TS, c>), dim3(nb), dim3(256), 0, s, g, U, F, st, vflag, (int)reverse); break;
+        RT_CASE(0) RT_CASE(1) RT_CASE(2) RT_CASE(3) RT_CASE(4) RT_CASE(5) RT_CASE(6) RT_CASE(7)
+#undef RT_CASE
+    }
+    RT_HIP(hipGetLastError());
+}
+
+// coarse residual: Grid.cu:1349-1448
+template <typename T, typename TS, int C>
+__device__ __forceinline__ void coarse_res_body(const Geom& g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R,
+                                                const TS* __restrict__ st, const uint8_t* __restrict__ vflag) {
+    int base;
+    if (!c_colour_thread<C>(g, blockIdx.x * blockDim.x + threadIdx.x, base)) return;
+    const int v = C * g.cs + base;
+    if (!(vflag[v] & NF_ACTIVE)) return;
+    const size_t N = (size_t)g.N;
+    T Au[3] = {T(0), T(0), T(0)};
+#pragma unroll
+    for (int nb = 0; nb < 27; nb++) {
+        const int s = c_nb_slot<C>(g, base, nb);
+        const T u0 = U.c[0][s], u1 = U.c[1][s], u2 = U.c[2][s];
+        const TS* p = st + (size_t)nb * 9 * N + v;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            Au[r] += (T)p[(3 * r + 0) * N] * u0;
+            Au[r] += (T)p[(3 * r + 1) * N] * u1;
+            Au[r] += (T)p[(3 * r + 2) * N] * u2;
+        }
+    }
+    R.c[0][v] = F.c[0][v] - Au[0];
+    R.c[1][v] = F.c[1][v] - Au[1];
+    R.c[2][v] = F.c[2][v] - Au[2];
+}
+
+template <typename T, typename TS>
+__global__ void __launch_bounds__(256) k_coarse_res(Geom g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R,
+                                                    const TS* __restrict__ st, const uint8_t* __restrict__ vflag) {
+    switch (blockIdx.y) {
+#define RT_CASE(c) case c: coarse_res_body<T, TS, c>(g, U, F, R, st, vflag); break;
+        RT_CASE(0) RT_CASE(1) RT_CASE(2) RT_CASE(3) RT_CASE(4) RT_CASE(5) RT_CASE(6) RT_CASE(7)
+#undef RT_CASE
+    }
+}
+
+template <typename T, typename TS>
+void launch_coarse_residual(const Geom& g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R, const TS* st,
+                            const uint8_t* vflag, hipStream_t s) {
+    dim3 grid(colour_blocks(g, 0, 256), 8);
+    hipLaunchKernelGGL((k_coarse_res<T, TS>), grid, dim3(256), 0, s, g, U, F, R, st, vflag);
+    RT_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// restriction F_c = P^T r_f over the (2 ratio - 1)^3 window with weights
+// prod (ratio - |d|) / ratio (x4: Grid.cu:1733-1794; x2: Grid.cu:1680-1731).
+// One thread per coarse node; rows of fine nodes are read with unit stride in x.
+// ---------------------------------------------------------------------------
+template <typename T, int RATIO>
+__global__ void __launch_bounds__(256) k_restrict(Geom gf, int ofx, int ofy, int ofz, Vec3<const T> Rf, Geom gc,
+                                                  int ocx, int ocy, int ocz, Vec3<T> Fc,
+                                                  const uint8_t* __restrict__ vflag_c) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncx = gc.n[0], ncy = gc.n[1], ncz = gc.n[2];
+    if (t >= ncx * ncy * ncz) return;
+    const int x = t % ncx, y = (t / ncx) % ncy, z = t / (ncx * ncy);
+    const int vc = oct_p(gc, x, y, z);
+    if (!(vflag_c[vc] & NF_ACTIVE)) return;
+    // coarse node position in fine-box coords
+    const int fx = (ocx + x) * RATIO - ofx, fy = (ocy + y) * RATIO - ofy, fz = (ocz + z) * RATIO - ofz;
+    constexpr int W = RATIO - 1;
+    constexpr T inv = T(1) / T(RATIO * RATIO * RATIO);
+    T s0 = T(0), s1 = T(0), s2 = T(0);
+    for (int dz = -W; dz <= W; dz++) {
+        const int pz = fz + dz;
+        if (pz < 0 || pz >= gf.n[2]) continue;
+        const int wz = RATIO - (dz < 0 ? -dz : dz);
+        for (int dy = -W; dy <= W; dy++) {
+            const int py = fy + dy;
+            if (py < 0 || py >= gf.n[1]) continue;
+            const int wyz = wz * (RATIO - (dy < 0 ? -dy : dy));
+#pragma unroll
+            for (int dx = -W; dx <= W; dx++) {
+                const int px = fx + dx;
+                if (px < 0 || px >= gf.n[0]) continue;
+                const T w = (T)(wyz * (RATIO - (dx < 0 ? -dx : dx))) * inv;
+                const int s = oct_p(gf, px, py, pz);
+                s0 += w * Rf.c[0][s];
+                s1 += w * Rf.c[1][s];
+                s2 += w * Rf.c[2][s];
+            }
+        }
+    }
+    Fc.c[0][vc] = s0; Fc.c[1][vc] = s1; Fc.c[2][vc] = s2;
+}
+
+template <typename T>
+void launch_restrict(const Geom& gf, const int of[3], Vec3<const T> Rf, const Geom& gc, const int oc[3], Vec3<T> Fc,
+                     const uint8_t* vflag_c, int ratio, hipStream_t s) {
+    long long tot = (long long)gc.n[0] * gc.n[1] * gc.n[2];
+    dim3 grid((unsigned)((tot + 255) / 256));
+    if (ratio == 4)
+        hipLaunchKernelGGL((k_restrict<T, 4>), grid, dim3(256), 0, s, gf, of[0], of[1], of[2], Rf, gc, oc[0], oc[1],
+                           oc[2], Fc, vflag_c);
+    else
+        hipLaunchKernelGGL((k_restrict<T, 2>), grid, dim3(256), 0, s, gf, of[0], of[1], of[2], Rf, gc, oc[0], oc[1],
+                           oc[2], Fc, vflag_c);
+    RT_HIP(hipGetLastError());
+}
+
+// prolongation u_f += P u_c (x4: Grid.cu:1859-1895; x2: Grid.cu:1818-1856)
+template <typename T, int RATIO>
+__global__ void __launch_bounds__(256) k_prolong(Geom gf, int ofx, int ofy, int ofz, Vec3<T> Uf,
+                                                 const uint8_t* __restrict__ vflag_f, Geom gc, int ocx, int ocy,
+                                                 int ocz, Vec3<const T> Uc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = gf.n[0], ny = gf.n[1], nz = gf.n[2];
+    if (t >= nx * ny * nz) return;
+    const int x = t % nx, y = (t / nx) % ny, z = t / (nx * ny);
+    const int v = oct_p(gf, x, y, z);
+    if (!(vflag_f[v] & NF_ACTIVE)) return;
+    const int ax = ofx + x, ay = ofy + y, az = ofz + z;
+    const int px = ax % RATIO, py = ay % RATIO, pz = az % RATIO;
+    const int bx = ax / RATIO - ocx, by = ay / RATIO - ocy, bz = az / RATIO - ocz;
+    constexpr T inv = T(1) / T(RATIO * RATIO * RATIO);
+    T c0 = T(0), c1 = T(0), c2 = T(0);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int wx = (i & 1) ? px : RATIO - px;
+        const int wy = (i & 2) ? py : RATIO - py;
+        const int wz = (i & 4) ? pz : RATIO - pz;
+        const int w = wx * wy * wz;
+        if (w == 0) continue;
+        const int s = oct_p(gc, bx + (i & 1), by + ((i >> 1) & 1), bz + (i >> 2));
+        const T ww = (T)w * inv;
+        c0 += ww * Uc.c[0][s];
+        c1 += ww * Uc.c[1][s];
+        c2 += ww * Uc.c[2][s];
+    }
+    Uf.c[0][v] += c0; Uf.c[1][v] += c1; Uf.c[2][v] += c2;
+}
+
+template <typename T>
+void launch_prolong(const Geom& gf, const int of[3], Vec3<T> Uf, const uint8_t* vflag_f, const Geom& gc,
+                    const int oc[3], Vec3<const T> Uc, int ratio, bool, hipStream_t s) {
+    long long tot = (long long)gf.n[0] * gf.n[1] * gf.n[2];
+    dim3 grid((unsigned)((tot + 255) / 256));
+    if (ratio == 4)
+        hipLaunchKernelGGL((k_prolong<T, 4>), grid, dim3(256), 0, s, gf, of[0], of[1], of[2], Uf, vflag_f, gc, oc[0],
+                           oc[1], oc[2], Uc);
+    else
+        hipLaunchKernelGGL((k_prolong<T, 2>), grid, dim3(256), 0, s, gf, of[0], of[1], of[2], Uf, vflag_f, gc, oc[0],
+                           oc[1], oc[2], Uc);
+    RT_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// coarsest: u = Ainv f on the active nodes (rowslot[r] = slot of node r)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_gather_rows(const int* __restrict__ rowslot, int nrows, Vec3<const T> F, double* __restrict__ fb) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const int s = rowslot[r];
+    fb[3 * r + 0] = (double)F.c[0][s];
+    fb[3 * r + 1] = (double)F.c[1][s];
+    fb[3 * r + 2] = (double)F.c[2][s];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_dense_gemv(const double* __restrict__ A, int n, const double* __restrict__ f,
+                                                   const int* __restrict__ rowslot, Vec3<T> U) {
+    const int row = blockIdx.x;
+    double s = 0;
+    const double* a = A + (size_t)row * n;
+    for (int j = threadIdx.x; j < n; j += 64) s += a[j] * f[j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (threadIdx.x == 0) U.c[row % 3][rowslot[row / 3]] = (T)s;
+}
+
+template <typename T>
+void launch_coarsest(const double* Ainv, int n3, const int* rowslot, int, Vec3<const T> F, Vec3<T> U, double* fbuf,
+                     hipStream_t s) {
+    const int nrows = n3 / 3;
+    hipLaunchKernelGGL((k_gather_rows<T>), dim3((nrows + 255) / 256), dim3(256), 0, s, rowslot, nrows, F, fbuf);
+    hipLaunchKernelGGL((k_dense_gemv<T>), dim3(n3), dim3(64), 0, s, Ainv, n3, (const double*)fbuf, rowslot, U);
+    RT_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// vector helpers and deterministic two-stage FP64 reductions
+// ---------------------------------------------------------------------------
+template <typename TA, typename TB>
+__global__ void k_convert(const TA* __restrict__ a, TB* __restrict__ b, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = (TB)a[i];
+}
+template <typename TA, typename TB>
+void launch_convert(const TA* a, TB* b, size_t n, hipStream_t s) {
+    unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    if (!nb) return;
+    hipLaunchKernelGGL((k_convert<TA, TB>), dim3(nb), dim3(256), 0, s, a, b, n);
+    RT_HIP(hipGetLastError());
+}
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ a, T v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
+}
+template <typename T>
+void launch_fill(T* a, T v, size_t n, hipStream_t s) {
+    unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    if (!nb) return;
+    hipLaunchKernelGGL((k_fill<T>), dim3(nb), dim3(256), 0, s, a, v, n);
+    RT_HIP(hipGetLastError());
+}
+
+__device__ __forceinline__ double blk_sum(double v) {
+    __shared__ double red[16];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; i++) t += red[i];
+    }
+    return t;
+}
+
+template <typename TA, typename TB>
+__global__ void __launch_bounds__(256) k_dot_partials(const TA* __restrict__ a, const TB* __restrict__ b, size_t n,
+                                                      double* __restrict__ partials) {
+    double s = 0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        s += (double)a[i] * (double)b[i];
+    s = blk_sum(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <typename TA, typename TB>
+void launch_dot_partials_t(const TA* a, const TB* b, size_t n, double* partials, int nparts, hipStream_t s) {
+    hipLaunchKernelGGL((k_dot_partials<TA, TB>), dim3(nparts), dim3(256), 0, s, a, b, n, partials);
+    RT_HIP(hipGetLastError());
+}
+void launch_dot_partials(const double* a, const double* b, size_t n, double* partials, int nparts, hipStream_t s) {
+    launch_dot_partials_t<double, double>(a, b, n, partials, nparts, s);
+}
+
+__global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__ p, int n, double* __restrict__ out) {
+    double s = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+    s = blk_sum(s);
+    if (threadIdx.x == 0) out[0] = s;
+}
+void launch_sum_partials(const double* partials, int nparts, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partials, nparts, out);
+    RT_HIP(hipGetLastError());
+}
+
+// x += a p; r -= a q; partials of r.r; rz = (TZ) r
+template <typename T, typename TZ>
+__global__ void __launch_bounds__(256) k_pcg_xr(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+                                                const T* __restrict__ q, double alpha, size_t n,
+                                                double* __restrict__ partials, TZ* __restrict__ rz) {
+    const T a = (T)alpha;
+    double s = 0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        x[i] += a * p[i];
+        const T ri = r[i] - a * q[i];
+        r[i] = ri;
+        s += (double)ri * (double)ri;
+        if (rz) rz[i] = (TZ)ri;
+    }
+    s = blk_sum(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+template <typename T, typename TZ>
+void launch_pcg_update_x_r(T* x, T* r, const T* p, const T* q, double alpha, size_t n, double* partials, int nparts,
+                           TZ* rz_out, hipStream_t s) {
+    hipLaunchKernelGGL((k_pcg_xr<T, TZ>), dim3(nparts), dim3(256), 0, s, x, r, p, q, alpha, n, partials, rz_out);
+    RT_HIP(hipGetLastError());
+}
+
+// p = z + b p
+template <typename T, typename TZ>
+__global__ void k_pcg_p(T* __restrict__ p, const TZ* __restrict__ z, double beta, size_t n) {
+    const T b = (T)beta;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = (T)z[i] + b * p[i];
+}
+template <typename T, typename TZ>
+void launch_pcg_update_p(T* p, const TZ* z, double beta, size_t n, hipStream_t s) {
+    unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL((k_pcg_p<T, TZ>), dim3(nb), dim3(256), 0, s, p, z, beta, n);
+    RT_HIP(hipGetLastError());
+}
+
+// r = b - q; partials of r.r; rz = (TZ) r
+template <typename T, typename TZ>
+__global__ void __launch_bounds__(256) k_sub_conv(const T* __restrict__ b, const T* __restrict__ q, T* __restrict__ r,
+                                                  TZ* __restrict__ rz, size_t n, double* __restrict__ partials) {
+    double s = 0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const T ri = b[i] - q[i];
+        r[i] = ri;
+        if (rz) rz[i] = (TZ)ri;
+        s += (double)ri * (double)ri;
+    }
+    s = blk_sum(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+template <typename T, typename TZ>
+void launch_sub_conv(const T* b, const T* q, T* r, TZ* rz, size_t n, double* partials, int nparts, hipStream_t s) {
+    hipLaunchKernelGGL((k_sub_conv<T, TZ>), dim3(nparts), dim3(256), 0, s, b, q, r, rz, n, partials);
+    RT_HIP(hipGetLastError());
+}
+
+// x += e (TE -> T)
+template <typename T, typename TE>
+__global__ void k_add_conv(T* __restrict__ x, const TE* __restrict__ e, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        x[i] += (T)e[i];
+}
+template <typename T, typename TE>
+void launch_add_conv(T* x, const TE* e, size_t n, hipStream_t s) {
+    unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL((k_add_conv<T, TE>), dim3(nb), dim3(256), 0, s, x, e, n);
+    RT_HIP(hipGetLastError());
+}
+
+// scalar helpers: out = a / b (device scalars), optionally negated
+__global__ void k_ratio(const double* a, const double* b, double* out) { out[0] = a[0] / b[0]; }
+void launch_ratio(const double* a, const double* b, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_ratio, dim3(1), dim3(1), 0, s, a, b, out);
+    RT_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// box (natural, x fastest) <-> octant slot layout
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_box_oct(Geom g, int bx, int by, int bz, const T* __restrict__ src, T* __restrict__ dst, int to_oct) {
+    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (t >= (size_t)bx * by * bz) return;
+    const int x = (int)(t % bx), y = (int)((t / bx) % by), z = (int)(t / ((size_t)bx * by));
+    const int s = oct_p(g, x, y, z);
+    if (to_oct) dst[s] = src[t]; else dst[t] = src[s];
+}
+template <typename T>
+void launch_box_to_oct(const Geom& g, const T* box, T* oct, int node_or_elem, hipStream_t s) {
+    int d = node_or_elem ? 1 : 0;
+    size_t tot = (size_t)(g.n[0] - d) * (g.n[1] - d) * (g.n[2] - d);
+    if (!tot) return;
+    hipLaunchKernelGGL((k_box_oct<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, g.n[0] - d,
+                       g.n[1] - d, g.n[2] - d, box, oct, 1);
+    RT_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_oct_to_box(const Geom& g, const T* oct, T* box, int node_or_elem, hipStream_t s) {
+    int d = node_or_elem ? 1 : 0;
+    size_t tot = (size_t)(g.n[0] - d) * (g.n[1] - d) * (g.n[2] - d);
+    if (!tot) return;
+    hipLaunchKernelGGL((k_box_oct<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, g.n[0] - d,
+                       g.n[1] - d, g.n[2] - d, oct, box, 0);
+    RT_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// instantiations
+// ---------------------------------------------------------------------------
+#define RT_INST_CS(T, TS)                                                                                           \
+    template void launch_coarse_gs<T, TS>(const Geom&, Vec3<T>, Vec3<const T>, const TS*, const uint8_t*, int, bool, \
+                                          hipStream_t);                                                             \
+    template void launch_coarse_residual<T, TS>(const Geom&, Vec3<const T>, Vec3<const T>, Vec3<T>, const TS*,      \
+                                                const uint8_t*, hipStream_t);
+RT_INST_CS(double, double)
+RT_INST_CS(float, float)
+#undef RT_INST_CS
+
+#define RT_INST_T(T)                                                                                                  \
+    template void launch_restrict<T>(const Geom&, const int*, Vec3<const T>, const Geom&, const int*, Vec3<T>,        \
+                                     const uint8_t*, int, hipStream_t);                                               \
+    template void launch_prolong<T>(const Geom&, const int*, Vec3<T>, const uint8_t*, const Geom&, const int*,        \
+                                    Vec3<const T>, int, bool, hipStream_t);                                           \
+    template void launch_coarsest<T>(const double*, int, const int*, int, Vec3<const T>, Vec3<T>, double*,            \
+                                     hipStream_t);                                                                    \
+    template void launch_fill<T>(T*, T, size_t, hipStream_t);                                                         \
+    template void launch_box_to_oct<T>(const Geom&, const T*, T*, int, hipStream_t);                                  \
+    template void launch_oct_to_box<T>(const Geom&, const T*, T*, int, hipStream_t);
+RT_INST_T(double)
+RT_INST_T(float)
+#undef RT_INST_T
+template void launch_fill<uint8_t>(uint8_t*, uint8_t, size_t, hipStream_t);
+template void launch_box_to_oct<uint8_t>(const Geom&, const uint8_t*, uint8_t*, int, hipStream_t);
+template void launch_oct_to_box<uint8_t>(const Geom&, const uint8_t*, uint8_t*, int, hipStream_t);
+template void launch_convert<double, float>(const double*, float*, size_t, hipStream_t);
+template void launch_convert<float, double>(const float*, double*, size_t, hipStream_t);
+template void launch_convert<float, float>(const float*, float*, size_t, hipStream_t);
+template void launch_convert<double, double>(const double*, double*, size_t, hipStream_t);
+template void launch_dot_partials_t<double, double>(const double*, const double*, size_t, double*, int, hipStream_t);
+template void launch_dot_partials_t<double, float>(const double*, const float*, size_t, double*, int, hipStream_t);
+template void launch_dot_partials_t<float, float>(const float*, const float*, size_t, double*, int, hipStream_t);
+#define RT_INST_PCG(T, TZ)                                                                                        \
+    template void launch_pcg_update_x_r<T, TZ>(T*, T*, const T*, const T*, double, size_t, double*, int, TZ*,        \
+                                               hipStream_t);                                                       \
+    template void launch_pcg_update_p<T, TZ>(T*, const TZ*, double, size_t, hipStream_t);                          \
+    template void launch_sub_conv<T, TZ>(const T*, const T*, T*, TZ*, size_t, double*, int, hipStream_t);          \
+    template void launch_add_conv<T, TZ>(T*, const TZ*, size_t, hipStream_t);
+RT_INST_PCG(double, float)
+RT_INST_PCG(double, double)
+RT_INST_PCG(float, float)
+RT_INST_PCG(float, double)
+#undef RT_INST_PCG
+
+}  // namespace rt

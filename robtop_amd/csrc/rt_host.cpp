@@ -1,0 +1,1429 @@
+// Host side of robtop_amd: hierarchy construction, device memory, the V-cycle
+// and MG-PCG drivers, the SIMP steps and the C ABI declared in
+// include/robtop_amd.h. Compiled by hipcc as HIP host code.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <thread>
+
+#include "../../include/robtop_amd.h"
+#include "rt_internal.h"
+
+namespace rt {
+
+thread_local std::string g_err;
+
+// ---------------------------------------------------------------------------
+// device buffer
+// ---------------------------------------------------------------------------
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t b) {
+        if (b == bytes && p) { RT_HIP(hipMemset(p, 0, b)); return; }
+        release();
+        if (b == 0) return;
+        RT_HIP(hipMalloc(&p, b));
+        RT_HIP(hipMemset(p, 0, b));
+        bytes = b;
+    }
+    void ensure(size_t b) { if (bytes < b) alloc(b); }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+// a 3-component vector: one contiguous buffer of 3N
+template <typename T>
+struct V3 {
+    DBuf b;
+    int N = 0;
+    bool ok() const { return b.p != nullptr; }
+    void alloc(int n) { N = n; b.alloc(sizeof(T) * 3 * (size_t)n); }
+    T* c(int k) const { return b.as<T>() + (size_t)k * N; }
+    T* data() const { return b.as<T>(); }
+    Vec3<T> v() const { return Vec3<T>{{c(0), c(1), c(2)}}; }
+    Vec3<const T> cv() const { return Vec3<const T>{{c(0), c(1), c(2)}}; }
+};
+
+template <typename T>
+struct LevelVecs {
+    V3<T> U, F, R;
+    bool ok() const { return U.ok(); }
+    void alloc(int N) { U.alloc(N); F.alloc(N); R.alloc(N); }
+};
+
+struct Level {
+    int layer = 0, scale = 1;
+    int o[3] = {0, 0, 0};
+    int e[3] = {0, 0, 0};
+    Geom g;
+    std::vector<uint8_t> eact, vact;      // box masks
+    std::vector<uint8_t> vflag_box;       // flags in box layout (host copy)
+    size_t n_eact = 0, n_vact = 0;
+    DBuf vflag;
+    LevelVecs<double> v64;
+    LevelVecs<float> v32;
+    DBuf st64, st32, KEc;
+    size_t nn_box() const { return (size_t)g.n[0] * g.n[1] * g.n[2]; }
+    size_t ne_box() const { return (size_t)e[0] * e[1] * e[2]; }
+};
+
+struct Hier {
+    int R = 0, nlayers = 0;
+    std::vector<int> layer_active;
+    std::vector<Level> lv;
+    hipStream_t s = nullptr;
+    double KE[576];
+    double E = 1e6, nu = 0.4;
+    float p = 3.f;
+    DBuf dM;                 // galerkin x4 M_k [64][576]
+    // finest level
+    DBuf rho, rhop, eflag, sens, sens_tmp;
+    std::vector<uint8_t> eflag_box, fixed_box;
+    V3<double> x64, f64;     // displacement and force
+    // PCG work
+    V3<double> r64, p64, q64;
+    V3<float> x32, b32, r32, p32, q32;
+    LevelVecs<double> pc64;  // preconditioner level-0 vectors
+    LevelVecs<float> pc32;
+    DBuf partials, scal, fbuf, stage;
+    int nparts_max = 0;
+    // coarsest
+    DBuf Ainv, rowslot;
+    int n3 = 0;
+    DBuf fixce;
+    int nfixce = 0;
+    int vprec = 64;
+    bool stencil_ready = false, material_set = false;
+    ~Hier() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// KE for an isotropic unit cube (trilinear element, exact 2-point Gauss):
+// a(u,v) = int lam div u div v + mu (grad u : grad v + grad u : grad v^T).
+// ---------------------------------------------------------------------------
+static void element_matrix(double E, double nu, double KE[576]) {
+    const double lam = E * nu / ((1 + nu) * (1 - 2 * nu));
+    const double mu = E / (2 * (1 + nu));
+    const double gp[2] = {0.5 - 0.5 / std::sqrt(3.0), 0.5 + 0.5 / std::sqrt(3.0)};
+    std::fill(KE, KE + 576, 0.0);
+    for (int gz = 0; gz < 2; gz++)
+        for (int gy = 0; gy < 2; gy++)
+            for (int gx = 0; gx < 2; gx++) {
+                const double xq[3] = {gp[gx], gp[gy], gp[gz]};
+                double gr[8][3];
+                for (int i = 0; i < 8; i++) {
+                    const int b[3] = {i & 1, (i >> 1) & 1, i >> 2};
+                    double f[3], d[3];
+                    for (int k = 0; k < 3; k++) {
+                        f[k] = b[k] ? xq[k] : 1 - xq[k];
+                        d[k] = b[k] ? 1.0 : -1.0;
+                    }
+                    gr[i][0] = d[0] * f[1] * f[2];
+                    gr[i][1] = f[0] * d[1] * f[2];
+                    gr[i][2] = f[0] * f[1] * d[2];
+                }
+                for (int i = 0; i < 8; i++)
+                    for (int a = 0; a < 3; a++)
+                        for (int j = 0; j < 8; j++)
+                            for (int b = 0; b < 3; b++) {
+                                double v = lam * gr[i][a] * gr[j][b] + mu * gr[i][b] * gr[j][a];
+                                if (a == b) v += mu * (gr[i][0] * gr[j][0] + gr[i][1] * gr[j][1] + gr[i][2] * gr[j][2]);
+                                KE[(3 * i + a) * 24 + 3 * j + b] += 0.125 * v;
+                            }
+            }
+}
+
+// M_k = P_k^T KE P_k for the 64 children of a x4 coarse element
+static void galerkin4_matrices(const double KE[576], std::vector<double>& M) {
+    M.assign(64 * 576, 0.0);
+    for (int k = 0; k < 64; k++) {
+        const int kx = k & 3, ky = (k >> 2) & 3, kz = k >> 4;
+        double W[8][8];
+        for (int i = 0; i < 8; i++) {
+            const int px = kx + (i & 1), py = ky + ((i >> 1) & 1), pz = kz + (i >> 2);
+            for (int A = 0; A < 8; A++)
+                W[i][A] = ((A & 1) ? px : 4 - px) * ((A & 2) ? py : 4 - py) * ((A & 4) ? pz : 4 - pz) / 64.0;
+        }
+        double T[24][24];   // KE P
+        for (int a = 0; a < 24; a++)
+            for (int B = 0; B < 8; B++)
+                for (int s = 0; s < 3; s++) {
+                    double v = 0;
+                    for (int j = 0; j < 8; j++) v += KE[a * 24 + 3 * j + s] * W[j][B];
+                    T[a][3 * B + s] = v;
+                }
+        for (int A = 0; A < 8; A++)
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 24; c++) {
+                    double v = 0;
+                    for (int i = 0; i < 8; i++) v += W[i][A] * T[3 * i + r][c];
+                    M[(size_t)k * 576 + (3 * A + r) * 24 + c] = v;
+                }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// hierarchy construction
+// ---------------------------------------------------------------------------
+static Hier* build(int R, const uint32_t* bits, size_t nwords) {
+    if (R < 4 || (R & (R - 1))) throw std::invalid_argument("R must be a power of two >= 4");
+    const size_t n0 = (size_t)R * R * R;
+    if (nwords * 32 < n0) throw std::invalid_argument("solid bit array too short for R^3 bits");
+    std::vector<std::vector<uint8_t>> act;
+    {
+        std::vector<uint8_t> a(n0);
+        for (size_t i = 0; i < n0; i++) a[i] = (bits[i >> 5] >> (i & 31)) & 1u;
+        act.push_back(std::move(a));
+    }
+    std::vector<int> counts;
+    auto count = [](const std::vector<uint8_t>& a) {
+        size_t c = 0;
+        for (uint8_t v : a) c += v;
+        return (int)c;
+    };
+    counts.push_back(count(act[0]));
+    if (counts[0] == 0) throw std::invalid_argument("no solid element");
+    int reso = R;
+    while (counts.back() > 400 && reso > 1) {
+        const int fr = reso;
+        reso >>= 1;
+        std::vector<uint8_t> c((size_t)reso * reso * reso, 0);
+        const std::vector<uint8_t>& f = act.back();
+        for (int z = 0; z < fr; z++)
+            for (int y = 0; y < fr; y++) {
+                const uint8_t* row = f.data() + (size_t)fr * (y + (size_t)fr * z);
+                uint8_t* crow = c.data() + (size_t)reso * ((y >> 1) + (size_t)reso * (z >> 1));
+                for (int x = 0; x < fr; x++)
+                    if (row[x]) crow[x >> 1] = 1;
+            }
+        act.push_back(std::move(c));
+        counts.push_back(count(act.back()));
+    }
+    const int nl = (int)act.size();
+    if (nl < 3) throw std::invalid_argument("fewer than three layers: the x4 first coarsening needs layers 0 and 2");
+    const int K = nl - 1, A = 1 << K;
+    int lo[3] = {R, R, R}, hi[3] = {0, 0, 0};
+    for (int z = 0; z < R; z++)
+        for (int y = 0; y < R; y++) {
+            const uint8_t* row = act[0].data() + (size_t)R * (y + (size_t)R * z);
+            int xa = -1, xb = -1;
+            for (int x = 0; x < R; x++)
+                if (row[x]) { if (xa < 0) xa = x; xb = x; }
+            if (xa < 0) continue;
+            lo[0] = std::min(lo[0], xa); hi[0] = std::max(hi[0], xb + 1);
+            lo[1] = std::min(lo[1], y); hi[1] = std::max(hi[1], y + 1);
+            lo[2] = std::min(lo[2], z); hi[2] = std::max(hi[2], z + 1);
+        }
+    std::unique_ptr<Hier> h(new Hier());
+    h->R = R;
+    h->nlayers = nl;
+    h->layer_active = counts;
+    RT_HIP(hipStreamCreate(&h->s));
+    for (int layer = 0; layer < nl; layer++) {
+        if (layer == 1) continue;
+        Level L;
+        L.layer = layer;
+        L.scale = 1 << layer;
+        const int r = R >> layer;
+        for (int k = 0; k < 3; k++) {
+            const int l0 = (lo[k] / A) * A, h0 = std::min(R, ((hi[k] + A - 1) / A) * A);
+            L.o[k] = l0 >> layer;
+            L.e[k] = (h0 - l0) >> layer;
+        }
+        L.g = make_geom(L.e[0] + 1, L.e[1] + 1, L.e[2] + 1);
+        L.eact.assign(L.ne_box(), 0);
+        for (int z = 0; z < L.e[2]; z++)
+            for (int y = 0; y < L.e[1]; y++)
+                for (int x = 0; x < L.e[0]; x++) {
+                    const size_t gi = (size_t)(x + L.o[0]) + (size_t)r * ((size_t)(y + L.o[1]) + (size_t)r * (z + L.o[2]));
+                    if (act[layer][gi]) { L.eact[x + (size_t)L.e[0] * (y + (size_t)L.e[1] * z)] = 1; L.n_eact++; }
+                }
+        const int nx = L.g.n[0], ny = L.g.n[1], nz = L.g.n[2];
+        L.vact.assign(L.nn_box(), 0);
+        L.vflag_box.assign(L.nn_box(), 0);
+        for (int z = 0; z < nz; z++)
+            for (int y = 0; y < ny; y++)
+                for (int x = 0; x < nx; x++) {
+                    int ne = 0;
+                    for (int w = 0; w < 8; w++) {
+                        const int ex = x + (w & 1) - 1, ey = y + ((w >> 1) & 1) - 1, ez = z + (w >> 2) - 1;
+                        if (ex < 0 || ey < 0 || ez < 0 || ex >= L.e[0] || ey >= L.e[1] || ez >= L.e[2]) continue;
+                        ne += L.eact[ex + (size_t)L.e[0] * (ey + (size_t)L.e[1] * ez)];
+                    }
+                    const size_t v = x + (size_t)nx * (y + (size_t)ny * z);
+                    if (ne) {
+                        L.vact[v] = 1;
+                        L.vflag_box[v] = (uint8_t)(NF_ACTIVE | (ne << 4));
+                        L.n_vact++;
+                    }
+                }
+        h->lv.push_back(std::move(L));
+    }
+    return h.release();
+}
+
+// ---------------------------------------------------------------------------
+// transfers between host box arrays and device octant arrays
+// ---------------------------------------------------------------------------
+template <typename T>
+static void upload_box(Hier& H, const Geom& g, int elem, const T* host, T* dev) {
+    const int d = elem ? 1 : 0;
+    const size_t cnt = (size_t)(g.n[0] - d) * (g.n[1] - d) * (g.n[2] - d);
+    H.stage.ensure(cnt * sizeof(T));
+    RT_HIP(hipMemcpyAsync(H.stage.p, host, cnt * sizeof(T), hipMemcpyHostToDevice, H.s));
+    launch_box_to_oct<T>(g, H.stage.as<T>(), dev, elem, H.s);
+    RT_HIP(hipStreamSynchronize(H.s));
+}
+
+template <typename T>
+static void download_box(Hier& H, const Geom& g, int elem, const T* dev, T* host) {
+    const int d = elem ? 1 : 0;
+    const size_t cnt = (size_t)(g.n[0] - d) * (g.n[1] - d) * (g.n[2] - d);
+    H.stage.ensure(cnt * sizeof(T));
+    launch_oct_to_box<T>(g, dev, H.stage.as<T>(), elem, H.s);
+    RT_HIP(hipMemcpyAsync(host, H.stage.p, cnt * sizeof(T), hipMemcpyDeviceToHost, H.s));
+    RT_HIP(hipStreamSynchronize(H.s));
+}
+
+// 3-vector of FP64 box data -> device T vector
+template <typename T>
+static void upload_vec(Hier& H, const Geom& g, const double* box, V3<T>& v) {
+    const size_t nb = (size_t)g.n[0] * g.n[1] * g.n[2];
+    std::vector<T> tmp(nb);
+    for (int c = 0; c < 3; c++) {
+        for (size_t i = 0; i < nb; i++) tmp[i] = (T)box[c * nb + i];
+        upload_box<T>(H, g, 0, tmp.data(), v.c(c));
+    }
+}
+template <typename T>
+static void download_vec(Hier& H, const Geom& g, const V3<T>& v, double* box) {
+    const size_t nb = (size_t)g.n[0] * g.n[1] * g.n[2];
+    std::vector<T> tmp(nb);
+    for (int c = 0; c < 3; c++) {
+        download_box<T>(H, g, 0, v.c(c), tmp.data());
+        for (size_t i = 0; i < nb; i++) box[c * nb + i] = (double)tmp[i];
+    }
+}
+
+static void upload_flags(Hier& H, Level& L) {
+    L.vflag.alloc(L.g.N);
+    upload_box<uint8_t>(H, L.g, 0, L.vflag_box.data(), L.vflag.as<uint8_t>());
+}
+
+static void ensure_partials(Hier& H) {
+    if (H.partials.p) return;
+    int need = 65536;
+    need = std::max(need, fine_apply_partials(H.lv[0].g));
+    for (auto& L : H.lv) need = std::max(need, (int)((L.g.N + 255) / 256));
+    H.nparts_max = need;
+    H.partials.alloc(sizeof(double) * (size_t)need);
+    H.scal.alloc(sizeof(double) * 16);
+}
+
+static void setup_device(Hier& H) {
+    for (auto& L : H.lv) upload_flags(H, L);
+    Level& L0 = H.lv[0];
+    const int N0 = L0.g.N;
+    H.eflag_box.assign(L0.ne_box(), 0);
+    for (size_t i = 0; i < L0.ne_box(); i++) H.eflag_box[i] = L0.eact[i] ? EF_ACTIVE : 0;
+    H.eflag.alloc(N0);
+    upload_box<uint8_t>(H, L0.g, 1, H.eflag_box.data(), H.eflag.as<uint8_t>());
+    H.rho.alloc(sizeof(float) * N0);
+    H.rhop.alloc(sizeof(float) * N0);
+    H.sens.alloc(sizeof(float) * N0);
+    H.sens_tmp.alloc(sizeof(float) * N0);
+    H.x64.alloc(N0);
+    H.f64.alloc(N0);
+    H.fixed_box.assign(L0.nn_box(), 0);
+    for (size_t li = 1; li < H.lv.size(); li++) {
+        Level& L = H.lv[li];
+        L.KEc.alloc(sizeof(double) * 576 * L.ne_box());
+        L.st64.alloc(sizeof(double) * 243 * (size_t)L.g.N);
+        L.st32.alloc(sizeof(float) * 243 * (size_t)L.g.N);
+    }
+    ensure_partials(H);
+}
+
+template <typename T> LevelVecs<T>& vecs(Level& L);
+template <> LevelVecs<double>& vecs<double>(Level& L) { if (!L.v64.ok()) L.v64.alloc(L.g.N); return L.v64; }
+template <> LevelVecs<float>& vecs<float>(Level& L) { if (!L.v32.ok()) L.v32.alloc(L.g.N); return L.v32; }
+template <typename T> const T* stencil(const Level& L);
+template <> const double* stencil<double>(const Level& L) { return L.st64.as<double>(); }
+template <> const float* stencil<float>(const Level& L) { return L.st32.as<float>(); }
+template <typename T> LevelVecs<T>& pcvecs(Hier& H);
+template <> LevelVecs<double>& pcvecs<double>(Hier& H) { if (!H.pc64.ok()) H.pc64.alloc(H.lv[0].g.N); return H.pc64; }
+template <> LevelVecs<float>& pcvecs<float>(Hier& H) { if (!H.pc32.ok()) H.pc32.alloc(H.lv[0].g.N); return H.pc32; }
+
+// ---------------------------------------------------------------------------
+// multigrid operations (level index li into H.lv; the finest level is 0)
+// ---------------------------------------------------------------------------
+template <typename T>
+static void gs(Hier& H, int li, Vec3<T> U, Vec3<const T> F, int n, bool reverse) {
+    Level& L = H.lv[li];
+    for (int it = 0; it < n; it++)
+        for (int cc = 0; cc < 8; cc++) {
+            const int c = reverse ? 7 - cc : cc;
+            if (li == 0)
+                launch_fine_gs<T>(L.g, U, F, H.rhop.as<float>(), L.vflag.as<uint8_t>(), c, reverse, H.s);
+            else
+                launch_coarse_gs<T, T>(L.g, U, F, stencil<T>(L), L.vflag.as<uint8_t>(), c, reverse, H.s);
+        }
+}
+
+template <typename T>
+static void residual(Hier& H, int li, Vec3<const T> U, Vec3<const T> F, Vec3<T> R) {
+    Level& L = H.lv[li];
+    if (li == 0)
+        launch_fine_residual<T>(L.g, U, F, R, H.rhop.as<float>(), L.vflag.as<uint8_t>(), H.s);
+    else
+        launch_coarse_residual<T, T>(L.g, U, F, R, stencil<T>(L), L.vflag.as<uint8_t>(), H.s);
+}
+
+template <typename T>
+static void restrict_to(Hier& H, int li, Vec3<const T> Rf, Vec3<T> Fc) {
+    Level& C = H.lv[li];
+    Level& Fl = H.lv[li - 1];
+    launch_restrict<T>(Fl.g, Fl.o, Rf, C.g, C.o, Fc, C.vflag.as<uint8_t>(), C.scale / Fl.scale, H.s);
+}
+
+template <typename T>
+static void prolong_from(Hier& H, int li, Vec3<T> Uf, Vec3<const T> Uc) {
+    Level& Fl = H.lv[li];
+    Level& C = H.lv[li + 1];
+    launch_prolong<T>(Fl.g, Fl.o, Uf, Fl.vflag.as<uint8_t>(), C.g, C.o, Uc, C.scale / Fl.scale, false, H.s);
+}
+
+template <typename T>
+static void coarsest(Hier& H, Vec3<const T> F, Vec3<T> U) {
+    if (!H.stencil_ready) throw std::logic_error("update_stencil has not been called");
+    launch_coarsest<T>(H.Ainv.as<double>(), H.n3, H.rowslot.as<int>(), 0, F, U, H.fbuf.as<double>(), H.s);
+}
+
+// V(pre, post) with caller-owned finest vectors (Grid.cpp:1068-1113 ordering).
+template <typename T>
+static void vcycle_core(Hier& H, V3<T>& U0, V3<T>& F0, V3<T>& R0, int pre, int post, bool symmetric) {
+    const int n = (int)H.lv.size();
+    auto Uv = [&](int i) { return i == 0 ? U0.v() : vecs<T>(H.lv[i]).U.v(); };
+    auto Fv = [&](int i) { return i == 0 ? F0.cv() : vecs<T>(H.lv[i]).F.cv(); };
+    auto Rv = [&](int i) { return i == 0 ? R0.v() : vecs<T>(H.lv[i]).R.v(); };
+    auto cvt = [](Vec3<T> a) { return Vec3<const T>{{a.c[0], a.c[1], a.c[2]}}; };
+    for (int i = 0; i < n; i++) {
+        if (i > 0) {
+            residual<T>(H, i - 1, cvt(Uv(i - 1)), Fv(i - 1), Rv(i - 1));
+            LevelVecs<T>& lvv = vecs<T>(H.lv[i]);
+            restrict_to<T>(H, i, cvt(Rv(i - 1)), lvv.F.v());
+            launch_fill<T>(lvv.U.data(), T(0), (size_t)3 * lvv.U.N, H.s);
+        }
+        if (i < n - 1) gs<T>(H, i, Uv(i), Fv(i), pre, false);
+        else coarsest<T>(H, Fv(i), Uv(i));
+    }
+    for (int i = n - 2; i >= 0; i--) {
+        prolong_from<T>(H, i, Uv(i), cvt(Uv(i + 1)));
+        gs<T>(H, i, Uv(i), Fv(i), post, symmetric);
+    }
+}
+
+template <typename TA, typename TB>
+static double dot(Hier& H, const TA* a, const TB* b, size_t n) {
+    const int np = 1024;
+    launch_dot_partials_t<TA, TB>(a, b, n, H.partials.as<double>(), np, H.s);
+    launch_sum_partials(H.partials.as<double>(), np, H.scal.as<double>(), H.s);
+    double v;
+    RT_HIP(hipMemcpyAsync(&v, H.scal.p, sizeof(double), hipMemcpyDeviceToHost, H.s));
+    RT_HIP(hipStreamSynchronize(H.s));
+    return v;
+}
+
+static double sum_partials_host(Hier& H, int np) {
+    launch_sum_partials(H.partials.as<double>(), np, H.scal.as<double>(), H.s);
+    double v;
+    RT_HIP(hipMemcpyAsync(&v, H.scal.p, sizeof(double), hipMemcpyDeviceToHost, H.s));
+    RT_HIP(hipStreamSynchronize(H.s));
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Galerkin + coarsest
+// ---------------------------------------------------------------------------
+static void factor_coarsest(Hier& H) {
+    Level& L = H.lv.back();
+    const size_t N = (size_t)L.g.N;
+    std::vector<double> st(243 * N);
+    RT_HIP(hipMemcpyAsync(st.data(), L.st64.p, st.size() * 8, hipMemcpyDeviceToHost, H.s));
+    RT_HIP(hipStreamSynchronize(H.s));
+    const int nx = L.g.n[0], ny = L.g.n[1], nz = L.g.n[2];
+    std::vector<int> slots, rowof(N, -1);
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++)
+                if (L.vact[x + (size_t)nx * (y + (size_t)ny * z)]) {
+                    const int s = oct_p(L.g, x, y, z);
+                    rowof[s] = (int)slots.size();
+                    slots.push_back(s);
+                }
+    const int nr = (int)slots.size(), n = 3 * nr;
+    std::vector<double> K((size_t)n * n, 0.0);
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                const int s = oct_p(L.g, x, y, z);
+                const int r = rowof[s];
+                if (r < 0) continue;
+                for (int nb = 0; nb < 27; nb++) {
+                    const int jx = x + nb % 3 - 1, jy = y + nb / 3 % 3 - 1, jz = z + nb / 9 - 1;
+                    if (jx < 0 || jy < 0 || jz < 0 || jx >= nx || jy >= ny || jz >= nz) continue;
+                    const int c = rowof[oct_p(L.g, jx, jy, jz)];
+                    if (c < 0) continue;
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++)
+                            K[(size_t)(3 * r + a) * n + 3 * c + b] = st[((size_t)nb * 9 + 3 * a + b) * N + s];
+                }
+            }
+    // Cholesky (lower, in place), rows in parallel
+    const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    auto par = [&](int lo, int hi, auto fn) {
+        if (hi - lo < 64 || nth == 1) { for (int i = lo; i < hi; i++) fn(i); return; }
+        std::vector<std::thread> ts;
+        const int chunk = (hi - lo + nth - 1) / nth;
+        for (int t = 0; t < nth; t++) {
+            const int a = lo + t * chunk, b = std::min(hi, a + chunk);
+            if (a >= b) break;
+            ts.emplace_back([=]() { for (int i = a; i < b; i++) fn(i); });
+        }
+        for (auto& th : ts) th.join();
+    };
+    for (int j = 0; j < n; j++) {
+        double d = K[(size_t)j * n + j];
+        for (int k = 0; k < j; k++) d -= K[(size_t)j * n + k] * K[(size_t)j * n + k];
+        if (!(d > 0)) throw std::domain_error("coarsest system is not positive definite (no supports?)");
+        d = std::sqrt(d);
+        K[(size_t)j * n + j] = d;
+        par(j + 1, n, [&](int i) {
+            double s = K[(size_t)i * n + j];
+            const double* ri = &K[(size_t)i * n];
+            const double* rj = &K[(size_t)j * n];
+            for (int k = 0; k < j; k++) s -= ri[k] * rj[k];
+            K[(size_t)i * n + j] = s / d;
+        });
+    }
+    // inverse: columns of (L L^T)^-1
+    std::vector<double> Ainv((size_t)n * n);
+    par(0, n, [&](int col) {
+        std::vector<double> b(n, 0.0);
+        b[col] = 1.0;
+        for (int i = col; i < n; i++) {
+            double s = b[i];
+            const double* ri = &K[(size_t)i * n];
+            for (int k = col; k < i; k++) s -= ri[k] * b[k];
+            b[i] = s / ri[i];
+        }
+        for (int i = n - 1; i >= 0; i--) {
+            double s = b[i];
+            for (int k = i + 1; k < n; k++) s -= K[(size_t)k * n + i] * b[k];
+            b[i] = s / K[(size_t)i * n + i];
+        }
+        for (int i = 0; i < n; i++) Ainv[(size_t)i * n + col] = b[i];
+    });
+    H.n3 = n;
+    H.Ainv.ensure(sizeof(double) * (size_t)n * n);
+    H.rowslot.ensure(sizeof(int) * (size_t)nr);
+    H.fbuf.ensure(sizeof(double) * (size_t)n);
+    RT_HIP(hipMemcpyAsync(H.Ainv.p, Ainv.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice, H.s));
+    RT_HIP(hipMemcpyAsync(H.rowslot.p, slots.data(), sizeof(int) * (size_t)nr, hipMemcpyHostToDevice, H.s));
+    RT_HIP(hipStreamSynchronize(H.s));
+}
+
+static void update_stencil(Hier& H) {
+    if (!H.material_set) throw std::logic_error("rt_set_material has not been called");
+    Level& L0 = H.lv[0];
+    launch_rhop(L0.g, H.rho.as<float>(), H.eflag.as<uint8_t>(), H.p, H.rhop.as<float>(), H.s);
+    Level& L2 = H.lv[1];
+    launch_galerkin4(L0.g, L2.e, H.rhop.as<float>(), H.dM.as<double>(), L2.KEc.as<double>(), H.s);
+    launch_galerkin4_fix(L0.g, L2.e, H.rhop.as<float>(), L0.vflag.as<uint8_t>(), H.eflag.as<uint8_t>(),
+                         H.fixce.as<int>(), H.nfixce, L2.KEc.as<double>(), H.s);
+    for (size_t li = 2; li < H.lv.size(); li++)
+        launch_galerkin2(H.lv[li - 1].KEc.as<double>(), H.lv[li - 1].e, H.lv[li].KEc.as<double>(), H.lv[li].e, H.s);
+    for (size_t li = 1; li < H.lv.size(); li++) {
+        Level& L = H.lv[li];
+        launch_assemble<double>(L.g, L.KEc.as<double>(), L.e, L.vflag.as<uint8_t>(), L.st64.as<double>(), H.s);
+        launch_convert<double, float>(L.st64.as<double>(), L.st32.as<float>(), (size_t)243 * L.g.N, H.s);
+    }
+    factor_coarsest(H);
+    H.stencil_ready = true;
+}
+
+// ---------------------------------------------------------------------------
+// MG-PCG. TV: Krylov vectors, TC: V-cycle. Dots in FP64.
+// ---------------------------------------------------------------------------
+template <typename TV> struct KV;
+template <> struct KV<double> {
+    static V3<double>& x(Hier& H) { return H.x64; }
+    static V3<double>& b(Hier& H) { return H.f64; }
+    static V3<double>& r(Hier& H) { return H.r64; }
+    static V3<double>& p(Hier& H) { return H.p64; }
+    static V3<double>& q(Hier& H) { return H.q64; }
+};
+template <> struct KV<float> {
+    static V3<float>& x(Hier& H) { return H.x32; }
+    static V3<float>& b(Hier& H) { return H.b32; }
+    static V3<float>& r(Hier& H) { return H.r32; }
+    static V3<float>& p(Hier& H) { return H.p32; }
+    static V3<float>& q(Hier& H) { return H.q32; }
+};
+
+template <typename TV>
+static double apply_dot(Hier& H, V3<TV>& X, V3<TV>& Y) {
+    Level& L0 = H.lv[0];
+    int np = 0;
+    launch_fine_apply<TV>(L0.g, X.cv(), Y.v(), H.rhop.as<float>(), L0.vflag.as<uint8_t>(), H.partials.as<double>(), &np,
+                          H.s);
+    return sum_partials_host(H, np);
+}
+
+static double true_rel_residual(Hier& H) {
+    Level& L0 = H.lv[0];
+    if (!H.r64.ok()) H.r64.alloc(L0.g.N);
+    if (!H.q64.ok()) H.q64.alloc(L0.g.N);
+    const size_t n = (size_t)3 * L0.g.N;
+    launch_fine_apply<double>(L0.g, H.x64.cv(), H.q64.v(), H.rhop.as<float>(), L0.vflag.as<uint8_t>(), nullptr,
+                              nullptr, H.s);
+    launch_sub_conv<double, double>(H.f64.data(), H.q64.data(), H.r64.data(), (double*)nullptr, n,
+                                    H.partials.as<double>(), 1024, H.s);
+    const double rr = sum_partials_host(H, 1024);
+    const double ff = dot<double, double>(H, H.f64.data(), H.f64.data(), n);
+    return ff > 0 ? std::sqrt(rr / ff) : 0.0;
+}
+
+template <typename TV, typename TC>
+static void pcg(Hier& H, double tol, int maxit, int pre, int post, rt_solve_stats* st) {
+    if (!H.stencil_ready) throw std::logic_error("update_stencil has not been called");
+    auto t0 = std::chrono::steady_clock::now();
+    Level& L0 = H.lv[0];
+    const int N = L0.g.N;
+    const size_t n = (size_t)3 * N;
+    V3<TV>& x = KV<TV>::x(H);
+    V3<TV>& b = KV<TV>::b(H);
+    V3<TV>& r = KV<TV>::r(H);
+    V3<TV>& p = KV<TV>::p(H);
+    V3<TV>& q = KV<TV>::q(H);
+    if (!x.ok()) x.alloc(N);
+    if (!b.ok()) b.alloc(N);
+    if (!r.ok()) r.alloc(N);
+    if (!p.ok()) p.alloc(N);
+    if (!q.ok()) q.alloc(N);
+    if ((void*)x.data() != (void*)H.x64.data()) {
+        launch_convert<double, TV>(H.x64.data(), x.data(), n, H.s);
+        launch_convert<double, TV>(H.f64.data(), b.data(), n, H.s);
+    }
+    LevelVecs<TC>& pc = pcvecs<TC>(H);
+    const int np = 1024;
+    // r = b - A x
+    launch_fine_apply<TV>(L0.g, x.cv(), q.v(), H.rhop.as<float>(), L0.vflag.as<uint8_t>(), nullptr, nullptr, H.s);
+    launch_sub_conv<TV, TC>(b.data(), q.data(), r.data(), pc.F.data(), n, H.partials.as<double>(), np, H.s);
+    double rr = sum_partials_host(H, np);
+    const double bb = dot<TV, TV>(H, b.data(), b.data(), n);
+    const double nb = std::sqrt(bb);
+    int it = 0;
+    double rel = nb > 0 ? std::sqrt(rr) / nb : 0.0;
+    if (nb > 0 && rel > tol) {
+        launch_fill<TC>(pc.U.data(), TC(0), n, H.s);
+        vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true);
+        double rz = dot<TV, TC>(H, r.data(), pc.U.data(), n);
+        launch_convert<TC, TV>(pc.U.data(), p.data(), n, H.s);
+        for (it = 1; it <= maxit; it++) {
+            const double pq = apply_dot<TV>(H, p, q);
+            if (!(pq > 0)) throw std::domain_error("PCG breakdown: p.Ap <= 0");
+            const double alpha = rz / pq;
+            launch_pcg_update_x_r<TV, TC>(x.data(), r.data(), p.data(), q.data(), alpha, n, H.partials.as<double>(), np,
+                                          pc.F.data(), H.s);
+            rr = sum_partials_host(H, np);
+            rel = std::sqrt(rr) / nb;
+            if (rel <= tol) break;
+            launch_fill<TC>(pc.U.data(), TC(0), n, H.s);
+            vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true);
+            const double rz_new = dot<TV, TC>(H, r.data(), pc.U.data(), n);
+            const double beta = rz_new / rz;
+            rz = rz_new;
+            launch_pcg_update_p<TV, TC>(p.data(), pc.U.data(), beta, n, H.s);
+        }
+        if (it > maxit) it = maxit;
+    }
+    if ((void*)x.data() != (void*)H.x64.data()) launch_convert<TV, double>(x.data(), H.x64.data(), n, H.s);
+    RT_HIP(hipStreamSynchronize(H.s));
+    const double tr = true_rel_residual(H);
+    auto t1 = std::chrono::steady_clock::now();
+    if (st) {
+        st->iterations = it;
+        st->rel_res = tr;
+        st->rel_res_pcg = rel;
+        st->seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+}
+
+// reference-semantics V-cycle on the finest displacement / force
+template <typename T>
+static double vcycle_on_solution(Hier& H, int pre, int post) {
+    if (!H.stencil_ready) throw std::logic_error("update_stencil has not been called");
+    Level& L0 = H.lv[0];
+    const size_t n = (size_t)3 * L0.g.N;
+    LevelVecs<T>& v = vecs<T>(L0);
+    launch_convert<double, T>(H.x64.data(), v.U.data(), n, H.s);
+    launch_convert<double, T>(H.f64.data(), v.F.data(), n, H.s);
+    vcycle_core<T>(H, v.U, v.F, v.R, pre, post, false);
+    residual<T>(H, 0, v.U.cv(), v.F.cv(), v.R.v());
+    launch_convert<T, double>(v.U.data(), H.x64.data(), n, H.s);
+    const double rr = dot<T, T>(H, v.R.data(), v.R.data(), n);
+    const double ff = dot<T, T>(H, v.F.data(), v.F.data(), n);
+    return std::sqrt(rr) / std::sqrt(ff);
+}
+
+// ---------------------------------------------------------------------------
+// boundary conditions
+// ---------------------------------------------------------------------------
+static void set_support(Hier& H, const uint8_t* fixed_box) {
+    Level& L0 = H.lv[0];
+    const int nx = L0.g.n[0], ny = L0.g.n[1], nz = L0.g.n[2];
+    const int ex = L0.e[0], ey = L0.e[1], ez = L0.e[2];
+    for (size_t v = 0; v < L0.nn_box(); v++) {
+        H.fixed_box[v] = (fixed_box[v] && L0.vact[v]) ? 1 : 0;
+        uint8_t f = L0.vflag_box[v] & ~(NF_FIXED | NF_NEARFIX);
+        if (H.fixed_box[v]) f |= NF_FIXED;
+        L0.vflag_box[v] = f;
+    }
+    // mark the other 26 neighbours of every fixed node
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                if (!H.fixed_box[x + (size_t)nx * (y + (size_t)ny * z)]) continue;
+                for (int nb = 0; nb < 27; nb++) {
+                    if (nb == 13) continue;
+                    const int jx = x + nb % 3 - 1, jy = y + nb / 3 % 3 - 1, jz = z + nb / 9 - 1;
+                    if (jx < 0 || jy < 0 || jz < 0 || jx >= nx || jy >= ny || jz >= nz) continue;
+                    const size_t j = jx + (size_t)nx * (jy + (size_t)ny * jz);
+                    if (L0.vact[j]) L0.vflag_box[j] |= NF_NEARFIX;
+                }
+            }
+    // elements touching a support, and the x4 coarse elements holding them
+    Level& L2 = H.lv[1];
+    std::vector<uint8_t> cmark(L2.ne_box(), 0);
+    for (int z = 0; z < ez; z++)
+        for (int y = 0; y < ey; y++)
+            for (int x = 0; x < ex; x++) {
+                const size_t e = x + (size_t)ex * (y + (size_t)ey * z);
+                uint8_t f = H.eflag_box[e] & ~EF_TOUCHFIX;
+                if (L0.eact[e]) {
+                    bool t = false;
+                    for (int k = 0; k < 8 && !t; k++)
+                        t = H.fixed_box[(x + (k & 1)) + (size_t)nx * ((y + ((k >> 1) & 1)) + (size_t)ny * (z + (k >> 2)))];
+                    if (t) {
+                        f |= EF_TOUCHFIX;
+                        cmark[(x >> 2) + (size_t)L2.e[0] * ((y >> 2) + (size_t)L2.e[1] * (z >> 2))] = 1;
+                    }
+                }
+                H.eflag_box[e] = f;
+            }
+    std::vector<int> ce;
+    for (size_t i = 0; i < cmark.size(); i++)
+        if (cmark[i]) ce.push_back((int)i);
+    H.nfixce = (int)ce.size();
+    H.fixce.release();
+    if (!ce.empty()) {
+        H.fixce.alloc(sizeof(int) * ce.size());
+        RT_HIP(hipMemcpy(H.fixce.p, ce.data(), sizeof(int) * ce.size(), hipMemcpyHostToDevice));
+    }
+    upload_box<uint8_t>(H, L0.g, 0, L0.vflag_box.data(), L0.vflag.as<uint8_t>());
+    upload_box<uint8_t>(H, L0.g, 1, H.eflag_box.data(), H.eflag.as<uint8_t>());
+    H.stencil_ready = false;
+}
+
+static void zero_fixed_force(Hier& H) {
+    // F is zero at supports (load nodes exclude supports, Grid.cpp:1333-1345)
+    Level& L0 = H.lv[0];
+    const size_t nb = L0.nn_box();
+    std::vector<double> box(3 * nb);
+    download_vec<double>(H, L0.g, H.f64, box.data());
+    bool any = false;
+    for (size_t v = 0; v < nb; v++)
+        if (H.fixed_box[v])
+            for (int c = 0; c < 3; c++)
+                if (box[c * nb + v] != 0) { box[c * nb + v] = 0; any = true; }
+    if (any) upload_vec<double>(H, L0.g, box.data(), H.f64);
+}
+
+// active (lexicographic) <-> box
+template <typename T>
+static void expand_active(const std::vector<uint8_t>& mask, const T* act, T* box, int ncomp) {
+    const size_t nb = mask.size();
+    size_t na = 0;
+    for (uint8_t m : mask) na += m;
+    for (int c = 0; c < ncomp; c++) {
+        size_t k = 0;
+        for (size_t i = 0; i < nb; i++) box[c * nb + i] = mask[i] ? act[c * na + k++] : T(0);
+    }
+}
+template <typename T>
+static void compress_active(const std::vector<uint8_t>& mask, const T* box, T* act, int ncomp) {
+    const size_t nb = mask.size();
+    size_t na = 0;
+    for (uint8_t m : mask) na += m;
+    for (int c = 0; c < ncomp; c++) {
+        size_t k = 0;
+        for (size_t i = 0; i < nb; i++)
+            if (mask[i]) act[c * na + k++] = box[c * nb + i];
+    }
+}
+
+}  // namespace rt
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace rt;
+
+struct rt_hier {
+    Hier* h;
+};
+
+#define RT_TRY(...)                                                                  \
+    try {                                                                            \
+        __VA_ARGS__;                                                                 \
+        return RT_OK;                                                                \
+    } catch (const std::invalid_argument& e) {                                       \
+        g_err = e.what();                                                            \
+        return RT_EARG;                                                              \
+    } catch (const std::domain_error& e) {                                           \
+        g_err = e.what();                                                            \
+        return RT_ENUM;                                                              \
+    } catch (const std::logic_error& e) {                                            \
+        g_err = e.what();                                                            \
+        return RT_ESTATE;                                                            \
+    } catch (const std::runtime_error& e) {                                          \
+        g_err = e.what();                                                            \
+        return std::string(e.what()).rfind("HIP", 0) == 0 ? RT_EHIP : RT_EINTERNAL;  \
+    } catch (const std::exception& e) {                                              \
+        g_err = e.what();                                                            \
+        return RT_EINTERNAL;                                                         \
+    }
+
+static Hier& H_(rt_hier* h) {
+    if (!h || !h->h) throw std::invalid_argument("null handle");
+    return *h->h;
+}
+static Hier& H_(const rt_hier* h) {
+    if (!h || !h->h) throw std::invalid_argument("null handle");
+    return *h->h;
+}
+static Level& LV(Hier& H, int level) {
+    if (level < 0 || level >= (int)H.lv.size()) throw std::invalid_argument("level out of range");
+    return H.lv[level];
+}
+static void need(const void* p) {
+    if (!p) throw std::invalid_argument("null pointer argument");
+}
+
+extern "C" {
+
+const char* rt_last_error(void) { return g_err.c_str(); }
+const char* rt_version(void) { return "robtop_amd 0.1 (gfx950)"; }
+
+int rt_create_from_voxels(int R, const uint32_t* solid_bits, size_t nwords, rt_hier** out) {
+    RT_TRY({
+        need(solid_bits); need(out);
+        *out = nullptr;
+        std::unique_ptr<Hier> h(build(R, solid_bits, nwords));
+        setup_device(*h);
+        *out = new rt_hier{h.release()};
+    })
+}
+
+int rt_destroy(rt_hier* h) {
+    RT_TRY({
+        if (h) { delete h->h; delete h; }
+    })
+}
+
+int rt_num_levels(const rt_hier* h) {
+    try { return (int)H_(h).lv.size(); } catch (const std::exception& e) { g_err = e.what(); return RT_EARG; }
+}
+int rt_num_layers(const rt_hier* h) {
+    try { return H_(h).nlayers; } catch (const std::exception& e) { g_err = e.what(); return RT_EARG; }
+}
+int rt_layer_active_elements(const rt_hier* h, int layer) {
+    try {
+        Hier& H = H_(h);
+        if (layer < 0 || layer >= H.nlayers) throw std::invalid_argument("layer out of range");
+        return H.layer_active[layer];
+    } catch (const std::exception& e) { g_err = e.what(); return RT_EARG; }
+}
+
+int rt_level_info(const rt_hier* h, int level, int* info) {
+    RT_TRY({
+        need(info);
+        Hier& H = H_(h);
+        Level& L = LV(H, level);
+        info[0] = L.layer; info[1] = L.scale;
+        for (int k = 0; k < 3; k++) { info[2 + k] = L.o[k]; info[5 + k] = L.e[k]; info[8 + k] = L.g.n[k]; }
+    })
+}
+
+int rt_level_masks(const rt_hier* h, int level, uint8_t* eact_box, uint8_t* vact_box) {
+    RT_TRY({
+        Hier& H = H_(h);
+        Level& L = LV(H, level);
+        if (eact_box) std::memcpy(eact_box, L.eact.data(), L.eact.size());
+        if (vact_box) std::memcpy(vact_box, L.vact.data(), L.vact.size());
+    })
+}
+
+int rt_num_active(const rt_hier* h, int level, size_t* ne, size_t* nn) {
+    RT_TRY({
+        Hier& H = H_(h);
+        Level& L = LV(H, level);
+        if (ne) *ne = L.n_eact;
+        if (nn) *nn = L.n_vact;
+    })
+}
+
+int rt_set_material(rt_hier* h, double youngs, double poisson, float power_penalty) {
+    RT_TRY({
+        Hier& H = H_(h);
+        if (!(youngs > 0) || !(poisson > -1 && poisson < 0.5)) throw std::invalid_argument("invalid material");
+        H.E = youngs; H.nu = poisson; H.p = power_penalty;
+        element_matrix(youngs, poisson, H.KE);
+        set_ke_constants(H.KE, H.s);
+        set_ke_galerkin(H.KE, H.s);
+        set_ke_simp(H.KE, H.s);
+        std::vector<double> M;
+        galerkin4_matrices(H.KE, M);
+        H.dM.alloc(sizeof(double) * M.size());
+        RT_HIP(hipMemcpy(H.dM.p, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice));
+        launch_rhop(H.lv[0].g, H.rho.as<float>(), H.eflag.as<uint8_t>(), H.p, H.rhop.as<float>(), H.s);
+        RT_HIP(hipStreamSynchronize(H.s));
+        H.material_set = true;
+        H.stencil_ready = false;
+    })
+}
+
+int rt_get_template_matrix(const rt_hier* h, double* ke) {
+    RT_TRY({ need(ke); std::memcpy(ke, H_(h).KE, sizeof(double) * 576); })
+}
+
+int rt_set_support_box(rt_hier* h, const uint8_t* fixed_box) {
+    RT_TRY({
+        need(fixed_box);
+        Hier& H = H_(h);
+        set_support(H, fixed_box);
+        zero_fixed_force(H);
+    })
+}
+
+int rt_set_support_active(rt_hier* h, const uint8_t* fixed_active) {
+    RT_TRY({
+        need(fixed_active);
+        Hier& H = H_(h);
+        std::vector<uint8_t> box(H.lv[0].nn_box());
+        expand_active<uint8_t>(H.lv[0].vact, fixed_active, box.data(), 1);
+        set_support(H, box.data());
+        zero_fixed_force(H);
+    })
+}
+
+int rt_set_force_box(rt_hier* h, const double* f_box) {
+    RT_TRY({
+        need(f_box);
+        Hier& H = H_(h);
+        Level& L0 = H.lv[0];
+        const size_t nb = L0.nn_box();
+        std::vector<double> box(f_box, f_box + 3 * nb);
+        for (size_t v = 0; v < nb; v++)
+            if (!L0.vact[v] || H.fixed_box[v])
+                for (int c = 0; c < 3; c++) box[c * nb + v] = 0;
+        upload_vec<double>(H, L0.g, box.data(), H.f64);
+    })
+}
+
+int rt_set_force_active(rt_hier* h, const double* f_active) {
+    RT_TRY({
+        need(f_active);
+        Hier& H = H_(h);
+        std::vector<double> box(3 * H.lv[0].nn_box());
+        expand_active<double>(H.lv[0].vact, f_active, box.data(), 3);
+        for (size_t v = 0; v < H.lv[0].nn_box(); v++)
+            if (H.fixed_box[v])
+                for (int c = 0; c < 3; c++) box[c * H.lv[0].nn_box() + v] = 0;
+        upload_vec<double>(H, H.lv[0].g, box.data(), H.f64);
+    })
+}
+
+int rt_get_force_box(const rt_hier* h, double* f_box) {
+    RT_TRY({ need(f_box); Hier& H = H_(h); download_vec<double>(H, H.lv[0].g, H.f64, f_box); })
+}
+
+int rt_set_shell_box(rt_hier* h, const uint8_t* shell_box) {
+    RT_TRY({
+        need(shell_box);
+        Hier& H = H_(h);
+        for (size_t e = 0; e < H.eflag_box.size(); e++) {
+            uint8_t f = H.eflag_box[e] & ~EF_SHELL;
+            if (shell_box[e] && H.lv[0].eact[e]) f |= EF_SHELL;
+            H.eflag_box[e] = f;
+        }
+        upload_box<uint8_t>(H, H.lv[0].g, 1, H.eflag_box.data(), H.eflag.as<uint8_t>());
+    })
+}
+
+int rt_init_density(rt_hier* h, float rho) {
+    RT_TRY({
+        Hier& H = H_(h);
+        std::vector<float> box(H.lv[0].ne_box());
+        for (size_t e = 0; e < box.size(); e++) box[e] = H.lv[0].eact[e] ? rho : 0.f;
+        upload_box<float>(H, H.lv[0].g, 1, box.data(), H.rho.as<float>());
+        launch_rhop(H.lv[0].g, H.rho.as<float>(), H.eflag.as<uint8_t>(), H.p, H.rhop.as<float>(), H.s);
+        RT_HIP(hipStreamSynchronize(H.s));
+        H.stencil_ready = false;
+    })
+}
+
+int rt_set_density_box(rt_hier* h, const float* rho_box) {
+    RT_TRY({
+        need(rho_box);
+        Hier& H = H_(h);
+        std::vector<float> box(rho_box, rho_box + H.lv[0].ne_box());
+        for (size_t e = 0; e < box.size(); e++) if (!H.lv[0].eact[e]) box[e] = 0.f;
+        upload_box<float>(H, H.lv[0].g, 1, box.data(), H.rho.as<float>());
+        launch_rhop(H.lv[0].g, H.rho.as<float>(), H.eflag.as<uint8_t>(), H.p, H.rhop.as<float>(), H.s);
+        RT_HIP(hipStreamSynchronize(H.s));
+        H.stencil_ready = false;
+    })
+}
+
+int rt_get_density_box(const rt_hier* h, float* rho_box) {
+    RT_TRY({ need(rho_box); Hier& H = H_(h); download_box<float>(H, H.lv[0].g, 1, H.rho.as<float>(), rho_box); })
+}
+
+int rt_set_density_active(rt_hier* h, const float* rho_active) {
+    RT_TRY({
+        need(rho_active);
+        Hier& H = H_(h);
+        std::vector<float> box(H.lv[0].ne_box());
+        expand_active<float>(H.lv[0].eact, rho_active, box.data(), 1);
+        upload_box<float>(H, H.lv[0].g, 1, box.data(), H.rho.as<float>());
+        launch_rhop(H.lv[0].g, H.rho.as<float>(), H.eflag.as<uint8_t>(), H.p, H.rhop.as<float>(), H.s);
+        RT_HIP(hipStreamSynchronize(H.s));
+        H.stencil_ready = false;
+    })
+}
+
+int rt_get_density_active(const rt_hier* h, float* rho_active) {
+    RT_TRY({
+        need(rho_active);
+        Hier& H = H_(h);
+        std::vector<float> box(H.lv[0].ne_box());
+        download_box<float>(H, H.lv[0].g, 1, H.rho.as<float>(), box.data());
+        compress_active<float>(H.lv[0].eact, box.data(), rho_active, 1);
+    })
+}
+
+int rt_get_rhop_box(const rt_hier* h, float* out) {
+    RT_TRY({ need(out); Hier& H = H_(h); download_box<float>(H, H.lv[0].g, 1, H.rhop.as<float>(), out); })
+}
+
+int rt_update_stencil(rt_hier* h) { RT_TRY({ update_stencil(H_(h)); }) }
+
+int rt_get_stencil_box(const rt_hier* h, int level, double* st_box) {
+    RT_TRY({
+        need(st_box);
+        Hier& H = H_(h);
+        if (level < 1) throw std::invalid_argument("the finest level has no stored stencil");
+        Level& L = LV(H, level);
+        const size_t nb = L.nn_box();
+        for (int k = 0; k < 243; k++)
+            download_box<double>(H, L.g, 0, L.st64.as<double>() + (size_t)k * L.g.N, st_box + k * nb);
+    })
+}
+
+int rt_set_vcycle_precision(rt_hier* h, int bits) {
+    RT_TRY({
+        if (bits != 32 && bits != 64) throw std::invalid_argument("precision must be 32 or 64");
+        H_(h).vprec = bits;
+    })
+}
+
+}  // extern "C"
+template <typename T>
+static V3<T>& level_vec(Hier& H, int level, int which) {
+    if (which < 0 || which > 2) throw std::invalid_argument("which must be RT_VEC_U/F/R");
+    LevelVecs<T>& v = vecs<T>(LV(H, level));
+    return which == 0 ? v.U : which == 1 ? v.F : v.R;
+}
+extern "C" {
+
+int rt_set_level_vec(rt_hier* h, int level, int which, const double* v_box) {
+    RT_TRY({
+        need(v_box);
+        Hier& H = H_(h);
+        if (H.vprec == 64) upload_vec<double>(H, LV(H, level).g, v_box, level_vec<double>(H, level, which));
+        else upload_vec<float>(H, LV(H, level).g, v_box, level_vec<float>(H, level, which));
+    })
+}
+
+int rt_get_level_vec(const rt_hier* h, int level, int which, double* v_box) {
+    RT_TRY({
+        need(v_box);
+        Hier& H = H_(h);
+        if (H.vprec == 64) download_vec<double>(H, LV(H, level).g, level_vec<double>(H, level, which), v_box);
+        else download_vec<float>(H, LV(H, level).g, level_vec<float>(H, level, which), v_box);
+    })
+}
+
+}  // extern "C"
+template <typename T>
+static void do_gs(Hier& H, int level, int n, int reverse) {
+    LevelVecs<T>& v = vecs<T>(LV(H, level));
+    if (level > 0 && !H.stencil_ready) throw std::logic_error("update_stencil has not been called");
+    gs<T>(H, level, v.U.v(), v.F.cv(), n, reverse != 0);
+    RT_HIP(hipStreamSynchronize(H.s));
+}
+extern "C" {
+
+int rt_gs_relax(rt_hier* h, int level, int nsweeps, int reverse) {
+    RT_TRY({
+        Hier& H = H_(h);
+        if (H.vprec == 64) do_gs<double>(H, level, nsweeps, reverse); else do_gs<float>(H, level, nsweeps, reverse);
+    })
+}
+
+}  // extern "C"
+template <typename T>
+static void do_res(Hier& H, int level) {
+    LevelVecs<T>& v = vecs<T>(LV(H, level));
+    if (level > 0 && !H.stencil_ready) throw std::logic_error("update_stencil has not been called");
+    residual<T>(H, level, v.U.cv(), v.F.cv(), v.R.v());
+    RT_HIP(hipStreamSynchronize(H.s));
+}
+extern "C" {
+
+int rt_update_residual(rt_hier* h, int level) {
+    RT_TRY({ Hier& H = H_(h); if (H.vprec == 64) do_res<double>(H, level); else do_res<float>(H, level); })
+}
+
+}  // extern "C"
+template <typename T>
+static void do_restrict(Hier& H, int level) {
+    if (level < 1) throw std::invalid_argument("restrict_residual targets a coarse level");
+    LevelVecs<T>& f = vecs<T>(LV(H, level - 1));
+    LevelVecs<T>& c = vecs<T>(LV(H, level));
+    restrict_to<T>(H, level, f.R.cv(), c.F.v());
+    RT_HIP(hipStreamSynchronize(H.s));
+}
+extern "C" {
+
+int rt_restrict_residual(rt_hier* h, int level) {
+    RT_TRY({ Hier& H = H_(h); if (H.vprec == 64) do_restrict<double>(H, level); else do_restrict<float>(H, level); })
+}
+
+}  // extern "C"
+template <typename T>
+static void do_prolong(Hier& H, int level) {
+    if (level < 0 || level + 1 >= (int)H.lv.size()) throw std::invalid_argument("prolongate_correction needs a coarser level");
+    LevelVecs<T>& f = vecs<T>(LV(H, level));
+    LevelVecs<T>& c = vecs<T>(LV(H, level + 1));
+    prolong_from<T>(H, level, f.U.v(), c.U.cv());
+    RT_HIP(hipStreamSynchronize(H.s));
+}
+extern "C" {
+
+int rt_prolongate_correction(rt_hier* h, int level) {
+    RT_TRY({ Hier& H = H_(h); if (H.vprec == 64) do_prolong<double>(H, level); else do_prolong<float>(H, level); })
+}
+
+int rt_reset_displacement(rt_hier* h, int level) {
+    RT_TRY({
+        Hier& H = H_(h);
+        Level& L = LV(H, level);
+        if (H.vprec == 64) launch_fill<double>(vecs<double>(L).U.data(), 0.0, (size_t)3 * L.g.N, H.s);
+        else launch_fill<float>(vecs<float>(L).U.data(), 0.f, (size_t)3 * L.g.N, H.s);
+        RT_HIP(hipStreamSynchronize(H.s));
+    })
+}
+
+int rt_solve_coarsest(rt_hier* h) {
+    RT_TRY({
+        Hier& H = H_(h);
+        Level& L = H.lv.back();
+        if (H.vprec == 64) coarsest<double>(H, vecs<double>(L).F.cv(), vecs<double>(L).U.v());
+        else coarsest<float>(H, vecs<float>(L).F.cv(), vecs<float>(L).U.v());
+        RT_HIP(hipStreamSynchronize(H.s));
+    })
+}
+
+int rt_relative_residual(rt_hier* h, double* rel) {
+    RT_TRY({
+        need(rel);
+        Hier& H = H_(h);
+        Level& L = H.lv[0];
+        const size_t n = (size_t)3 * L.g.N;
+        if (H.vprec == 64) {
+            LevelVecs<double>& v = vecs<double>(L);
+            *rel = std::sqrt(dot<double, double>(H, v.R.data(), v.R.data(), n) / dot<double, double>(H, v.F.data(), v.F.data(), n));
+        } else {
+            LevelVecs<float>& v = vecs<float>(L);
+            *rel = std::sqrt(dot<float, float>(H, v.R.data(), v.R.data(), n) / dot<float, float>(H, v.F.data(), v.F.data(), n));
+        }
+    })
+}
+
+int rt_vcycle(rt_hier* h, int pre, int post, double* rel_res) {
+    RT_TRY({
+        Hier& H = H_(h);
+        double r = H.vprec == 64 ? vcycle_on_solution<double>(H, pre, post) : vcycle_on_solution<float>(H, pre, post);
+        if (rel_res) *rel_res = r;
+    })
+}
+
+int rt_solve(rt_hier* h, double tol, int maxit, int mode, int pre, int post, rt_solve_stats* stats) {
+    RT_TRY({
+        Hier& H = H_(h);
+        if (pre < 0 || post < 0 || maxit < 0) throw std::invalid_argument("negative sweep or iteration count");
+        if (mode == RT_SOLVE_FP64) pcg<double, double>(H, tol, maxit, pre, post, stats);
+        else if (mode == RT_SOLVE_FP32) pcg<float, float>(H, tol, maxit, pre, post, stats);
+        else if (mode == RT_SOLVE_MIXED) pcg<double, float>(H, tol, maxit, pre, post, stats);
+        else throw std::invalid_argument("unknown solver mode");
+    })
+}
+
+int rt_set_displacement_box(rt_hier* h, const double* u_box) {
+    RT_TRY({
+        need(u_box);
+        Hier& H = H_(h);
+        Level& L0 = H.lv[0];
+        const size_t nb = L0.nn_box();
+        std::vector<double> box(u_box, u_box + 3 * nb);
+        for (size_t v = 0; v < nb; v++)
+            if (!L0.vact[v])
+                for (int c = 0; c < 3; c++) box[c * nb + v] = 0;
+        upload_vec<double>(H, L0.g, box.data(), H.x64);
+    })
+}
+
+int rt_get_displacement_box(const rt_hier* h, double* u_box) {
+    RT_TRY({ need(u_box); Hier& H = H_(h); download_vec<double>(H, H.lv[0].g, H.x64, u_box); })
+}
+
+int rt_get_displacement_active(const rt_hier* h, double* u_active) {
+    RT_TRY({
+        need(u_active);
+        Hier& H = H_(h);
+        std::vector<double> box(3 * H.lv[0].nn_box());
+        download_vec<double>(H, H.lv[0].g, H.x64, box.data());
+        compress_active<double>(H.lv[0].vact, box.data(), u_active, 3);
+    })
+}
+
+int rt_compliance(rt_hier* h, double* c) {
+    RT_TRY({
+        need(c);
+        Hier& H = H_(h);
+        *c = dot<double, double>(H, H.f64.data(), H.x64.data(), (size_t)3 * H.lv[0].g.N);
+    })
+}
+
+int rt_compute_sensitivity(rt_hier* h) {
+    RT_TRY({
+        Hier& H = H_(h);
+        launch_fill<float>(H.sens.as<float>(), 0.f, (size_t)H.lv[0].g.N, H.s);
+        launch_sensitivity<double>(H.lv[0].g, H.x64.cv(), H.rho.as<float>(), H.eflag.as<uint8_t>(), H.p,
+                                   H.sens.as<float>(), H.s);
+        RT_HIP(hipStreamSynchronize(H.s));
+    })
+}
+
+int rt_filter_sensitivity(rt_hier* h, double radius) {
+    RT_TRY({
+        Hier& H = H_(h);
+        if (!(radius > 0)) throw std::invalid_argument("filter radius must be positive");
+        RT_HIP(hipMemcpyAsync(H.sens_tmp.p, H.sens.p, sizeof(float) * H.lv[0].g.N, hipMemcpyDeviceToDevice, H.s));
+        launch_filter(H.lv[0].g, H.sens_tmp.as<float>(), H.eflag.as<uint8_t>(), (float)radius, H.sens.as<float>(), H.s);
+        RT_HIP(hipStreamSynchronize(H.s));
+    })
+}
+
+int rt_get_sensitivity_box(const rt_hier* h, float* g_box) {
+    RT_TRY({ need(g_box); Hier& H = H_(h); download_box<float>(H, H.lv[0].g, 1, H.sens.as<float>(), g_box); })
+}
+
+int rt_set_sensitivity_box(rt_hier* h, const float* g_box) {
+    RT_TRY({ need(g_box); Hier& H = H_(h); upload_box<float>(H, H.lv[0].g, 1, g_box, H.sens.as<float>()); })
+}
+
+int rt_update_densities(rt_hier* h, float vgoal, float step, float damp, float rho_min, float* multiplier, int* trials) {
+    RT_TRY({
+        Hier& H = H_(h);
+        Level& L0 = H.lv[0];
+        const int np = 1024;
+        launch_maxabs(L0.g, H.sens.as<float>(), H.eflag.as<uint8_t>(), H.partials.as<double>(), np, H.s);
+        std::vector<double> pm(np);
+        RT_HIP(hipMemcpyAsync(pm.data(), H.partials.p, sizeof(double) * np, hipMemcpyDeviceToHost, H.s));
+        RT_HIP(hipStreamSynchronize(H.s));
+        float gmax = 0.f;
+        for (double v : pm) gmax = std::max(gmax, (float)v);
+        const double nact = (double)L0.n_eact;
+        float lo = 0.f, up = gmax, thr = (lo + up) / 2;
+        float V = 2.f;
+        int itn = 0, ntr = 0;
+        do {   // bisection of optimization.cu:186-219
+            thr = (lo + up) / 2;
+            launch_oc_trial(L0.g, H.rho.as<float>(), H.sens.as<float>(), H.eflag.as<uint8_t>(), thr, step, damp,
+                            rho_min, nullptr, H.partials.as<double>(), np, H.s);
+            V = (float)(sum_partials_host(H, np) / nact);
+            ntr++;
+            if (V > vgoal) lo = thr;
+            else if (V < vgoal) up = thr;
+        } while (std::fabs(V - vgoal) > 1e-4 && itn++ < 30);
+        launch_oc_trial(L0.g, H.rho.as<float>(), H.sens.as<float>(), H.eflag.as<uint8_t>(), thr, step, damp, rho_min,
+                        H.rho.as<float>(), H.partials.as<double>(), np, H.s);
+        launch_rhop(L0.g, H.rho.as<float>(), H.eflag.as<uint8_t>(), H.p, H.rhop.as<float>(), H.s);
+        RT_HIP(hipStreamSynchronize(H.s));
+        H.stencil_ready = false;
+        if (multiplier) *multiplier = thr;
+        if (trials) *trials = ntr;
+    })
+}
+
+int rt_volume(rt_hier* h, double* v) {
+    RT_TRY({
+        need(v);
+        Hier& H = H_(h);
+        launch_sum_active(H.lv[0].g, H.rho.as<float>(), H.eflag.as<uint8_t>(), H.partials.as<double>(), 1024, H.s);
+        *v = sum_partials_host(H, 1024) / (double)H.lv[0].n_eact;
+    })
+}
+
+int rt_voxelize_mesh(const float* coords, int nverts, const int* faces, int nfaces, int prefer_reso, uint32_t* bits,
+                     size_t maxwords, int* out_reso, float* out_box) {
+    RT_TRY({
+        need(coords); need(faces); need(bits); need(out_reso); need(out_box);
+        if (nverts <= 0 || nfaces <= 0 || prefer_reso <= 0) throw std::invalid_argument("empty mesh or bad resolution");
+        float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+        for (int i = 0; i < nverts; i++)
+            for (int j = 0; j < 3; j++) { lo[j] = std::min(lo[j], coords[3 * i + j]); hi[j] = std::max(hi[j], coords[3 * i + j]); }
+        unsigned padd = 1;
+        for (int i = 0; i < 32; i++) { if ((unsigned)prefer_reso < padd) break; padd <<= 1; }
+        const float len[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+        const float maxlen = std::max(std::max(len[0], len[1]), len[2]);
+        float bmin[3], bmax[3], unit[3];
+        for (int j = 0; j < 3; j++) {   // padd2SatifyResolution (voxelize_main.cpp:312-340), voxinfo (util.h:57-68)
+            const float pl = maxlen * (float)padd / (float)prefer_reso;
+            bmin[j] = lo[j] - (pl - len[j]) / 2;
+            bmax[j] = hi[j] + (pl - len[j]) / 2;
+            unit[j] = (bmax[j] - bmin[j]) / float(padd);
+        }
+        const int R = (int)padd;
+        const size_t nw = ((size_t)R * R * R + 31) / 32;
+        if (maxwords < nw) throw std::invalid_argument("bit buffer too small for R^3 bits");
+        std::vector<float> tris((size_t)9 * nfaces);
+        for (int t = 0; t < nfaces; t++)
+            for (int k = 0; k < 3; k++) {
+                const int vi = faces[3 * t + k];
+                if (vi < 0 || vi >= nverts) throw std::invalid_argument("face index out of range");
+                for (int j = 0; j < 3; j++) tris[(size_t)9 * t + 3 * k + j] = coords[3 * vi + j];
+            }
+        hipStream_t s;
+        RT_HIP(hipStreamCreate(&s));
+        DBuf dt, db;
+        dt.alloc(sizeof(float) * tris.size());
+        db.alloc(sizeof(uint32_t) * nw);
+        RT_HIP(hipMemcpyAsync(dt.p, tris.data(), sizeof(float) * tris.size(), hipMemcpyHostToDevice, s));
+        launch_voxelize(dt.as<float>(), nfaces, bmin, unit, R, db.as<uint32_t>(), s);
+        RT_HIP(hipMemcpyAsync(bits, db.p, sizeof(uint32_t) * nw, hipMemcpyDeviceToHost, s));
+        RT_HIP(hipStreamSynchronize(s));
+        RT_HIP(hipStreamDestroy(s));
+        for (int j = 0; j < 3; j++) { out_reso[j] = R; out_box[j] = bmin[j]; out_box[3 + j] = bmax[j]; }
+    })
+}
+
+int rt_synchronize(rt_hier* h) { RT_TRY({ RT_HIP(hipStreamSynchronize(H_(h).s)); }) }
+
+}  // extern "C"
+template <typename T>
+static double time_vcycle(Hier& H, int pre, int post, int n) {
+    if (!H.stencil_ready) throw std::logic_error("update_stencil has not been called");
+    LevelVecs<T>& pc = pcvecs<T>(H);
+    const size_t nn = (size_t)3 * H.lv[0].g.N;
+    launch_convert<double, T>(H.f64.data(), pc.F.data(), nn, H.s);
+    hipEvent_t a, b;
+    RT_HIP(hipEventCreate(&a));
+    RT_HIP(hipEventCreate(&b));
+    RT_HIP(hipEventRecord(a, H.s));
+    for (int i = 0; i < n; i++) {
+        launch_fill<T>(pc.U.data(), T(0), nn, H.s);
+        vcycle_core<T>(H, pc.U, pc.F, pc.R, pre, post, true);
+    }
+    RT_HIP(hipEventRecord(b, H.s));
+    RT_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    RT_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms / std::max(1, n);
+}
+extern "C" {
+
+int rt_time_vcycle(rt_hier* h, int bits, int pre, int post, int n, double* ms) {
+    RT_TRY({
+        need(ms);
+        Hier& H = H_(h);
+        *ms = bits == 64 ? time_vcycle<double>(H, pre, post, n) : time_vcycle<float>(H, pre, post, n);
+    })
+}
+
+}  // extern "C"
+template <typename T>
+static double time_sweep(Hier& H, int n) {
+    LevelVecs<T>& pc = pcvecs<T>(H);
+    const size_t nn = (size_t)3 * H.lv[0].g.N;
+    launch_convert<double, T>(H.f64.data(), pc.F.data(), nn, H.s);
+    hipEvent_t a, b;
+    RT_HIP(hipEventCreate(&a));
+    RT_HIP(hipEventCreate(&b));
+    RT_HIP(hipEventRecord(a, H.s));
+    gs<T>(H, 0, pc.U.v(), pc.F.cv(), n, false);
+    RT_HIP(hipEventRecord(b, H.s));
+    RT_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    RT_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms / std::max(1, n);
+}
+extern "C" {
+
+int rt_time_fine_sweep(rt_hier* h, int bits, int n, double* ms) {
+    RT_TRY({
+        need(ms);
+        Hier& H = H_(h);
+        *ms = bits == 64 ? time_sweep<double>(H, n) : time_sweep<float>(H, n);
+    })
+}
+
+}  // extern "C"

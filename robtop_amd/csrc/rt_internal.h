@@ -1,0 +1,153 @@
+// Internal declarations shared by the HIP kernels and the host library.
+//
+// Layout ("octant" layout). Every level stores nodes, and the finest level
+// stores elements, on a box padded by two ghost layers below and at least one
+// above, split into 8 colour blocks by coordinate parity:
+//   q = p + 2 (padded coords), colour(q) = (qx&1) | (qy&1)<<1 | (qz&1)<<2,
+//   index(q) = colour*cs + ((qz>>1)*hy + (qy>>1))*hx + (qx>>1).
+// The box origin is a multiple of the coarsest spacing, so parity in the box is
+// the absolute parity the reference colours by (Grid.cu:579). A colour pass of
+// the Gauss-Seidel smoother therefore reads and writes contiguous runs, and each
+// of its 27 neighbour streams is a contiguous run of another colour block.
+// Ghost slots and absent nodes/elements hold zeros and are never written.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+#include <stdexcept>
+
+namespace rt {
+
+enum NodeFlag : uint8_t {
+    NF_ACTIVE = 1,      // node exists (some incident element is solid)
+    NF_FIXED = 2,       // support node (all three components fixed)
+    NF_NEARFIX = 4,     // some other node of the 27-neighbourhood is fixed
+    NF_LOAD = 8,        // load node (information only)
+    // bits 4..7: number of solid incident elements (1..8)
+};
+enum ElemFlag : uint8_t {
+    EF_ACTIVE = 1,
+    EF_SHELL = 2,       // density forced to 1 by the OC update (optimization.cu:159)
+    EF_TOUCHFIX = 4,    // some vertex is fixed
+};
+
+struct Geom {
+    int n[3];           // interior node box
+    int h[3];           // padded half dims, h = (n + 5) / 2
+    int cs;             // colour stride h0*h1*h2
+    int N;              // 8*cs slots
+};
+
+inline Geom make_geom(int nx, int ny, int nz) {
+    Geom g;
+    g.n[0] = nx; g.n[1] = ny; g.n[2] = nz;
+    for (int k = 0; k < 3; k++) g.h[k] = (g.n[k] + 5) / 2;
+    long long cs = (long long)g.h[0] * g.h[1] * g.h[2];
+    if (cs * 8 >= (1ll << 31)) throw std::runtime_error("level too large for 32-bit slot indices");
+    g.cs = (int)cs;
+    g.N = 8 * g.cs;
+    return g;
+}
+
+// padded coords -> slot
+__host__ __device__ inline int oct_index(const Geom& g, int qx, int qy, int qz) {
+    int c = (qx & 1) | ((qy & 1) << 1) | ((qz & 1) << 2);
+    return c * g.cs + ((qz >> 1) * g.h[1] + (qy >> 1)) * g.h[0] + (qx >> 1);
+}
+// interior node/element coords -> slot
+__host__ __device__ inline int oct_p(const Geom& g, int x, int y, int z) { return oct_index(g, x + 2, y + 2, z + 2); }
+
+template <typename T> struct Vec3 { T* c[3]; };
+
+// ---------------------------------------------------------------------------
+// kernel launchers (defined in the .hip files); all launch on stream `s`
+// ---------------------------------------------------------------------------
+// k_fine.hip
+void set_ke_constants(const double* ke576, hipStream_t s);
+template <typename T>
+void launch_fine_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* rhop, const uint8_t* vflag, int colour,
+                    bool reverse, hipStream_t s);
+template <typename T>
+void launch_fine_residual(const Geom& g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R, const float* rhop,
+                          const uint8_t* vflag, hipStream_t s);
+template <typename T>
+void launch_fine_apply(const Geom& g, Vec3<const T> X, Vec3<T> Y, const float* rhop, const uint8_t* vflag,
+                       double* partials, int* nparts, hipStream_t s);
+int fine_apply_partials(const Geom& g);
+
+// k_coarse.hip
+template <typename T, typename TS>
+void launch_coarse_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const TS* st, const uint8_t* vflag, int colour,
+                      bool reverse, hipStream_t s);
+template <typename T, typename TS>
+void launch_coarse_residual(const Geom& g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R, const TS* st,
+                            const uint8_t* vflag, hipStream_t s);
+template <typename T>
+void launch_restrict(const Geom& gf, const int of[3], Vec3<const T> Rf, const Geom& gc, const int oc[3], Vec3<T> Fc,
+                     const uint8_t* vflag_c, int ratio, hipStream_t s);
+template <typename T>
+void launch_prolong(const Geom& gf, const int of[3], Vec3<T> Uf, const uint8_t* vflag_f, const Geom& gc,
+                    const int oc[3], Vec3<const T> Uc, int ratio, bool skip_fixed, hipStream_t s);
+template <typename T>
+void launch_coarsest(const double* Ainv, int n3, const int* rowslot, int unused, Vec3<const T> F, Vec3<T> U,
+                     double* fbuf, hipStream_t s);
+template <typename TA, typename TB>
+void launch_convert(const TA* a, TB* b, size_t n, hipStream_t s);
+template <typename T>
+void launch_fill(T* a, T v, size_t n, hipStream_t s);
+void launch_dot_partials(const double* a, const double* b, size_t n, double* partials, int nparts, hipStream_t s);
+template <typename TA, typename TB>
+void launch_dot_partials_t(const TA* a, const TB* b, size_t n, double* partials, int nparts, hipStream_t s);
+void launch_sum_partials(const double* partials, int nparts, double* out, hipStream_t s);
+template <typename T, typename TZ>
+void launch_pcg_update_x_r(T* x, T* r, const T* p, const T* q, double alpha, size_t n, double* partials, int nparts,
+                           TZ* rz_out, hipStream_t s);
+template <typename T, typename TZ>
+void launch_pcg_update_p(T* p, const TZ* z, double beta, size_t n, hipStream_t s);
+template <typename T, typename TZ>
+void launch_sub_conv(const T* b, const T* q, T* r, TZ* rz, size_t n, double* partials, int nparts, hipStream_t s);
+template <typename T, typename TE>
+void launch_add_conv(T* x, const TE* e, size_t n, hipStream_t s);
+void launch_ratio(const double* a, const double* b, double* out, hipStream_t s);
+template <typename T>
+void launch_box_to_oct(const Geom& g, const T* box, T* oct, int node_or_elem, hipStream_t s);
+template <typename T>
+void launch_oct_to_box(const Geom& g, const T* oct, T* box, int node_or_elem, hipStream_t s);
+
+// k_galerkin.hip
+void set_ke_galerkin(const double* ke576, hipStream_t s);
+void launch_galerkin4(const Geom& gf, const int ec[3], const float* rhop, const double* M, double* KEc, hipStream_t s);
+void launch_galerkin4_fix(const Geom& gf, const int ec[3], const float* rhop, const uint8_t* vflag,
+                          const uint8_t* eflag, const int* ce, int nce, double* KEc, hipStream_t s);
+void launch_galerkin2(const double* KEf, const int ef[3], double* KEp, const int ep[3], hipStream_t s);
+template <typename TS>
+void launch_assemble(const Geom& g, const double* KEc, const int e[3], const uint8_t* vflag, TS* st, hipStream_t s);
+
+// k_simp.hip
+void set_ke_simp(const double* ke576, hipStream_t s);
+void launch_rhop(const Geom& g, const float* rho, const uint8_t* eflag, float p, float* rhop, hipStream_t s);
+template <typename T>
+void launch_sensitivity(const Geom& g, Vec3<const T> U, const float* rho, const uint8_t* eflag, float p, float* sens,
+                        hipStream_t s);
+void launch_filter(const Geom& g, const float* gin, const uint8_t* eflag, float R, float* gout, hipStream_t s);
+void launch_oc_trial(const Geom& g, const float* rho, const float* sens, const uint8_t* eflag, float thr, float step,
+                     float damp, float rhomin, float* rho_out, double* partials, int nparts, hipStream_t s);
+void launch_maxabs(const Geom& g, const float* a, const uint8_t* eflag, double* partials, int nparts, hipStream_t s);
+void launch_sum_active(const Geom& g, const float* a, const uint8_t* eflag, double* partials, int nparts,
+                       hipStream_t s);
+
+// k_voxel.hip
+void launch_voxelize(const float* tris, int ntris, const float bmin[3], const float unit[3], int R, uint32_t* bits,
+                     hipStream_t s);
+
+#define RT_HIP(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " __FILE__ \
+                                     ":" + std::to_string(__LINE__));                                  \
+    } while (0)
+
+}  // namespace rt

@@ -1,0 +1,304 @@
+"""Host-side mirror of the reference's grid interface (grid/Grid.h).
+
+``HierarchyGrid`` and ``Grid`` keep the reference's method names and argument
+meaning; every call goes through the C ABI of librobtop_amd.so to the HIP
+kernels. Arrays exchanged with the host are numpy arrays in the level's box
+layout (x fastest): nodes ``(nz, ny, nx)``-ordered flat vectors, 3-vectors as
+``[3, nnodes]``.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, ptr, as_c, lib
+
+
+class Grid:
+    """One stored level of the hierarchy (reference class grid::Grid, Grid.h:172-532)."""
+
+    def __init__(self, hier, index):
+        self._h = hier
+        self.index = index
+        info = np.zeros(11, np.int32)
+        check(lib().rt_level_info(hier.handle, index, ptr(info)))
+        self._layer = int(info[0])
+        self.scale = int(info[1])
+        self.origin = info[2:5].copy()
+        self.elements = info[5:8].copy()
+        self.nodes = info[8:11].copy()
+
+    # geometry ---------------------------------------------------------------
+    @property
+    def n_nodes_box(self):
+        return int(np.prod(self.nodes))
+
+    @property
+    def n_elements_box(self):
+        return int(np.prod(self.elements))
+
+    def masks(self):
+        e = np.zeros(self.n_elements_box, np.uint8)
+        v = np.zeros(self.n_nodes_box, np.uint8)
+        check(lib().rt_level_masks(self._h.handle, self.index, ptr(e), ptr(v)))
+        return e.astype(bool), v.astype(bool)
+
+    def is_finest(self):
+        return self.index == 0
+
+    def is_coarsest(self):
+        return self.index == self._h.n_levels - 1
+
+    # per-level vectors (V-cycle precision) --------------------------------------
+    def _set(self, which, v):
+        a = as_c(v, np.float64).reshape(3, self.n_nodes_box)
+        check(lib().rt_set_level_vec(self._h.handle, self.index, which, ptr(a)))
+
+    def _get(self, which):
+        a = np.zeros((3, self.n_nodes_box))
+        check(lib().rt_get_level_vec(self._h.handle, self.index, which, ptr(a)))
+        return a
+
+    def getDisplacement(self):
+        return self._get(_lib.RT_VEC_U)
+
+    def setDisplacement(self, u):
+        self._set(_lib.RT_VEC_U, u)
+
+    def getForce(self):
+        return self._get(_lib.RT_VEC_F)
+
+    def setForce(self, f):
+        self._set(_lib.RT_VEC_F, f)
+
+    def getResidual(self):
+        return self._get(_lib.RT_VEC_R)
+
+    def setResidual(self, r):
+        self._set(_lib.RT_VEC_R, r)
+
+    def getStencil(self):
+        st = np.zeros((27, 9, self.n_nodes_box))
+        check(lib().rt_get_stencil_box(self._h.handle, self.index, ptr(st)))
+        return st
+
+    # operators (Grid.h:357-437) ------------------------------------------------
+    def gs_relax(self, n_times=1, reverse=False):
+        check(lib().rt_gs_relax(self._h.handle, self.index, n_times, int(reverse)))
+
+    def update_residual(self):
+        check(lib().rt_update_residual(self._h.handle, self.index))
+
+    def restrict_residual(self):
+        check(lib().rt_restrict_residual(self._h.handle, self.index))
+
+    def prolongate_correction(self):
+        check(lib().rt_prolongate_correction(self._h.handle, self.index))
+
+    def reset_displacement(self):
+        check(lib().rt_reset_displacement(self._h.handle, self.index))
+
+    def solve_fem_host(self):
+        if not self.is_coarsest():
+            raise ValueError("solve_fem_host runs on the coarsest level")
+        check(lib().rt_solve_coarsest(self._h.handle))
+
+    def relative_residual(self):
+        if self.index != 0:
+            raise ValueError("relative_residual is reported for the finest level")
+        r = ctypes.c_double()
+        check(lib().rt_relative_residual(self._h.handle, ctypes.byref(r)))
+        return r.value
+
+    # finest-level SIMP quantities ------------------------------------------------
+    def compliance(self):
+        return self._h.compliance()
+
+    def filterSensitivity(self, radius):
+        self._h.filter_sensitivity(radius)
+
+
+class HierarchyGrid:
+    """Reference class grid::HierarchyGrid (Grid.h:534-692) over one GPU."""
+
+    def __init__(self, handle):
+        self.handle = handle
+        self.n_levels = check(lib().rt_num_levels(handle))
+        self._grids = [Grid(self, i) for i in range(self.n_levels)]
+
+    # construction ------------------------------------------------------------------
+    @classmethod
+    def genFromMesh(cls, solid_bits, reso):
+        """HierarchyGrid::genFromMesh(solid_bits, reso) (Grid.cpp:455-712):
+        LSB-first solid bits of a cube of side ``reso``."""
+        R = int(reso if np.isscalar(reso) else reso[0])
+        bits = as_c(solid_bits, np.uint32)
+        h = ctypes.c_void_p()
+        check(lib().rt_create_from_voxels(R, ptr(bits), bits.size, ctypes.byref(h)))
+        return cls(h)
+
+    from_voxels = genFromMesh
+
+    def close(self):
+        if self.handle:
+            lib().rt_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __getitem__(self, i):
+        return self._grids[i]
+
+    def __len__(self):
+        return self.n_levels
+
+    def n_grid(self):
+        return self.n_levels
+
+    @property
+    def n_layers(self):
+        return check(lib().rt_num_layers(self.handle))
+
+    def layer_active_elements(self, layer):
+        return check(lib().rt_layer_active_elements(self.handle, layer))
+
+    def num_active(self, level=0):
+        ne, nn = ctypes.c_size_t(), ctypes.c_size_t()
+        check(lib().rt_num_active(self.handle, level, ctypes.byref(ne), ctypes.byref(nn)))
+        return ne.value, nn.value
+
+    # material / boundary ---------------------------------------------------------------
+    def set_material(self, youngs=1e6, poisson=0.4, power_penalty=3.0):
+        check(lib().rt_set_material(self.handle, youngs, poisson, power_penalty))
+
+    def template_matrix(self):
+        ke = np.zeros(576)
+        check(lib().rt_get_template_matrix(self.handle, ptr(ke)))
+        return ke.reshape(24, 24)
+
+    def set_support(self, fixed_box):
+        a = as_c(fixed_box, np.uint8).reshape(-1)
+        check(lib().rt_set_support_box(self.handle, ptr(a)))
+
+    def set_force(self, f_box):
+        a = as_c(f_box, np.float64).reshape(3, -1)
+        check(lib().rt_set_force_box(self.handle, ptr(a)))
+
+    def get_force(self):
+        a = np.zeros((3, self[0].n_nodes_box))
+        check(lib().rt_get_force_box(self.handle, ptr(a)))
+        return a
+
+    def set_shell(self, shell_box):
+        a = as_c(shell_box, np.uint8).reshape(-1)
+        check(lib().rt_set_shell_box(self.handle, ptr(a)))
+
+    # densities ---------------------------------------------------------------------------
+    def init_rho(self, rho):
+        check(lib().rt_init_density(self.handle, float(rho)))
+
+    def set_density(self, rho_box):
+        a = as_c(rho_box, np.float32).reshape(-1)
+        check(lib().rt_set_density_box(self.handle, ptr(a)))
+
+    def get_density(self):
+        a = np.zeros(self[0].n_elements_box, np.float32)
+        check(lib().rt_get_density_box(self.handle, ptr(a)))
+        return a
+
+    def get_rhop(self):
+        a = np.zeros(self[0].n_elements_box, np.float32)
+        check(lib().rt_get_rhop_box(self.handle, ptr(a)))
+        return a
+
+    # multigrid ------------------------------------------------------------------------------
+    def update_stencil(self):
+        check(lib().rt_update_stencil(self.handle))
+
+    def set_precision(self, bits):
+        check(lib().rt_set_vcycle_precision(self.handle, bits))
+
+    def v_cycle(self, pre_relax=1, post_relax=1):
+        r = ctypes.c_double()
+        check(lib().rt_vcycle(self.handle, pre_relax, post_relax, ctypes.byref(r)))
+        return r.value
+
+    def solve(self, tol=1e-6, maxit=200, mode=_lib.RT_SOLVE_MIXED, pre=1, post=1):
+        st = _lib.SolveStats()
+        check(lib().rt_solve(self.handle, tol, maxit, mode, pre, post, ctypes.byref(st)))
+        return st
+
+    def set_displacement(self, u_box):
+        a = as_c(u_box, np.float64).reshape(3, -1)
+        check(lib().rt_set_displacement_box(self.handle, ptr(a)))
+
+    def get_displacement(self):
+        a = np.zeros((3, self[0].n_nodes_box))
+        check(lib().rt_get_displacement_box(self.handle, ptr(a)))
+        return a
+
+    # SIMP step pieces ------------------------------------------------------------------------
+    def compliance(self):
+        c = ctypes.c_double()
+        check(lib().rt_compliance(self.handle, ctypes.byref(c)))
+        return c.value
+
+    def compute_sensitivity(self):
+        check(lib().rt_compute_sensitivity(self.handle))
+
+    def filter_sensitivity(self, radius):
+        check(lib().rt_filter_sensitivity(self.handle, float(radius)))
+
+    def get_sensitivity(self):
+        a = np.zeros(self[0].n_elements_box, np.float32)
+        check(lib().rt_get_sensitivity_box(self.handle, ptr(a)))
+        return a
+
+    def set_sensitivity(self, g_box):
+        a = as_c(g_box, np.float32).reshape(-1)
+        check(lib().rt_set_sensitivity_box(self.handle, ptr(a)))
+
+    def update_densities(self, vgoal, step, damp, rho_min):
+        thr = ctypes.c_float()
+        ntr = ctypes.c_int()
+        check(lib().rt_update_densities(self.handle, vgoal, step, damp, rho_min, ctypes.byref(thr), ctypes.byref(ntr)))
+        return thr.value, ntr.value
+
+    def volume(self):
+        v = ctypes.c_double()
+        check(lib().rt_volume(self.handle, ctypes.byref(v)))
+        return v.value
+
+    def synchronize(self):
+        check(lib().rt_synchronize(self.handle))
+
+    def time_vcycle(self, bits=32, pre=1, post=1, n=10):
+        ms = ctypes.c_double()
+        check(lib().rt_time_vcycle(self.handle, bits, pre, post, n, ctypes.byref(ms)))
+        return ms.value
+
+    def time_fine_sweep(self, bits=32, n=10):
+        ms = ctypes.c_double()
+        check(lib().rt_time_fine_sweep(self.handle, bits, n, ctypes.byref(ms)))
+        return ms.value
+
+
+def voxelize_mesh(coords, faces, prefer_reso):
+    """voxelize_mesh (voxelizer/voxelize_main.cpp:344-404) on the GPU.
+    Returns (LSB-first bits, R, box[2, 3])."""
+    coords = as_c(coords, np.float32).reshape(-1)
+    faces = as_c(faces, np.int32).reshape(-1)
+    R = 1
+    while R <= prefer_reso:
+        R <<= 1
+    nw = (R ** 3 + 31) // 32
+    bits = np.zeros(nw, np.uint32)
+    reso = np.zeros(3, np.int32)
+    box = np.zeros(6, np.float32)
+    check(lib().rt_voxelize_mesh(ptr(coords), coords.size // 3, ptr(faces), faces.size // 3, int(prefer_reso),
+                                 ptr(bits), nw, ptr(reso), ptr(box)))
+    return bits, int(reso[0]), box.reshape(2, 3)

@@ -1,0 +1,126 @@
+"""SIMP loop around the GPU solve, with the reference's driver names
+(optimization/optimization.h, utils/snippet.h, utils/test_utils.cpp)."""
+import math
+import time
+from dataclasses import dataclass
+
+from ._lib import RT_SOLVE_MIXED
+
+
+@dataclass
+class Parameter:
+    """struct Parameter (optimization.h:11-22); defaults of config_parser.cpp:4-46.
+    ``filter_radius`` is an int in the reference (optimization.h:17)."""
+    damp_ratio: float = 0.5
+    design_step: float = 0.03
+    power_penalty: float = 3.0
+    volume_ratio: float = 0.3
+    volume_decrease: float = 0.02
+    filter_radius: int = 2
+    min_rho: float = 1e-3
+    gridreso: int = 200
+    youngs_modulu: float = 1e6
+    poisson_ratio: float = 0.4
+
+
+class converge_criteria:
+    """SIMP stop test (snippet.h:156-213): relative change of the sum of the
+    last two values against the two before, and of each constraint; stop once
+    the test has passed more than ``max_counter`` times net."""
+
+    def __init__(self, n_constrain, max_counter, stop_thres):
+        self._n = n_constrain
+        self._max = max_counter
+        self._thres = stop_thres
+        self._stop = 0
+        self._itn = 0
+        self._old = [math.nan] * 20
+        self._oldc = [[math.nan] * 20 for _ in range(n_constrain)]
+
+    def update(self, value, constrain):
+        k = self._itn % 20
+        self._old[k] = value
+        for i in range(self._n):
+            self._oldc[i][k] = constrain[i]
+
+        def at(a, d):
+            return a[(self._itn - d) % 20]
+
+        def fdiv(a, b):   # IEEE division as in the C++ original
+            if b != 0 or math.isnan(b):
+                return a / b if not math.isnan(b) else math.nan
+            return math.nan if (a == 0 or math.isnan(a)) else math.copysign(math.inf, a)
+
+        fch = fdiv(abs(at(self._old, 0) + at(self._old, 1) - at(self._old, 2) - at(self._old, 3)),
+                   at(self._old, 2) + at(self._old, 3))
+        converged = fch < self._thres
+        for i in range(self._n):
+            a = self._oldc[i]
+            gch = fdiv(abs(at(a, 0) + at(a, 1) - at(a, 2) - at(a, 3)), at(a, 2) + at(a, 3))
+            if gch > 1e-3:
+                converged = False
+                break
+        if converged:
+            self._stop += 1
+        else:
+            self._stop = max(self._stop - 1, 0)
+        self._itn += 1
+        return self._stop > self._max
+
+
+def solveFEM(hg, tol=1e-6, maxit=200, mode=RT_SOLVE_MIXED, pre=1, post=1):
+    """Solve K(rho) u = f to ||r||/||f|| <= tol with MG-PCG. The reference
+    iterates V-cycles (solveFEM, optimization.cpp:85-91)."""
+    return hg.solve(tol=tol, maxit=maxit, mode=mode, pre=pre, post=post)
+
+
+def computeSensitivity(hg, filter_radius):
+    """computeSensitivity (optimization.cu:109-133): element sensitivity then
+    the sensitivity filter (Grid.cu:2411-2437)."""
+    hg.compute_sensitivity()
+    if filter_radius > 0:
+        hg.filter_sensitivity(filter_radius)
+
+
+def updateDensities(hg, vgoal, params):
+    """updateDensities (optimization.cu:164-226): OC bisection; returns the multiplier."""
+    thr, _ = hg.update_densities(vgoal, params.design_step, params.damp_ratio, params.min_rho)
+    return thr
+
+
+def ordinary_topopt(hg, params, n_iter=100, tol=1e-6, mode=RT_SOLVE_MIXED, pre=1, post=1, init_rho=None,
+                    log=None, stop=True):
+    """The SIMP loop of TestSuit::testOrdinaryTopopt (test_utils.cpp:589-660):
+    Vgoal schedule, update_stencil, solve, compliance, stop test, sensitivity,
+    filter, OC update. The solve is MG-PCG to ``tol`` instead of V-cycles to 1e-2.
+    Returns per-iteration records."""
+    if init_rho is not None:
+        hg.init_rho(init_rho)
+    vgoal = params.volume_ratio
+    stop_check = converge_criteria(1, 5, 1e-3)
+    recs = []
+    for itn in range(1, n_iter + 1):
+        t0 = time.perf_counter()
+        vgoal *= (1 - params.volume_decrease)
+        vc = vgoal - params.volume_ratio
+        if vgoal < params.volume_ratio:
+            vgoal = params.volume_ratio
+        hg.update_stencil()
+        st = solveFEM(hg, tol=tol, mode=mode, pre=pre, post=post)
+        c = hg.compliance()
+        if math.isnan(c) or abs(c) < 1e-11:
+            raise FloatingPointError(f"compliance {c} at iteration {itn}")
+        done = stop_check.update(c, [vc]) and vgoal <= params.volume_ratio
+        rec = dict(iter=itn, compliance=c, vgoal=vgoal, pcg_iters=st.iterations, rel_res=st.rel_res)
+        if not (stop and done):
+            computeSensitivity(hg, params.filter_radius)
+            thr = updateDensities(hg, vgoal, params)
+            rec["multiplier"] = thr
+        hg.synchronize()
+        rec["seconds"] = time.perf_counter() - t0
+        recs.append(rec)
+        if log:
+            log(rec)
+        if stop and done:
+            break
+    return recs

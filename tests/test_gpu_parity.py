@@ -1,0 +1,218 @@
+"""Parity of the HIP path with the oracle, operator by operator, through the C ABI.
+
+FP64 kernels are compared with the FP64 oracle at 1e-12 (relative to the max
+abs value) with identical rho^p on both sides; FP32 kernels at 2e-5.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import robtop_amd as rt
+from problems import active_cube, bits_of, problem_inputs, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = {64: 1e-12, 32: 2e-5}
+
+
+class Pair:
+    def __init__(self, kind, seed=0, E=1e6, nu=0.4, p=3.0):
+        a = active_cube(kind, 64 if kind.endswith("4") or kind == "config0" else 32)
+        R = a.shape[0]
+        bits = bits_of(a)
+        print(f"[pair] kind={kind} seed={seed}", flush=True)
+        self.o = oracle.Hierarchy(R, bits)
+        self.g = rt.HierarchyGrid.genFromMesh(bits, R)
+        self.o.set_material(E, nu, p)
+        self.g.set_material(E, nu, p)
+        lv0 = self.o.levels[0]
+        fixed, f, rho = problem_inputs(lv0, seed)
+        self.fixed, self.f, self.rho = fixed, f, rho
+        self.o.set_fixed(fixed)
+        self.g.set_support(fixed)
+        self.o.set_rho(rho)
+        self.g.set_density(rho)
+        self.o.set_rhop(self.g.get_rhop())     # identical rho^p on both sides
+        self.g.set_force(f)
+        self.o.set_vec(0, 1, f)
+        self.rng = np.random.default_rng(seed + 100)
+
+    def rand_vec(self, li, fixed_zero=True):
+        lv = self.o.levels[li]
+        v = self.rng.standard_normal((3, lv["nn"])) * lv["vact"][None]
+        if li == 0 and fixed_zero:
+            v[:, self.fixed.astype(bool)] = 0
+        return v
+
+
+@pytest.fixture(scope="module", params=["holes3", "lshape4"])
+def pair(request):
+    return Pair(request.param, seed=1)
+
+
+def test_levels_and_masks(pair):
+    g, o = pair.g, pair.o
+    assert g.n_levels == o.nlevels
+    assert g.n_layers == o.nlayers
+    for L in range(o.nlayers):
+        assert g.layer_active_elements(L) == o.layer_active(L)
+    for i, lv in enumerate(o.levels):
+        gi = g[i]
+        assert gi._layer == lv["layer"]
+        assert np.array_equal(gi.origin, lv["o"]) and np.array_equal(gi.elements, lv["e"])
+        e, v = gi.masks()
+        assert np.array_equal(e, lv["eact"]) and np.array_equal(v, lv["vact"])
+
+
+def test_template_matrix(pair):
+    assert rel_err(pair.g.template_matrix(), oracle.ke(1e6, 0.4)) < 1e-14
+
+
+def test_rhop_is_fp32_powf(pair):
+    rp = pair.g.get_rhop()
+    ref = (pair.rho.astype(np.float64) ** 3).astype(np.float32)
+    act = pair.o.levels[0]["eact"]
+    ulp = np.abs(rp[act].view(np.int32) - ref[act].view(np.int32))
+    assert ulp.max() <= 2
+    assert np.all(rp[~act] == 0)
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_fine_residual(pair, bits):
+    g, o = pair.g, pair.o
+    g.set_precision(bits)
+    u, f = pair.rand_vec(0), pair.rand_vec(0)
+    g[0].setDisplacement(u); g[0].setForce(f)
+    o.set_vec(0, 0, u); o.set_vec(0, 1, f)
+    g[0].update_residual(); o.update_residual(0)
+    assert rel_err(g[0].getResidual(), o.get_vec(0, 2)) < TOL[bits]
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+@pytest.mark.parametrize("reverse", [False, True])
+def test_fine_gs(pair, bits, reverse):
+    g, o = pair.g, pair.o
+    g.set_precision(bits)
+    u, f = pair.rand_vec(0), pair.rand_vec(0)
+    g[0].setDisplacement(u); g[0].setForce(f)
+    o.set_vec(0, 0, u); o.set_vec(0, 1, f)
+    g[0].gs_relax(2, reverse=reverse); o.gs_relax(0, 2, reverse)
+    assert rel_err(g[0].getDisplacement(), o.get_vec(0, 0)) < TOL[bits]
+
+
+def test_fine_gs_fixed_values_ignored(pair):
+    """Nonzero displacement/force at supports: neighbours still treat them as 0
+    (Grid.cu:981-984) and the support node itself becomes f / n_e."""
+    g, o = pair.g, pair.o
+    g.set_precision(64)
+    u, f = pair.rand_vec(0, fixed_zero=False), pair.rand_vec(0, fixed_zero=False)
+    g[0].setDisplacement(u); g[0].setForce(f)
+    o.set_vec(0, 0, u); o.set_vec(0, 1, f)
+    g[0].gs_relax(1); o.gs_relax(0, 1)
+    assert rel_err(g[0].getDisplacement(), o.get_vec(0, 0)) < 1e-12
+    g[0].update_residual(); o.update_residual(0)
+    assert rel_err(g[0].getResidual(), o.get_vec(0, 2)) < 1e-12
+
+
+def test_galerkin_stencils(pair):
+    g, o = pair.g, pair.o
+    g.update_stencil(); o.update_stencil()
+    for li in range(1, o.nlevels):
+        assert rel_err(g[li].getStencil(), o.get_stencil(li)) < 1e-12, li
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_transfers(pair, bits):
+    g, o = pair.g, pair.o
+    g.set_precision(bits)
+    for li in range(1, o.nlevels):
+        r = pair.rand_vec(li - 1, fixed_zero=False)
+        g[li - 1].setResidual(r); o.set_vec(li - 1, 2, r)
+        g[li].restrict_residual(); o.restrict_residual(li)
+        assert rel_err(g[li].getForce(), o.get_vec(li, 1)) < TOL[bits], ("restrict", li)
+        uf, uc = pair.rand_vec(li - 1, fixed_zero=False), pair.rand_vec(li, fixed_zero=False)
+        g[li - 1].setDisplacement(uf); o.set_vec(li - 1, 0, uf)
+        g[li].setDisplacement(uc); o.set_vec(li, 0, uc)
+        g[li - 1].prolongate_correction(); o.prolongate_correction(li - 1)
+        assert rel_err(g[li - 1].getDisplacement(), o.get_vec(li - 1, 0)) < TOL[bits], ("prolong", li)
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_coarse_gs_residual_coarsest(pair, bits):
+    g, o = pair.g, pair.o
+    g.update_stencil(); o.update_stencil()
+    g.set_precision(bits)
+    for li in range(1, o.nlevels - 1):
+        u, f = pair.rand_vec(li), pair.rand_vec(li)
+        g[li].setDisplacement(u); g[li].setForce(f)
+        o.set_vec(li, 0, u); o.set_vec(li, 1, f)
+        g[li].update_residual(); o.update_residual(li)
+        assert rel_err(g[li].getResidual(), o.get_vec(li, 2)) < TOL[bits] * 10
+        g[li].gs_relax(1); o.gs_relax(li, 1)
+        assert rel_err(g[li].getDisplacement(), o.get_vec(li, 0)) < TOL[bits] * 10
+        g[li].gs_relax(1, reverse=True); o.gs_relax(li, 1, True)
+        assert rel_err(g[li].getDisplacement(), o.get_vec(li, 0)) < TOL[bits] * 10
+    L = o.nlevels - 1
+    f = pair.rand_vec(L)
+    g[L].setForce(f); o.set_vec(L, 1, f)
+    g[L].solve_fem_host(); o.solve_coarsest()
+    assert rel_err(g[L].getDisplacement(), o.get_vec(L, 0)) < (1e-9 if bits == 64 else 1e-5)
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_vcycle(pair, bits):
+    g, o = pair.g, pair.o
+    g.update_stencil(); o.update_stencil()
+    g.set_precision(bits)
+    g.set_displacement(np.zeros((3, o.levels[0]["nn"])))
+    o.set_vec(0, 0, np.zeros((3, o.levels[0]["nn"])))
+    o.set_vec(0, 1, pair.f)
+    for k in range(3):
+        rg = g.v_cycle(1, 1)
+        ro = o.v_cycle(1, 1)
+        assert abs(rg - ro) / ro < (1e-9 if bits == 64 else 1e-4), (k, rg, ro)
+    assert rel_err(g.get_displacement(), o.get_vec(0, 0)) < (1e-10 if bits == 64 else 1e-4)
+    g.set_precision(64)
+
+
+@pytest.mark.parametrize("mode", [rt.RT_SOLVE_FP64, rt.RT_SOLVE_MIXED, rt.RT_SOLVE_FP32])
+def test_pcg_solution(pair, mode):
+    """MG-PCG to 1e-8 against the oracle's V-cycle iteration converged to 1e-12."""
+    g, o = pair.g, pair.o
+    g.update_stencil(); o.update_stencil()
+    nn = o.levels[0]["nn"]
+    o.set_vec(0, 0, np.zeros((3, nn))); o.set_vec(0, 1, pair.f)
+    for _ in range(200):
+        if o.v_cycle(2, 2) < 1e-12:
+            break
+    u_ref = o.get_vec(0, 0)
+    c_ref = o.compliance()
+    g.set_displacement(np.zeros((3, nn)))
+    tol = 1e-8 if mode != rt.RT_SOLVE_FP32 else 1e-6
+    st = g.solve(tol=tol, maxit=200, mode=mode, pre=1, post=1)
+    assert st.rel_res < tol * 5, st.rel_res
+    err = 1e-6 if mode != rt.RT_SOLVE_FP32 else 1e-4
+    assert rel_err(g.get_displacement(), u_ref) < err
+    assert abs(g.compliance() - c_ref) / abs(c_ref) < err
+
+
+def test_sensitivity_filter_oc(pair):
+    g, o = pair.g, pair.o
+    g.update_stencil(); o.update_stencil()
+    nn = o.levels[0]["nn"]
+    u = pair.rand_vec(0) * 1e-3
+    g.set_displacement(u); o.set_vec(0, 0, u)
+    g.compute_sensitivity()
+    gs = g.get_sensitivity()
+    os_ = o.sensitivity()
+    assert rel_err(gs, os_) < 2e-6
+    g.filter_sensitivity(2); gf = g.get_sensitivity()
+    of = o.filter(2.0, gs)
+    assert rel_err(gf, of) < 2e-6
+    # OC from identical sensitivities
+    g.set_sensitivity(of)
+    thr_g, ntr_g = g.update_densities(0.4, 0.03, 0.5, 1e-3)
+    thr_o, ntr_o = o.oc_update(0.4, 0.03, 0.5, 1e-3, of)
+    assert ntr_g == ntr_o and thr_g == thr_o
+    assert rel_err(g.get_density(), o.get_rho()) < 2e-6
+    assert abs(g.volume() - 0.4) <= 1e-4 + 1e-6 or ntr_g == 31
