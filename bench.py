@@ -1,0 +1,208 @@
+"""Benchmark: seconds per SIMP iteration at 511x256x256 (cantilever, R = 512)
+to relative residual 1e-6, and the V-cycle's algorithmic HBM bandwidth.
+
+One step = update_stencil (Galerkin) + MG-PCG solve to 1e-6 (FP64 Krylov,
+FP32 V-cycle) + compliance + sensitivity + filter (r = 2) + OC update, all on
+the GPU through the C ABI. Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 511,256,256]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", default="511,256,256")
+    ap.add_argument("--pre", type=int, default=1)
+    ap.add_argument("--post", type=int, default=1)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--vcycles", type=int, default=20, help="graphless V-cycles timed for the roofline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--cpu-only", action="store_true")
+    return ap.parse_args()
+
+
+def algorithmic_vcycle_bytes(levels, pre, post, coarsest_dofs, P=4):
+    """Compulsory bytes of one V(pre, post): each array element an operation
+    must read or write, counted once per operation; u/f/r in FP32, stencils in
+    FP32, 1 flag byte per node, rho^p 4 B per element."""
+    n0, e0 = levels[0]["nodes"], levels[0]["elements"]
+    sweep0 = n0 * (9 * P + 1) + 4 * e0
+    res0 = sweep0
+    total = (pre + post) * sweep0 + res0
+    nlev = len(levels)
+    for i in range(1, nlev):
+        nf, nc = levels[i - 1]["nodes"], levels[i]["nodes"]
+        total += 3 * P * nf + 3 * P * nc            # restrict (read r_f, write f_c)
+        total += 3 * P * nc + nf * (6 * P + 1)      # prolong (read u_c, read+write u_f, flag)
+        if i < nlev - 1:
+            nl = levels[i]["nodes"]
+            per = nl * (243 * P + 9 * P + 1)
+            total += (pre + post) * per + per       # sweeps + residual
+    total += coarsest_dofs ** 2 * 8
+    return total
+
+
+def build_problem(ex, ey, ez, seed):
+    import robtop_amd as rt
+    from robtop_amd.workloads import box_bits, cube_side, cantilever_bc, smoothed_random_density
+    R = cube_side(ex, ey, ez)
+    bits = box_bits(R, ex, ey, ez)
+    hg = rt.HierarchyGrid.genFromMesh(bits, R)
+    hg.set_material(1e6, 0.4, 3.0)
+    fixed, f, nload = cantilever_bc(hg)
+    hg.set_support(fixed)
+    hg.set_force(f)
+    g0 = hg[0]
+    eact, _ = g0.masks()
+    rho = smoothed_random_density(eact, tuple(g0.elements), seed)
+    hg.set_density(rho)
+    return hg, bits, R, fixed, f, rho
+
+
+def cpu_baseline(bits, R, fixed, f, rho, pre, post, n_vcycles, n_applies):
+    """Oracle (FP64 C++/OpenMP restatement of the reference kernels) on the host
+    cores: one fine-level GS sweep and one fine residual at the full size; the
+    SIMP-iteration time is extrapolated from the GPU run's V-cycle and operator
+    counts (fine level only; coarse levels, Galerkin and OC excluded, so this is a
+    lower bound of the CPU path)."""
+    import oracle
+    o = oracle.Hierarchy(R, bits)
+    o.set_material(1e6, 0.4, 3.0)
+    o.set_fixed(fixed)
+    o.set_rho(rho)
+    nn = o.levels[0]["nn"]
+    o.set_vec(0, 1, f)
+    o.set_vec(0, 0, np.zeros((3, nn)))
+    t0 = time.perf_counter()
+    o.gs_relax(0, 1)
+    t1 = time.perf_counter()
+    o.update_residual(0)
+    t2 = time.perf_counter()
+    t_sweep, t_res = t1 - t0, t2 - t1
+    est = n_vcycles * ((pre + post) * t_sweep + t_res) + n_applies * t_res
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=est, unit="s/iter", cores=cores, kind="port",
+                sample=(f"oracle FP64 fine-level GS sweep ({t_sweep:.2f} s) and residual ({t_res:.2f} s) at "
+                        f"full size on {cores} threads, scaled by the GPU run's {n_vcycles:.1f} V-cycles and "
+                        f"{n_applies:.1f} operator applies per SIMP iteration; fine level only (lower bound)"))
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    ex, ey, ez = (int(v) for v in a.size.split(","))
+    import robtop_amd as rt
+    log(f"building {ex}x{ey}x{ez}")
+    hg, bits, R, fixed, f, rho = build_problem(ex, ey, ez, a.seed)
+    log(f"hierarchy: {hg.n_levels} levels")
+    g0 = hg[0]
+    ne0, nn0 = hg.num_active(0)
+    levels = []
+    for i in range(hg.n_levels):
+        ne, nn = hg.num_active(i)
+        levels.append(dict(nodes=nn, elements=ne))
+    coarsest_dofs = 3 * levels[-1]["nodes"]
+
+    vgoal, step, damp, rmin, rfilt = 0.4, 0.03, 0.5, 1e-3, 2
+
+    def simp_step():
+        hg.update_stencil()
+        st = hg.solve(tol=a.tol, maxit=500, mode=rt.RT_SOLVE_MIXED, pre=a.pre, post=a.post)
+        c = hg.compliance()
+        hg.compute_sensitivity()
+        hg.filter_sensitivity(rfilt)
+        hg.update_densities(vgoal, step, damp, rmin)
+        return st, c
+
+    for w in range(a.warmup):
+        st, _ = simp_step()
+        log(f"warmup {w}: {st.iterations} PCG iterations, rel_res {st.rel_res:.2e}")
+    hg.synchronize()
+    iters, rels, comps = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        st, c = simp_step()
+        log(f"step: {st.iterations} PCG iterations, rel_res {st.rel_res:.2e}, c {c:.6e}")
+        iters.append(st.iterations)
+        rels.append(st.rel_res)
+        comps.append(c)
+    hg.synchronize()
+    t1 = time.perf_counter()
+    s_per_iter = (t1 - t0) / max(1, a.steps)
+
+    # V-cycle and dominant-kernel timing (HIP events on the handle's stream)
+    hg.update_stencil()
+    t_v = hg.time_vcycle(32, a.pre, a.post, a.vcycles)
+    t_sweep = hg.time_fine_sweep(32, 10)
+    B_V = algorithmic_vcycle_bytes(levels, a.pre, a.post, coarsest_dofs)
+    sweep_bytes = levels[0]["nodes"] * (9 * 4 + 1) + 4 * levels[0]["elements"]
+    launch_bytes = sweep_bytes / 8.0            # one colour pass
+    launch_ms = t_sweep / 8.0
+    achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "traffic_fine_gs.json")
+    if os.path.exists(tp):
+        try:
+            traffic = json.load(open(tp)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    out = {
+        "metric": "s/SIMP iter at 511×256×256 to rel-res 1e-6; V-cycle HBM GB/s vs peak",
+        "value": s_per_iter,
+        "unit": "s/iter",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": s_per_iter * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"cantilever {ex}x{ey}x{ez} voxels (R={R}), seeded smoothed-random density, "
+                        f"MG-PCG mixed V({a.pre},{a.post}) to {a.tol:g}, filter r=2, OC",
+            "active_elements": ne0, "active_nodes": nn0, "dofs": 3 * nn0, "levels": hg.n_levels,
+            "pcg_iters_per_step": float(np.mean(iters)), "rel_res": float(np.max(rels)),
+            "compliance_last": comps[-1],
+        },
+        "vcycle": {"ms": t_v, "algorithmic_bytes": B_V, "GBps": B_V / (t_v * 1e-3) / 1e9,
+                   "frac_of_peak": B_V / (t_v * 1e-3) / 1e9 / HBM_PEAK_GBS, "pre": a.pre, "post": a.post},
+        "roofline": {"kernel": "k_fine_gs (one colour pass of the finest-level smoother)", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "bytes_per_launch": launch_bytes, "launch_ms": launch_ms},
+    }
+    if not a.no_cpu:
+        n_v = float(np.mean(iters)) + 1.0
+        n_app = float(np.mean(iters)) + 2.0
+        hg.close()
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(bits, R, fixed, f, rho, a.pre, a.post, n_v, n_app)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -42,7 +42,8 @@ enum {
 enum {
     RT_SOLVE_FP64 = 0,      /* FP64 PCG, FP64 V-cycle preconditioner */
     RT_SOLVE_FP32 = 1,      /* FP32 PCG vectors (FP64 dots), FP32 V-cycle */
-    RT_SOLVE_MIXED = 2      /* FP64 PCG vectors and dots, FP32 V-cycle preconditioner */
+    RT_SOLVE_MIXED = 2,     /* FP64 PCG vectors and dots, FP32 V-cycle preconditioner */
+    RT_SOLVE_FP32_IR = 3    /* FP64 iterative refinement (true residual) around FP32 MG-PCG */
 };
 
 /* vector selectors for the per-level accessors */

@@ -614,35 +614,27 @@ static double true_rel_residual(Hier& H) {
     return ff > 0 ? std::sqrt(rr / ff) : 0.0;
 }
 
+// PCG on caller-given vectors x (initial guess, updated) and b; stops when the
+// recurrence residual reaches tol * |b|. Returns the iteration count; `rel`
+// receives the final recurrence residual ratio.
 template <typename TV, typename TC>
-static void pcg(Hier& H, double tol, int maxit, int pre, int post, rt_solve_stats* st) {
-    if (!H.stencil_ready) throw std::logic_error("update_stencil has not been called");
-    auto t0 = std::chrono::steady_clock::now();
+static int pcg_core(Hier& H, V3<TV>& x, V3<TV>& b, double tol, int maxit, int pre, int post, double* rel_out) {
     Level& L0 = H.lv[0];
     const int N = L0.g.N;
     const size_t n = (size_t)3 * N;
-    V3<TV>& x = KV<TV>::x(H);
-    V3<TV>& b = KV<TV>::b(H);
     V3<TV>& r = KV<TV>::r(H);
     V3<TV>& p = KV<TV>::p(H);
     V3<TV>& q = KV<TV>::q(H);
-    if (!x.ok()) x.alloc(N);
-    if (!b.ok()) b.alloc(N);
     if (!r.ok()) r.alloc(N);
     if (!p.ok()) p.alloc(N);
     if (!q.ok()) q.alloc(N);
-    if ((void*)x.data() != (void*)H.x64.data()) {
-        launch_convert<double, TV>(H.x64.data(), x.data(), n, H.s);
-        launch_convert<double, TV>(H.f64.data(), b.data(), n, H.s);
-    }
     LevelVecs<TC>& pc = pcvecs<TC>(H);
     const int np = 1024;
     // r = b - A x
     launch_fine_apply<TV>(L0.g, x.cv(), q.v(), H.rhop.as<float>(), L0.vflag.as<uint8_t>(), nullptr, nullptr, H.s);
     launch_sub_conv<TV, TC>(b.data(), q.data(), r.data(), pc.F.data(), n, H.partials.as<double>(), np, H.s);
     double rr = sum_partials_host(H, np);
-    const double bb = dot<TV, TV>(H, b.data(), b.data(), n);
-    const double nb = std::sqrt(bb);
+    const double nb = std::sqrt(dot<TV, TV>(H, b.data(), b.data(), n));
     int it = 0;
     double rel = nb > 0 ? std::sqrt(rr) / nb : 0.0;
     if (nb > 0 && rel > tol) {
@@ -668,6 +660,27 @@ static void pcg(Hier& H, double tol, int maxit, int pre, int post, rt_solve_stat
         }
         if (it > maxit) it = maxit;
     }
+    if (rel_out) *rel_out = rel;
+    return it;
+}
+
+template <typename TV, typename TC>
+static void pcg(Hier& H, double tol, int maxit, int pre, int post, rt_solve_stats* st) {
+    if (!H.stencil_ready) throw std::logic_error("update_stencil has not been called");
+    auto t0 = std::chrono::steady_clock::now();
+    Level& L0 = H.lv[0];
+    const int N = L0.g.N;
+    const size_t n = (size_t)3 * N;
+    V3<TV>& x = KV<TV>::x(H);
+    V3<TV>& b = KV<TV>::b(H);
+    if (!x.ok()) x.alloc(N);
+    if (!b.ok()) b.alloc(N);
+    if ((void*)x.data() != (void*)H.x64.data()) {
+        launch_convert<double, TV>(H.x64.data(), x.data(), n, H.s);
+        launch_convert<double, TV>(H.f64.data(), b.data(), n, H.s);
+    }
+    double rel = 0;
+    const int it = pcg_core<TV, TC>(H, x, b, tol, maxit, pre, post, &rel);
     if ((void*)x.data() != (void*)H.x64.data()) launch_convert<TV, double>(x.data(), H.x64.data(), n, H.s);
     RT_HIP(hipStreamSynchronize(H.s));
     const double tr = true_rel_residual(H);
@@ -676,6 +689,35 @@ static void pcg(Hier& H, double tol, int maxit, int pre, int post, rt_solve_stat
         st->iterations = it;
         st->rel_res = tr;
         st->rel_res_pcg = rel;
+        st->seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+}
+
+// Iterative refinement: FP64 true residual r = f - K x outside, FP32 MG-PCG on
+// K e = r inside (each inner solve reduces its residual by `inner`), x += e.
+static void pcg_ir(Hier& H, double tol, int maxit, int pre, int post, double inner, rt_solve_stats* st) {
+    if (!H.stencil_ready) throw std::logic_error("update_stencil has not been called");
+    auto t0 = std::chrono::steady_clock::now();
+    Level& L0 = H.lv[0];
+    const int N = L0.g.N;
+    const size_t n = (size_t)3 * N;
+    if (!H.x32.ok()) H.x32.alloc(N);
+    if (!H.b32.ok()) H.b32.alloc(N);
+    int total = 0;
+    double rel = true_rel_residual(H), rel_in = 0;   // leaves r64 = f - K x64
+    for (int outer = 0; outer < 50 && rel > tol && total < maxit; outer++) {
+        launch_fill<float>(H.x32.data(), 0.f, n, H.s);
+        launch_convert<double, float>(H.r64.data(), H.b32.data(), n, H.s);
+        const double want = std::max(inner, 0.5 * tol / rel);
+        total += pcg_core<float, float>(H, H.x32, H.b32, want, maxit - total, pre, post, &rel_in);
+        launch_add_conv<double, float>(H.x64.data(), H.x32.data(), n, H.s);
+        rel = true_rel_residual(H);
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (st) {
+        st->iterations = total;
+        st->rel_res = rel;
+        st->rel_res_pcg = rel_in;
         st->seconds = std::chrono::duration<double>(t1 - t0).count();
     }
 }
@@ -1208,6 +1250,7 @@ int rt_solve(rt_hier* h, double tol, int maxit, int mode, int pre, int post, rt_
         if (mode == RT_SOLVE_FP64) pcg<double, double>(H, tol, maxit, pre, post, stats);
         else if (mode == RT_SOLVE_FP32) pcg<float, float>(H, tol, maxit, pre, post, stats);
         else if (mode == RT_SOLVE_MIXED) pcg<double, float>(H, tol, maxit, pre, post, stats);
+        else if (mode == RT_SOLVE_FP32_IR) pcg_ir(H, tol, maxit, pre, post, 1e-4, stats);
         else throw std::invalid_argument("unknown solver mode");
     })
 }

@@ -167,17 +167,21 @@ def test_vcycle(pair, bits):
     g.set_displacement(np.zeros((3, o.levels[0]["nn"])))
     o.set_vec(0, 0, np.zeros((3, o.levels[0]["nn"])))
     o.set_vec(0, 1, pair.f)
+    # FP32: the residual of an FP32 iterate carries cancellation error of order
+    # eps32 * |K u| / |r|, so the reported ratio is compared at 1e-3
     for k in range(3):
         rg = g.v_cycle(1, 1)
         ro = o.v_cycle(1, 1)
-        assert abs(rg - ro) / ro < (1e-9 if bits == 64 else 1e-4), (k, rg, ro)
-    assert rel_err(g.get_displacement(), o.get_vec(0, 0)) < (1e-10 if bits == 64 else 1e-4)
+        assert abs(rg - ro) / ro < (1e-9 if bits == 64 else 1e-3), (k, rg, ro)
+    assert rel_err(g.get_displacement(), o.get_vec(0, 0)) < (1e-10 if bits == 64 else 2e-4)
     g.set_precision(64)
 
 
-@pytest.mark.parametrize("mode", [rt.RT_SOLVE_FP64, rt.RT_SOLVE_MIXED, rt.RT_SOLVE_FP32])
+@pytest.mark.parametrize("mode", [rt.RT_SOLVE_FP64, rt.RT_SOLVE_MIXED, rt.RT_SOLVE_FP32_IR, rt.RT_SOLVE_FP32])
 def test_pcg_solution(pair, mode):
-    """MG-PCG to 1e-8 against the oracle's V-cycle iteration converged to 1e-12."""
+    """MG-PCG to 1e-8 against the oracle's V-cycle iteration converged to 1e-12.
+    Plain FP32 PCG stops on its FP32 recurrence; its true residual is only
+    bounded by the FP32 residual gap, so it is checked at 1e-6 / 1e-4."""
     g, o = pair.g, pair.o
     g.update_stencil(); o.update_stencil()
     nn = o.levels[0]["nn"]
@@ -188,10 +192,14 @@ def test_pcg_solution(pair, mode):
     u_ref = o.get_vec(0, 0)
     c_ref = o.compliance()
     g.set_displacement(np.zeros((3, nn)))
-    tol = 1e-8 if mode != rt.RT_SOLVE_FP32 else 1e-6
-    st = g.solve(tol=tol, maxit=200, mode=mode, pre=1, post=1)
-    assert st.rel_res < tol * 5, st.rel_res
-    err = 1e-6 if mode != rt.RT_SOLVE_FP32 else 1e-4
+    plain32 = mode == rt.RT_SOLVE_FP32
+    tol = 1e-6 if plain32 else 1e-8
+    st = g.solve(tol=tol, maxit=300, mode=mode, pre=1, post=1)
+    if plain32:
+        assert st.rel_res_pcg <= tol and st.rel_res < 1e-4, (st.rel_res_pcg, st.rel_res)
+    else:
+        assert st.rel_res < tol * 5, st.rel_res
+    err = 1e-3 if plain32 else 1e-6
     assert rel_err(g.get_displacement(), u_ref) < err
     assert abs(g.compliance() - c_ref) / abs(c_ref) < err
 
