@@ -27,8 +27,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", default="511,256,256")
-    ap.add_argument("--pre", type=int, default=1)
-    ap.add_argument("--post", type=int, default=1)
+    ap.add_argument("--pre", type=int, default=2)
+    ap.add_argument("--post", type=int, default=2)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--vcycles", type=int, default=20, help="graphless V-cycles timed for the roofline")

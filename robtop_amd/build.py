@@ -26,6 +26,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 
 UNITS = {
     "k_fine.hip": [],
+    "k_fine_modal.hip": [],
     "k_coarse.hip": [],
     "k_galerkin.hip": [],
     "k_simp.hip": [],

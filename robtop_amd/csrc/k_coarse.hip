@@ -411,6 +411,21 @@ void launch_sub_conv(const T* b, const T* q, T* r, TZ* rz, size_t n, double* par
     RT_HIP(hipGetLastError());
 }
 
+// x *= a
+template <typename T>
+__global__ void k_scale(T* __restrict__ x, double a, size_t n) {
+    const T s = (T)a;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) x[i] *= s;
+}
+template <typename T>
+void launch_scale(T* x, double a, size_t n, hipStream_t s) {
+    unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL((k_scale<T>), dim3(nb), dim3(256), 0, s, x, a, n);
+    RT_HIP(hipGetLastError());
+}
+template void launch_scale<float>(float*, double, size_t, hipStream_t);
+template void launch_scale<double>(double*, double, size_t, hipStream_t);
+
 // x += e (TE -> T)
 template <typename T, typename TE>
 __global__ void k_add_conv(T* __restrict__ x, const TE* __restrict__ e, size_t n) {

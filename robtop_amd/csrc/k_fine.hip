@@ -1,137 +1,194 @@
-// Finest-level matrix-free operators on the octant layout (see rt_internal.h).
+// Finest-level nodal Gauss-Seidel on the octant layout (see rt_internal.h).
 //
-// For a node v the reference forms, element by element (Grid.cu:922-1066 for
-// the smoother, :1501-1557 for the residual),
-//     KU_v = sum_{e ni v} rho_e^p * KE[v_loc(e), :] * u_e
-// where the element incident at offset w = (w%2-1, w/2%2-1, w/4-1) holds v as
-// local vertex 7-w. Support ("fixed") neighbours contribute nothing; a fixed
-// node gets S = n_e I and KU = 0 in the smoother and r = f in the residual.
+// For a node v the reference forms, element by element (Grid.cu:922-1066),
+//     KeU_v = sum_{e ni v} rho_e^p * sum_{j in e, j != v} KE[v_loc(e)][j_loc(e)] u_j
+//     S_v   = sum_{e ni v} rho_e^p * KE[v_loc(e)][v_loc(e)]
+// where the element incident at w = (w%2-1, w/2%2-1, w/4-1) holds v as local
+// vertex 7-w; support ("fixed") neighbours contribute nothing, a fixed node
+// gets S = n_e I and KeU = 0; then x, y, z are relaxed in turn.
 //
-// Here each thread owns one node. The colour is a template parameter, so every
-// neighbour and element slot is base + (compile-time constant combination of
-// the runtime strides); threads of a wavefront own consecutive nodes of one
-// colour block and every load is a contiguous run. KE lives in constant memory
-// and is addressed with compile-time indices (scalar loads).
+// One thread per node of the colour being relaxed; the colour is a template
+// parameter, so each of the 27 neighbour slots and 8 element slots is the
+// thread's base index plus a wave-uniform offset. Loads are raw buffer loads
+// with that offset in a scalar register (no per-load address arithmetic). The
+// loop runs over neighbours and accumulates into 8 per-element partials, so a
+// neighbour's displacement is live only while it is used. KE enters as 10
+// signed scalar constants (ke_pattern.h): every FMA takes its coefficient from
+// an SGPR with a sign modifier.
 #include "rt_internal.h"
+#include "ke_pattern.h"
+
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+#include <utility>
 
 namespace rt {
 
-__constant__ float c_KE32[576];
-__constant__ double c_KE64[576];
+template <typename T> struct KeC { T c[RT_KE_NVAL]; };
 
-template <typename T> __device__ __forceinline__ T ke_at(int i);
-template <> __device__ __forceinline__ float ke_at<float>(int i) { return c_KE32[i]; }
-template <> __device__ __forceinline__ double ke_at<double>(int i) { return c_KE64[i]; }
+static double g_kec[RT_KE_NVAL];
 
-void set_ke_constants(const double* ke576, hipStream_t s) {
-    float k32[576];
-    for (int i = 0; i < 576; i++) k32[i] = (float)ke576[i];
-    RT_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_KE64), ke576, sizeof(double) * 576, 0, hipMemcpyHostToDevice, s));
-    RT_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_KE32), k32, sizeof(float) * 576, 0, hipMemcpyHostToDevice, s));
-    RT_HIP(hipStreamSynchronize(s));
+// the 10 distinct magnitudes of KE, read off the table (validated entry by entry)
+void set_ke_constants(const double* ke576, hipStream_t) {
+    bool seen[RT_KE_NVAL] = {false};
+    for (int i = 0; i < 576; i++) {
+        const int c = rt_ke_code[i];
+        if (c && !seen[std::abs(c) - 1]) { g_kec[std::abs(c) - 1] = c > 0 ? ke576[i] : -ke576[i]; seen[std::abs(c) - 1] = true; }
+    }
+    double scale = 0;
+    for (int i = 0; i < 576; i++) scale = std::max(scale, std::fabs(ke576[i]));
+    for (int i = 0; i < 576; i++) {
+        const int c = rt_ke_code[i];
+        const double v = c == 0 ? 0.0 : (c > 0 ? g_kec[c - 1] : -g_kec[-c - 1]);
+        if (std::fabs(v - ke576[i]) > 1e-12 * scale)
+            throw std::runtime_error("element matrix does not follow the isotropic hexahedron pattern");
+    }
 }
 
-struct ColourGrid {
-    int nh[3];   // number of interior positions of this colour per axis
-    int total;
-};
-
-template <int C>
-__device__ __forceinline__ bool colour_thread(const Geom& g, int t, int& base) {
-    constexpr int cx = C & 1, cy = (C >> 1) & 1, cz = C >> 2;
-    const int nhx = (g.n[0] - cx + 1) >> 1, nhy = (g.n[1] - cy + 1) >> 1, nhz = (g.n[2] - cz + 1) >> 1;
-    if (t >= nhx * nhy * nhz) return false;
-    const int i = t % nhx;
-    const int r = t / nhx;
-    const int j = r % nhy;
-    const int k = r / nhy;
-    base = ((k + 1) * g.h[1] + (j + 1)) * g.h[0] + (i + 1);
-    return true;
+template <typename T>
+static KeC<T> kec() {
+    KeC<T> k;
+    for (int i = 0; i < RT_KE_NVAL; i++) k.c[i] = (T)g_kec[i];
+    return k;
 }
 
-// slot of the neighbour at (dx,dy,dz) of a colour-C node with half-index base
+template <typename T> __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ uint8_t bload_u8(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_amdgcn_raw_buffer_load_b8(r, voff, soff, 0);
+}
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes, 0x00020000);
+}
+
+// offset (in slots, shifted to be >= 0) of the point at (DX,DY,DZ) in {-1,0,1}^3
+// from a colour-C node whose half-index base is b: slot = (b - hxy - hx - 1) + off
 template <int C, int DX, int DY, int DZ>
-__device__ __forceinline__ int nb_slot(const Geom& g, int base) {
+__device__ __forceinline__ int rel_off(const Geom& g) {
     constexpr int cx = C & 1, cy = (C >> 1) & 1, cz = C >> 2;
     constexpr int col = C ^ ((DX & 1) | ((DY & 1) << 1) | ((DZ & 1) << 2));
     constexpr int ox = (cx + DX) >> 1, oy = (cy + DY) >> 1, oz = (cz + DZ) >> 1;
-    return col * g.cs + base + (oz * g.h[1] + oy) * g.h[0] + ox;
+    return col * g.cs + ((oz + 1) * g.h[1] + (oy + 1)) * g.h[0] + (ox + 1);
 }
 
-template <int C>
-__device__ __forceinline__ int nb_slot_rt(const Geom& g, int base, int nb) {
-    constexpr int cx = C & 1, cy = (C >> 1) & 1, cz = C >> 2;
-    const int dx = nb % 3 - 1, dy = nb / 3 % 3 - 1, dz = nb / 9 - 1;
-    const int col = C ^ ((dx & 1) | ((dy & 1) << 1) | ((dz & 1) << 2));
-    const int ox = (cx + dx) >> 1, oy = (cy + dy) >> 1, oz = (cz + dz) >> 1;
-    return col * g.cs + base + (oz * g.h[1] + oy) * g.h[0] + ox;
+constexpr int ke_code_at(int i) { return rt_ke_code[i]; }
+
+template <int CODE, typename T>
+__device__ __forceinline__ T kc(const KeC<T>& K) {
+    static_assert(CODE != 0, "isotropic KE has no zero entry");
+    if constexpr (CODE > 0) return K.c[CODE - 1];
+    else return -K.c[-CODE - 1];
 }
 
-// Element incident at w (offset (w%2-1, w/2%2-1, w/4-1)) of a colour-C node.
-template <int C>
-__device__ __forceinline__ int elem_slot(const Geom& g, int base, int w) {
-    constexpr int cx = C & 1, cy = (C >> 1) & 1, cz = C >> 2;
-    const int ox = w % 2 - 1, oy = w / 2 % 2 - 1, oz = w / 4 - 1;
-    const int col = C ^ ((ox & 1) | ((oy & 1) << 1) | ((oz & 1) << 2));
-    const int hx = (cx + ox) >> 1, hy = (cy + oy) >> 1, hz = (cz + oz) >> 1;
-    return col * g.cs + base + (hz * g.h[1] + hy) * g.h[0] + hx;
-}
-
-// KU (and, for the smoother, the self block S) of one colour-C node.
-// WITH_SELF: include the node's own displacement (residual/apply);
-// CHECK_FIX: zero the displacement of fixed neighbours (Grid.cu:981-984, :1533-1535).
-template <typename T, int C, bool WITH_SELF, bool CHECK_FIX, bool WANT_S>
-__device__ __forceinline__ void node_KU(const Geom& g, int base, const T* __restrict__ U0, const T* __restrict__ U1,
-                                        const T* __restrict__ U2, const float* __restrict__ rhop,
-                                        const uint8_t* __restrict__ vflag, T KU[3], T S[9]) {
-    T u[27][3];
-#pragma unroll
-    for (int nb = 0; nb < 27; nb++) {
-        if (!WITH_SELF && nb == 13) { u[nb][0] = u[nb][1] = u[nb][2] = T(0); continue; }
-        const int s = nb_slot_rt<C>(g, base, nb);
-        T a = U0[s], b = U1[s], c = U2[s];
-        if (CHECK_FIX) {
-            if (vflag[s] & NF_FIXED) { a = T(0); b = T(0); c = T(0); }
+// acc[W] += KE[vi rows][vj cols] u_nb for neighbour NB (position NB%3, NB/3%3,
+// NB/9 in the 3x3x3 block around the node) if it is vertex vj != vi of element W
+template <typename T, int NB, int W>
+__device__ __forceinline__ void nb_elem(T (&acc)[8][3], const KeC<T>& K, T u0, T u1, T u2) {
+    constexpr int px = NB % 3, py = NB / 3 % 3, pz = NB / 9;
+    constexpr int jx = px - (W & 1), jy = py - ((W >> 1) & 1), jz = pz - (W >> 2);
+    if constexpr (jx >= 0 && jx <= 1 && jy >= 0 && jy <= 1 && jz >= 0 && jz <= 1) {
+        constexpr int vj = jx + 2 * jy + 4 * jz, vi = 7 - W;
+        if constexpr (vj != vi) {
+            acc[W][0] += kc<ke_code_at((3 * vi + 0) * 24 + 3 * vj + 0)>(K) * u0;
+            acc[W][0] += kc<ke_code_at((3 * vi + 0) * 24 + 3 * vj + 1)>(K) * u1;
+            acc[W][0] += kc<ke_code_at((3 * vi + 0) * 24 + 3 * vj + 2)>(K) * u2;
+            acc[W][1] += kc<ke_code_at((3 * vi + 1) * 24 + 3 * vj + 0)>(K) * u0;
+            acc[W][1] += kc<ke_code_at((3 * vi + 1) * 24 + 3 * vj + 1)>(K) * u1;
+            acc[W][1] += kc<ke_code_at((3 * vi + 1) * 24 + 3 * vj + 2)>(K) * u2;
+            acc[W][2] += kc<ke_code_at((3 * vi + 2) * 24 + 3 * vj + 0)>(K) * u0;
+            acc[W][2] += kc<ke_code_at((3 * vi + 2) * 24 + 3 * vj + 1)>(K) * u1;
+            acc[W][2] += kc<ke_code_at((3 * vi + 2) * 24 + 3 * vj + 2)>(K) * u2;
         }
-        u[nb][0] = a; u[nb][1] = b; u[nb][2] = c;
     }
+}
+
+template <typename T, int NB, int... Ws>
+__device__ __forceinline__ void nb_elems(T (&acc)[8][3], const KeC<T>& K, T u0, T u1, T u2,
+                                         std::integer_sequence<int, Ws...>) {
+    (nb_elem<T, NB, Ws>(acc, K, u0, u1, u2), ...);
+}
+
+struct GsBufs {
+    __amdgpu_buffer_rsrc_t u0, u1, u2, rho, flag;
+};
+
+template <typename T, int C, bool CHECK_FIX, int NB>
+__device__ __forceinline__ void nb_step(const Geom& g, int vb, const KeC<T>& K, const GsBufs& B, T (&acc)[8][3]) {
+    if constexpr (NB != 13) {
+        const int so = rel_off<C, NB % 3 - 1, NB / 3 % 3 - 1, NB / 9 - 1>(g);
+        const int vo = vb * (int)sizeof(T), sob = so * (int)sizeof(T);
+        T u0 = bload<T>(B.u0, vo, sob);
+        T u1 = bload<T>(B.u1, vo, sob);
+        T u2 = bload<T>(B.u2, vo, sob);
+        if constexpr (CHECK_FIX) {
+            if (bload_u8(B.flag, vb, so) & NF_FIXED) { u0 = T(0); u1 = T(0); u2 = T(0); }
+        }
+        nb_elems<T, NB>(acc, K, u0, u1, u2, std::make_integer_sequence<int, 8>{});
+    }
+}
+
+template <typename T, int C, bool CHECK_FIX, int... NBs>
+__device__ __forceinline__ void nb_all(const Geom& g, int vb, const KeC<T>& K, const GsBufs& B, T (&acc)[8][3],
+                                       std::integer_sequence<int, NBs...>) {
+    (nb_step<T, C, CHECK_FIX, NBs>(g, vb, K, B, acc), ...);
+}
+
+template <typename T, int C, int W>
+__device__ __forceinline__ void elem_step(const Geom& g, int vb, const KeC<T>& K, const GsBufs& B,
+                                          const T (&acc)[8][3], T KU[3], T S[9]) {
+    constexpr int vi = 7 - W;
+    const int so = rel_off<C, (W & 1) - 1, ((W >> 1) & 1) - 1, (W >> 2) - 1>(g);
+    const T pen = (T)bload<float>(B.rho, vb * 4, so * 4);
+    KU[0] += pen * acc[W][0];
+    KU[1] += pen * acc[W][1];
+    KU[2] += pen * acc[W][2];
+    S[0] += pen * kc<ke_code_at((3 * vi + 0) * 24 + 3 * vi + 0)>(K);
+    S[1] += pen * kc<ke_code_at((3 * vi + 0) * 24 + 3 * vi + 1)>(K);
+    S[2] += pen * kc<ke_code_at((3 * vi + 0) * 24 + 3 * vi + 2)>(K);
+    S[3] += pen * kc<ke_code_at((3 * vi + 1) * 24 + 3 * vi + 0)>(K);
+    S[4] += pen * kc<ke_code_at((3 * vi + 1) * 24 + 3 * vi + 1)>(K);
+    S[5] += pen * kc<ke_code_at((3 * vi + 1) * 24 + 3 * vi + 2)>(K);
+    S[6] += pen * kc<ke_code_at((3 * vi + 2) * 24 + 3 * vi + 0)>(K);
+    S[7] += pen * kc<ke_code_at((3 * vi + 2) * 24 + 3 * vi + 1)>(K);
+    S[8] += pen * kc<ke_code_at((3 * vi + 2) * 24 + 3 * vi + 2)>(K);
+}
+
+template <typename T, int C, int... Ws>
+__device__ __forceinline__ void elem_all(const Geom& g, int vb, const KeC<T>& K, const GsBufs& B, const T (&acc)[8][3],
+                                         T KU[3], T S[9], std::integer_sequence<int, Ws...>) {
+    (elem_step<T, C, Ws>(g, vb, K, B, acc, KU, S), ...);
+}
+
+template <typename T, int C, bool CHECK_FIX>
+__device__ __forceinline__ void gs_node(const Geom& g, int base, const KeC<T>& K, const GsBufs& B, T KU[3], T S[9]) {
+    const int vb = base - g.h[0] * g.h[1] - g.h[0] - 1;
+    T acc[8][3];
+#pragma unroll
+    for (int w = 0; w < 8; w++) acc[w][0] = acc[w][1] = acc[w][2] = T(0);
+    nb_all<T, C, CHECK_FIX>(g, vb, K, B, acc, std::make_integer_sequence<int, 27>{});
     KU[0] = KU[1] = KU[2] = T(0);
-    if (WANT_S) {
 #pragma unroll
-        for (int i = 0; i < 9; i++) S[i] = T(0);
-    }
-#pragma unroll
-    for (int w = 0; w < 8; w++) {
-        const T pen = (T)rhop[elem_slot<C>(g, base, w)];
-        const int vi = 7 - w;
-        T acc[3] = {T(0), T(0), T(0)};
-#pragma unroll
-        for (int vj = 0; vj < 8; vj++) {
-            if (!WITH_SELF && vj == vi) continue;
-            const int nbj = (vj % 2 + w % 2) + 3 * (vj / 2 % 2 + w / 2 % 2) + 9 * (vj / 4 + w / 4);
-#pragma unroll
-            for (int r = 0; r < 3; r++)
-#pragma unroll
-                for (int q = 0; q < 3; q++) acc[r] += ke_at<T>((3 * vi + r) * 24 + 3 * vj + q) * u[nbj][q];
-        }
-#pragma unroll
-        for (int r = 0; r < 3; r++) KU[r] += pen * acc[r];
-        if (WANT_S) {
-#pragma unroll
-            for (int i = 0; i < 9; i++) S[i] += pen * ke_at<T>((3 * vi + i / 3) * 24 + 3 * vi + i % 3);
-        }
-    }
+    for (int i = 0; i < 9; i++) S[i] = T(0);
+    elem_all<T, C>(g, vb, K, B, acc, KU, S, std::make_integer_sequence<int, 8>{});
 }
 
+// GS update of the colour-C node with half-index base (Grid.cu:1007-1062)
 template <typename T, int C>
-__device__ __forceinline__ void fine_gs_body(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* __restrict__ rhop,
-                                             const uint8_t* __restrict__ vflag, bool reverse) {
-    int base;
-    if (!colour_thread<C>(g, blockIdx.x * blockDim.x + threadIdx.x, base)) return;
+__device__ __forceinline__ void gs_update(const Geom& g, int base, Vec3<T> U, Vec3<const T> F,
+                                          const uint8_t* __restrict__ vflag, const KeC<T>& K, const GsBufs& B,
+                                          bool reverse) {
     const int v = C * g.cs + base;
     const uint8_t fl = vflag[v];
     if (!(fl & NF_ACTIVE)) return;
-    T f[3] = {F.c[0][v], F.c[1][v], F.c[2][v]};
+    const T f[3] = {F.c[0][v], F.c[1][v], F.c[2][v]};
     if (fl & NF_FIXED) {
         // S = n_e I, KeU = 0 (Grid.cu:1007-1014)
         const T ne = (T)(fl >> 4);
@@ -139,10 +196,8 @@ __device__ __forceinline__ void fine_gs_body(const Geom& g, Vec3<T> U, Vec3<cons
         return;
     }
     T KU[3], S[9];
-    if (fl & NF_NEARFIX)
-        node_KU<T, C, false, true, true>(g, base, U.c[0], U.c[1], U.c[2], rhop, vflag, KU, S);
-    else
-        node_KU<T, C, false, false, true>(g, base, U.c[0], U.c[1], U.c[2], rhop, vflag, KU, S);
+    if (fl & NF_NEARFIX) gs_node<T, C, true>(g, base, K, B, KU, S);
+    else gs_node<T, C, false>(g, base, K, B, KU, S);
     T u[3] = {U.c[0][v], U.c[1][v], U.c[2][v]};
     // component-wise Gauss-Seidel with the 3x3 self block (Grid.cu:1056-1062)
     if (!reverse) {
@@ -157,119 +212,206 @@ __device__ __forceinline__ void fine_gs_body(const Geom& g, Vec3<T> U, Vec3<cons
     U.c[0][v] = u[0]; U.c[1][v] = u[1]; U.c[2][v] = u[2];
 }
 
+// Colours C and C^1 (same y, z parity, opposite x parity) couple only through
+// the x-neighbours in the same row, so one workgroup owns a whole row
+// (y, z) and relaxes its even-x nodes, then (after a barrier, which makes the
+// new values visible to the block) its odd-x nodes; REV swaps the two.
+// Every other neighbour belongs to a row of another colour pair and is
+// read-only during the launch. Four launches make a sweep, in the reference's
+// colour order 0..7 (or 7..0).
 template <typename T, int C>
-__global__ void __launch_bounds__(256) k_fine_gs(Geom g, Vec3<T> U, Vec3<const T> F, const float* __restrict__ rhop,
-                                                 const uint8_t* __restrict__ vflag, int reverse) {
-    fine_gs_body<T, C>(g, U, F, rhop, vflag, reverse != 0);
+__device__ __forceinline__ void gs_row_phase(const Geom& g, int j, int k, Vec3<T> U, Vec3<const T> F,
+                                             const uint8_t* __restrict__ vflag, const KeC<T>& K, const GsBufs& B,
+                                             bool reverse) {
+    const int nhx = (g.n[0] - (C & 1) + 1) >> 1;
+    const int rowbase = ((k + 1) * g.h[1] + (j + 1)) * g.h[0] + 1;
+    for (int i = threadIdx.x; i < nhx; i += blockDim.x) gs_update<T, C>(g, rowbase + i, U, F, vflag, K, B, reverse);
 }
 
-template <typename T>
-void launch_fine_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* rhop, const uint8_t* vflag, int colour,
-                    bool reverse, hipStream_t s) {
-    const int cx = colour & 1, cy = (colour >> 1) & 1, cz = colour >> 2;
-    const long long tot = (long long)((g.n[0] - cx + 1) >> 1) * ((g.n[1] - cy + 1) >> 1) * ((g.n[2] - cz + 1) >> 1);
-    if (tot <= 0) return;
-    dim3 grid((unsigned)((tot + 255) / 256));
-    switch (colour) {
-#define RT_CASE(c) case c: hipLaunchKernelGGL((k_fine_gs<T, c>), grid, dim3(256), 0, s, g, U, F, rhop, vflag, (int)reverse); break;
-        RT_CASE(0) RT_CASE(1) RT_CASE(2) RT_CASE(3) RT_CASE(4) RT_CASE(5) RT_CASE(6) RT_CASE(7)
-#undef RT_CASE
-    }
-    RT_HIP(hipGetLastError());
-}
-
-// ---------------------------------------------------------------------------
-// residual r = f - K u and apply y = K x (WS rules). One launch covers all
-// colours: blockIdx.y is the colour, so the compile-time offsets still apply.
-// ---------------------------------------------------------------------------
-template <typename T, int C, bool APPLY>
-__device__ __forceinline__ T fine_res_body(const Geom& g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R,
-                                          const float* __restrict__ rhop, const uint8_t* __restrict__ vflag) {
-    int base;
-    if (!colour_thread<C>(g, blockIdx.x * blockDim.x + threadIdx.x, base)) return T(0);
-    const int v = C * g.cs + base;
-    const uint8_t fl = vflag[v];
-    if (!(fl & NF_ACTIVE)) return T(0);
-    if (fl & NF_FIXED) {
-        // KU = 0 at fixed nodes (Grid.cu:1547-1549)
-        if (APPLY) { R.c[0][v] = T(0); R.c[1][v] = T(0); R.c[2][v] = T(0); return T(0); }
-        R.c[0][v] = F.c[0][v]; R.c[1][v] = F.c[1][v]; R.c[2][v] = F.c[2][v];
-        return T(0);
-    }
-    T KU[3], S[9];
-    if (fl & NF_NEARFIX)
-        node_KU<T, C, true, true, false>(g, base, U.c[0], U.c[1], U.c[2], rhop, vflag, KU, S);
-    else
-        node_KU<T, C, true, false, false>(g, base, U.c[0], U.c[1], U.c[2], rhop, vflag, KU, S);
-    if (APPLY) {
-        R.c[0][v] = KU[0]; R.c[1][v] = KU[1]; R.c[2][v] = KU[2];
-        return KU[0] * U.c[0][v] + KU[1] * U.c[1][v] + KU[2] * U.c[2][v];
-    }
-    R.c[0][v] = F.c[0][v] - KU[0]; R.c[1][v] = F.c[1][v] - KU[1]; R.c[2][v] = F.c[2][v] - KU[2];
-    return T(0);
-}
-
-__device__ __forceinline__ double block_sum_256(double v) {
-    __shared__ double red[4];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) red[wid] = v;
+template <typename T, int CY, int CZ, bool REV>
+__global__ void __launch_bounds__(256) k_fine_gs_pair(Geom g, Vec3<T> U, Vec3<const T> F,
+                                                      const float* __restrict__ rhop,
+                                                      const uint8_t* __restrict__ vflag, KeC<T> K) {
+    const int nhy = (g.n[1] - CY + 1) >> 1;
+    const int j = blockIdx.x % nhy, k = blockIdx.x / nhy;
+    const int nbytes = g.N * (int)sizeof(T);
+    const GsBufs B = {rsrc(U.c[0], nbytes), rsrc(U.c[1], nbytes), rsrc(U.c[2], nbytes), rsrc(rhop, g.N * 4),
+                      rsrc(vflag, g.N)};
+    constexpr int CA = (REV ? 1 : 0) | (CY << 1) | (CZ << 2);
+    gs_row_phase<T, CA>(g, j, k, U, F, vflag, K, B, REV);
     __syncthreads();
-    double t = 0;
-    if (threadIdx.x == 0) t = (red[0] + red[1]) + (red[2] + red[3]);
-    return t;
+    gs_row_phase<T, CA ^ 1>(g, j, k, U, F, vflag, K, B, REV);
 }
 
-template <typename T, bool APPLY>
-__global__ void __launch_bounds__(256) k_fine_res(Geom g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R,
-                                                  const float* __restrict__ rhop, const uint8_t* __restrict__ vflag,
-                                                  double* __restrict__ partials) {
-    T d = T(0);
-    switch (blockIdx.y) {
-#define RT_CASE(c) case c: d = fine_res_body<T, c, APPLY>(g, U, F, R, rhop, vflag); break;
-        RT_CASE(0) RT_CASE(1) RT_CASE(2) RT_CASE(3) RT_CASE(4) RT_CASE(5) RT_CASE(6) RT_CASE(7)
+// ---------------------------------------------------------------------------
+// LDS-staged row kernel. The workgroup first loads, with independent loads,
+// the 3x3 block of node rows around its row (u with supports zeroed, flags) and
+// the 2x2 element rows (rho^p) into LDS, then relaxes the row's two colours from
+// LDS; the first colour's new values go to LDS as well as to HBM. Latency is
+// paid once per row instead of once per neighbour.
+// LDS (dynamic): u[3][9][nx+2] T, rho[4][nx+1] float, flag[9][nx+2] uint8.
+// ---------------------------------------------------------------------------
+template <typename T, int... NBs>
+__device__ __forceinline__ void row_nbs(const T* su, int W, int xi, const KeC<T>& K, T (&acc)[8][3],
+                                        std::integer_sequence<int, NBs...>) {
+    // neighbour NB at (dx,dy,dz) = (NB%3-1, NB/3%3-1, NB/9-1): row r = (dz+1)*3 + (dy+1), column xi + dx
+    auto one = [&](auto nbc) {
+        constexpr int NB = decltype(nbc)::value;
+        if constexpr (NB != 13) {
+            constexpr int r = (NB / 9) * 3 + (NB / 3 % 3);
+            constexpr int dx = NB % 3 - 1;
+            const T u0 = su[(0 * 9 + r) * W + xi + dx];
+            const T u1 = su[(1 * 9 + r) * W + xi + dx];
+            const T u2 = su[(2 * 9 + r) * W + xi + dx];
+            nb_elems<T, NB>(acc, K, u0, u1, u2, std::make_integer_sequence<int, 8>{});
+        }
+    };
+    (one(std::integral_constant<int, NBs>{}), ...);
+}
+
+template <typename T, int... Ws>
+__device__ __forceinline__ void row_elems(const float* sr, int W1, int xi, const KeC<T>& K, const T (&acc)[8][3],
+                                          T KU[3], T S[9], std::integer_sequence<int, Ws...>) {
+    auto one = [&](auto wc) {
+        constexpr int Wd = decltype(wc)::value;
+        constexpr int vi = 7 - Wd;
+        // element at (x + (w&1) - 1, y + ((w>>1)&1) - 1, z + (w>>2) - 1); rho rows (ez, ey) -> 2*ez' + ey'
+        const T pen = (T)sr[(((Wd >> 2) * 2) + ((Wd >> 1) & 1)) * W1 + xi - 1 + (Wd & 1)];
+        KU[0] += pen * acc[Wd][0];
+        KU[1] += pen * acc[Wd][1];
+        KU[2] += pen * acc[Wd][2];
+        S[0] += pen * kc<ke_code_at((3 * vi + 0) * 24 + 3 * vi + 0)>(K);
+        S[1] += pen * kc<ke_code_at((3 * vi + 0) * 24 + 3 * vi + 1)>(K);
+        S[2] += pen * kc<ke_code_at((3 * vi + 0) * 24 + 3 * vi + 2)>(K);
+        S[3] += pen * kc<ke_code_at((3 * vi + 1) * 24 + 3 * vi + 0)>(K);
+        S[4] += pen * kc<ke_code_at((3 * vi + 1) * 24 + 3 * vi + 1)>(K);
+        S[5] += pen * kc<ke_code_at((3 * vi + 1) * 24 + 3 * vi + 2)>(K);
+        S[6] += pen * kc<ke_code_at((3 * vi + 2) * 24 + 3 * vi + 0)>(K);
+        S[7] += pen * kc<ke_code_at((3 * vi + 2) * 24 + 3 * vi + 1)>(K);
+        S[8] += pen * kc<ke_code_at((3 * vi + 2) * 24 + 3 * vi + 2)>(K);
+    };
+    (one(std::integral_constant<int, Ws>{}), ...);
+}
+
+template <typename T, int C>
+__device__ __forceinline__ void gs_row_lds_phase(const Geom& g, int y, int z, T* su, const float* sr,
+                                                 const uint8_t* sf, int W, Vec3<T> U, Vec3<const T> F,
+                                                 const KeC<T>& K, bool reverse) {
+    constexpr int cx = C & 1;
+    const int nx = g.n[0];
+    for (int x = 2 * (int)threadIdx.x + cx; x < nx; x += 2 * (int)blockDim.x) {
+        const int xi = x + 1;
+        const uint8_t fl = sf[4 * W + xi];
+        if (!(fl & NF_ACTIVE)) continue;
+        const int v = oct_p(g, x, y, z);
+        const T f[3] = {F.c[0][v], F.c[1][v], F.c[2][v]};
+        if (fl & NF_FIXED) {
+            const T ne = (T)(fl >> 4);
+            U.c[0][v] = f[0] / ne; U.c[1][v] = f[1] / ne; U.c[2][v] = f[2] / ne;
+            continue;   // stays 0 in LDS: supports contribute nothing to their neighbours
+        }
+        T acc[8][3];
+#pragma unroll
+        for (int w = 0; w < 8; w++) acc[w][0] = acc[w][1] = acc[w][2] = T(0);
+        row_nbs<T>(su, W, xi, K, acc, std::make_integer_sequence<int, 27>{});
+        T KU[3] = {T(0), T(0), T(0)}, S[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) S[i] = T(0);
+        row_elems<T>(sr, W - 1, xi, K, acc, KU, S, std::make_integer_sequence<int, 8>{});
+        T u[3] = {su[(0 * 9 + 4) * W + xi], su[(1 * 9 + 4) * W + xi], su[(2 * 9 + 4) * W + xi]};
+        if (!reverse) {
+            u[0] = (f[0] - S[1] * u[1] - S[2] * u[2] - KU[0]) / S[0];
+            u[1] = (f[1] - S[3] * u[0] - S[5] * u[2] - KU[1]) / S[4];
+            u[2] = (f[2] - S[6] * u[0] - S[7] * u[1] - KU[2]) / S[8];
+        } else {
+            u[2] = (f[2] - S[6] * u[0] - S[7] * u[1] - KU[2]) / S[8];
+            u[1] = (f[1] - S[3] * u[0] - S[5] * u[2] - KU[1]) / S[4];
+            u[0] = (f[0] - S[1] * u[1] - S[2] * u[2] - KU[0]) / S[0];
+        }
+        U.c[0][v] = u[0]; U.c[1][v] = u[1]; U.c[2][v] = u[2];
+        su[(0 * 9 + 4) * W + xi] = u[0]; su[(1 * 9 + 4) * W + xi] = u[1]; su[(2 * 9 + 4) * W + xi] = u[2];
+    }
+}
+
+template <typename T, int CY, int CZ, bool REV>
+__global__ void __launch_bounds__(256) k_fine_gs_row(Geom g, Vec3<T> U, Vec3<const T> F,
+                                                     const float* __restrict__ rhop,
+                                                     const uint8_t* __restrict__ vflag, KeC<T> K) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int W = nx + 2;
+    T* su = (T*)smem;
+    float* sr = (float*)(su + 27 * W);
+    uint8_t* sf = (uint8_t*)(sr + 4 * (W - 1));
+    const int nhy = (ny - CY + 1) >> 1;
+    const int y = 2 * (blockIdx.x % nhy) + CY, z = 2 * (blockIdx.x / nhy) + CZ;
+    // node rows (dy, dz) in {-1,0,1}^2, x in [-1, nx]
+    for (int idx = threadIdx.x; idx < 9 * W; idx += blockDim.x) {
+        const int r = idx / W, xi = idx - r * W;
+        const int x = xi - 1, yy = y + (r % 3) - 1, zz = z + (r / 3) - 1;
+        T a = T(0), b = T(0), c = T(0);
+        uint8_t fl = 0;
+        if (x >= 0 && x < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
+            const int s = oct_p(g, x, yy, zz);
+            fl = vflag[s];
+            if ((fl & NF_ACTIVE) && !(fl & NF_FIXED)) { a = U.c[0][s]; b = U.c[1][s]; c = U.c[2][s]; }
+        }
+        su[(0 * 9 + r) * W + xi] = a; su[(1 * 9 + r) * W + xi] = b; su[(2 * 9 + r) * W + xi] = c;
+        sf[r * W + xi] = fl;
+    }
+    // element rows (ey, ez) in {y-1, y} x {z-1, z}, x in [-1, nx-1]
+    for (int idx = threadIdx.x; idx < 4 * (W - 1); idx += blockDim.x) {
+        const int r = idx / (W - 1), xi = idx - r * (W - 1);
+        const int ex = xi - 1, ey = y - 1 + (r & 1), ez = z - 1 + (r >> 1);
+        float v = 0.f;
+        if (ex >= 0 && ex < nx - 1 && ey >= 0 && ey < ny - 1 && ez >= 0 && ez < nz - 1) v = rhop[oct_p(g, ex, ey, ez)];
+        sr[idx] = v;
+    }
+    __syncthreads();
+    constexpr int CA = (REV ? 1 : 0) | (CY << 1) | (CZ << 2);
+    gs_row_lds_phase<T, CA>(g, y, z, su, sr, sf, W, U, F, K, REV);
+    __syncthreads();
+    gs_row_lds_phase<T, CA ^ 1>(g, y, z, su, sr, sf, W, U, F, K, REV);
+}
+
+template <typename T>
+static size_t row_lds_bytes(const Geom& g) {
+    const size_t W = (size_t)g.n[0] + 2;
+    return 27 * W * sizeof(T) + 4 * (W - 1) * sizeof(float) + 9 * W;
+}
+
+// pair p = 0..3 relaxes colours (2p, 2p+1), or (2p+1, 2p) when reverse
+template <typename T>
+void launch_fine_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* rhop, const uint8_t* vflag, int pair,
+                    bool reverse, hipStream_t s) {
+    const int cy = pair & 1, cz = pair >> 1;
+    const long long rows = (long long)((g.n[1] - cy + 1) >> 1) * ((g.n[2] - cz + 1) >> 1);
+    if (rows <= 0) return;
+    dim3 grid((unsigned)rows);
+    const KeC<T> K = kec<T>();
+    const size_t lds = row_lds_bytes<T>(g);
+    if (lds <= 150 * 1024) {
+        switch (pair * 2 + (reverse ? 1 : 0)) {
+#define RT_CASE(p, r) case p * 2 + r: { auto kf = k_fine_gs_row<T, ((p) & 1), ((p) >> 1), (r) != 0>; \
+            RT_HIP(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+            hipLaunchKernelGGL(kf, grid, dim3(256), lds, s, g, U, F, rhop, vflag, K); } break;
+            RT_CASE(0, 0) RT_CASE(0, 1) RT_CASE(1, 0) RT_CASE(1, 1) RT_CASE(2, 0) RT_CASE(2, 1) RT_CASE(3, 0) RT_CASE(3, 1)
+#undef RT_CASE
+        }
+        RT_HIP(hipGetLastError());
+        return;
+    }
+    switch (pair * 2 + (reverse ? 1 : 0)) {
+#define RT_CASE(p, r) case p * 2 + r: hipLaunchKernelGGL((k_fine_gs_pair<T, ((p) & 1), ((p) >> 1), (r) != 0>), grid, dim3(256), 0, s, g, U, F, rhop, vflag, K); break;
+        RT_CASE(0, 0) RT_CASE(0, 1) RT_CASE(1, 0) RT_CASE(1, 1) RT_CASE(2, 0) RT_CASE(2, 1) RT_CASE(3, 0) RT_CASE(3, 1)
 #undef RT_CASE
     }
-    if (APPLY && partials) {
-        double t = block_sum_256((double)d);
-        if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = t;
-    }
-}
-
-static inline unsigned max_colour_blocks(const Geom& g) {
-    long long tot = (long long)((g.n[0] + 1) >> 1) * ((g.n[1] + 1) >> 1) * ((g.n[2] + 1) >> 1);
-    return (unsigned)((tot + 255) / 256);
-}
-
-template <typename T>
-void launch_fine_residual(const Geom& g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R, const float* rhop,
-                          const uint8_t* vflag, hipStream_t s) {
-    dim3 grid(max_colour_blocks(g), 8);
-    hipLaunchKernelGGL((k_fine_res<T, false>), grid, dim3(256), 0, s, g, U, F, R, rhop, vflag, (double*)nullptr);
     RT_HIP(hipGetLastError());
 }
 
-template <typename T>
-void launch_fine_apply(const Geom& g, Vec3<const T> X, Vec3<T> Y, const float* rhop, const uint8_t* vflag,
-                       double* partials, int* nparts, hipStream_t s) {
-    dim3 grid(max_colour_blocks(g), 8);
-    hipLaunchKernelGGL((k_fine_res<T, true>), grid, dim3(256), 0, s, g, X, X, Y, rhop, vflag, partials);
-    RT_HIP(hipGetLastError());
-    if (nparts) *nparts = (int)(grid.x * grid.y);
-}
-
-int fine_apply_partials(const Geom& g) { return (int)(max_colour_blocks(g) * 8); }
-
-#define RT_INST(T)                                                                                               \
-    template void launch_fine_gs<T>(const Geom&, Vec3<T>, Vec3<const T>, const float*, const uint8_t*, int, bool, \
-                                    hipStream_t);                                                                \
-    template void launch_fine_residual<T>(const Geom&, Vec3<const T>, Vec3<const T>, Vec3<T>, const float*,       \
-                                          const uint8_t*, hipStream_t);                                          \
-    template void launch_fine_apply<T>(const Geom&, Vec3<const T>, Vec3<T>, const float*, const uint8_t*, double*, \
-                                       int*, hipStream_t);
-RT_INST(float)
-RT_INST(double)
-#undef RT_INST
+template void launch_fine_gs<float>(const Geom&, Vec3<float>, Vec3<const float>, const float*, const uint8_t*, int,
+                                    bool, hipStream_t);
+template void launch_fine_gs<double>(const Geom&, Vec3<double>, Vec3<const double>, const float*, const uint8_t*, int,
+                                     bool, hipStream_t);
 
 }  // namespace rt

@@ -333,7 +333,7 @@ static void upload_flags(Hier& H, Level& L) {
 static void ensure_partials(Hier& H) {
     if (H.partials.p) return;
     int need = 65536;
-    need = std::max(need, fine_apply_partials(H.lv[0].g));
+    need = std::max(need, fine_apply_partials_modal(H.lv[0].g));
     for (auto& L : H.lv) need = std::max(need, (int)((L.g.N + 255) / 256));
     H.nparts_max = need;
     H.partials.alloc(sizeof(double) * (size_t)need);
@@ -380,21 +380,25 @@ template <> LevelVecs<float>& pcvecs<float>(Hier& H) { if (!H.pc32.ok()) H.pc32.
 template <typename T>
 static void gs(Hier& H, int li, Vec3<T> U, Vec3<const T> F, int n, bool reverse) {
     Level& L = H.lv[li];
-    for (int it = 0; it < n; it++)
-        for (int cc = 0; cc < 8; cc++) {
-            const int c = reverse ? 7 - cc : cc;
-            if (li == 0)
-                launch_fine_gs<T>(L.g, U, F, H.rhop.as<float>(), L.vflag.as<uint8_t>(), c, reverse, H.s);
-            else
-                launch_coarse_gs<T, T>(L.g, U, F, stencil<T>(L), L.vflag.as<uint8_t>(), c, reverse, H.s);
+    for (int it = 0; it < n; it++) {
+        if (li == 0) {
+            // four launches, each relaxing the colour pair (2p, 2p+1) row by row
+            for (int pp = 0; pp < 4; pp++)
+                launch_fine_gs<T>(L.g, U, F, H.rhop.as<float>(), L.vflag.as<uint8_t>(), reverse ? 3 - pp : pp,
+                                  reverse, H.s);
+        } else {
+            for (int cc = 0; cc < 8; cc++)
+                launch_coarse_gs<T, T>(L.g, U, F, stencil<T>(L), L.vflag.as<uint8_t>(), reverse ? 7 - cc : cc,
+                                       reverse, H.s);
         }
+    }
 }
 
 template <typename T>
 static void residual(Hier& H, int li, Vec3<const T> U, Vec3<const T> F, Vec3<T> R) {
     Level& L = H.lv[li];
     if (li == 0)
-        launch_fine_residual<T>(L.g, U, F, R, H.rhop.as<float>(), L.vflag.as<uint8_t>(), H.s);
+        launch_fine_residual_modal<T>(L.g, U, F, R, H.rhop.as<float>(), L.vflag.as<uint8_t>(), H.s);
     else
         launch_coarse_residual<T, T>(L.g, U, F, R, stencil<T>(L), L.vflag.as<uint8_t>(), H.s);
 }
@@ -595,7 +599,7 @@ template <typename TV>
 static double apply_dot(Hier& H, V3<TV>& X, V3<TV>& Y) {
     Level& L0 = H.lv[0];
     int np = 0;
-    launch_fine_apply<TV>(L0.g, X.cv(), Y.v(), H.rhop.as<float>(), L0.vflag.as<uint8_t>(), H.partials.as<double>(), &np,
+    launch_fine_apply_modal<TV>(L0.g, X.cv(), Y.v(), H.rhop.as<float>(), L0.vflag.as<uint8_t>(), H.partials.as<double>(), &np,
                           H.s);
     return sum_partials_host(H, np);
 }
@@ -605,7 +609,7 @@ static double true_rel_residual(Hier& H) {
     if (!H.r64.ok()) H.r64.alloc(L0.g.N);
     if (!H.q64.ok()) H.q64.alloc(L0.g.N);
     const size_t n = (size_t)3 * L0.g.N;
-    launch_fine_apply<double>(L0.g, H.x64.cv(), H.q64.v(), H.rhop.as<float>(), L0.vflag.as<uint8_t>(), nullptr,
+    launch_fine_apply_modal<double>(L0.g, H.x64.cv(), H.q64.v(), H.rhop.as<float>(), L0.vflag.as<uint8_t>(), nullptr,
                               nullptr, H.s);
     launch_sub_conv<double, double>(H.f64.data(), H.q64.data(), H.r64.data(), (double*)nullptr, n,
                                     H.partials.as<double>(), 1024, H.s);
@@ -630,8 +634,16 @@ static int pcg_core(Hier& H, V3<TV>& x, V3<TV>& b, double tol, int maxit, int pr
     if (!q.ok()) q.alloc(N);
     LevelVecs<TC>& pc = pcvecs<TC>(H);
     const int np = 1024;
-    // r = b - A x
-    launch_fine_apply<TV>(L0.g, x.cv(), q.v(), H.rhop.as<float>(), L0.vflag.as<uint8_t>(), nullptr, nullptr, H.s);
+    // r = b - A x, with the initial guess scaled by the energy-optimal factor
+    // (x.b)/(x.Ax): never worse than zero in the energy norm, and much better
+    // than the raw previous solution after a density update.
+    const double xAx = apply_dot<TV>(H, x, q);
+    if (xAx > 0) {
+        const double xb = dot<TV, TV>(H, x.data(), b.data(), n);
+        const double alpha = xb > 0 ? xb / xAx : 0.0;
+        launch_scale<TV>(x.data(), alpha, n, H.s);
+        launch_scale<TV>(q.data(), alpha, n, H.s);
+    }
     launch_sub_conv<TV, TC>(b.data(), q.data(), r.data(), pc.F.data(), n, H.partials.as<double>(), np, H.s);
     double rr = sum_partials_host(H, np);
     const double nb = std::sqrt(dot<TV, TV>(H, b.data(), b.data(), n));
@@ -955,6 +967,7 @@ int rt_set_material(rt_hier* h, double youngs, double poisson, float power_penal
         set_ke_constants(H.KE, H.s);
         set_ke_galerkin(H.KE, H.s);
         set_ke_simp(H.KE, H.s);
+        set_modal_constants(H.KE);
         std::vector<double> M;
         galerkin4_matrices(H.KE, M);
         H.dM.alloc(sizeof(double) * M.size());

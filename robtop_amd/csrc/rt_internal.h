@@ -67,15 +67,18 @@ template <typename T> struct Vec3 { T* c[3]; };
 // k_fine.hip
 void set_ke_constants(const double* ke576, hipStream_t s);
 template <typename T>
-void launch_fine_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* rhop, const uint8_t* vflag, int colour,
+void launch_fine_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* rhop, const uint8_t* vflag, int pair,
                     bool reverse, hipStream_t s);
+
+// k_fine_modal.hip (element-centric residual / apply in the sum-difference basis)
+void set_modal_constants(const double* ke576);
 template <typename T>
-void launch_fine_residual(const Geom& g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R, const float* rhop,
-                          const uint8_t* vflag, hipStream_t s);
+void launch_fine_residual_modal(const Geom& g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R, const float* rhop,
+                                const uint8_t* vflag, hipStream_t s);
 template <typename T>
-void launch_fine_apply(const Geom& g, Vec3<const T> X, Vec3<T> Y, const float* rhop, const uint8_t* vflag,
-                       double* partials, int* nparts, hipStream_t s);
-int fine_apply_partials(const Geom& g);
+void launch_fine_apply_modal(const Geom& g, Vec3<const T> X, Vec3<T> Y, const float* rhop, const uint8_t* vflag,
+                             double* partials, int* nparts, hipStream_t s);
+int fine_apply_partials_modal(const Geom& g);
 
 // k_coarse.hip
 template <typename T, typename TS>
@@ -110,6 +113,8 @@ template <typename T, typename TZ>
 void launch_sub_conv(const T* b, const T* q, T* r, TZ* rz, size_t n, double* partials, int nparts, hipStream_t s);
 template <typename T, typename TE>
 void launch_add_conv(T* x, const TE* e, size_t n, hipStream_t s);
+template <typename T>
+void launch_scale(T* x, double a, size_t n, hipStream_t s);
 void launch_ratio(const double* a, const double* b, double* out, hipStream_t s);
 template <typename T>
 void launch_box_to_oct(const Geom& g, const T* box, T* oct, int node_or_elem, hipStream_t s);
