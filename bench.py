@@ -34,6 +34,12 @@ def parse():
     ap.add_argument("--vcycles", type=int, default=20, help="graphless V-cycles timed for the roofline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-only", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="kernels only, for rocprofv3 counter runs: two fine sweeps, one V-cycle, then the "
+                         "calibration copy (256 MiB device-to-device)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="turn the counter_collection.csv files of a --profile run under rocprofv3 --pmc "
+                         "FETCH_SIZE / --pmc WRITE_SIZE into HBM bytes per fine-sweep launch (JSON)")
     return ap.parse_args()
 
 
@@ -107,11 +113,77 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+CALIB_BYTES = 256 * 1024 * 1024
+
+
+def profile_run(a):
+    """Kernel-only run for counter collection: the fine-level smoother (two
+    sweeps = 8 colour-pair launches), one V-cycle, then a plain copy of
+    CALIB_BYTES between two device buffers as the calibration dispatch."""
+    import torch
+    ex, ey, ez = (int(v) for v in a.size.split(","))
+    hg, *_ = build_problem(ex, ey, ez, a.seed)
+    hg.update_stencil()
+    log(f"profile {ex}x{ey}x{ez}: fine sweeps")
+    hg.time_fine_sweep(32, 2)
+    log("V-cycle")
+    hg.time_vcycle(32, a.pre, a.post, 1)
+    hg.synchronize()
+    log("calibration copy")
+    x = torch.ones(CALIB_BYTES // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    log("done")
+
+
+def pmc_summary(fetch_csv, write_csv, size):
+    """HBM bytes per fine-sweep launch from FETCH_SIZE / WRITE_SIZE (KiB per
+    dispatch). Each counter is scaled by what it reads for the calibration
+    copy: the largest copyBuffer dispatch, which moves CALIB_BYTES each way."""
+    import csv
+    import re
+    from collections import defaultdict
+
+    def per_kernel(path, counter):
+        agg, calib = defaultdict(list), 0.0
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            name = re.sub(r"\(.*", "", r["Kernel_Name"]).strip()
+            v = float(r["Counter_Value"]) * 1024.0
+            if "copyBuffer" in name:
+                calib = max(calib, v)
+            agg[name].append(v)
+        return agg, calib
+
+    fa, fc = per_kernel(fetch_csv, "FETCH_SIZE")
+    wa, wc = per_kernel(write_csv, "WRITE_SIZE")
+    kf, kw = fc / CALIB_BYTES, wc / CALIB_BYTES
+    gs = [k for k in fa if "k_fine_gs_pair" in k]
+    fetch = sum(sum(fa[k]) for k in gs) / sum(len(fa[k]) for k in gs) / kf
+    write = sum(sum(wa[k]) for k in gs) / sum(len(wa[k]) for k in gs) / kw
+    ex, ey, ez = size
+    nodes, elems = (ex + 1) * (ey + 1) * (ez + 1), ex * ey * ez
+    alg = (nodes * (9 * 4 + 1) + 4 * elems) / 4.0
+    return {"kernel": "k_fine_gs_pair", "size": list(size), "bytes_per_launch": fetch + write,
+            "fetch_bytes_per_launch": fetch, "write_bytes_per_launch": write,
+            "calibration": {"copy_bytes": CALIB_BYTES, "fetch_factor": kf, "write_factor": kw},
+            "algorithmic_bytes_per_launch": alg, "traffic_over_algorithmic": (fetch + write) / alg}
+
+
 def main():
     a = parse()
+    if a.pmc_summary:
+        print(json.dumps(pmc_summary(*a.pmc_summary, tuple(int(v) for v in a.size.split(",")))))
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if a.profile:
+        profile_run(a)
+        return
     ex, ey, ez = (int(v) for v in a.size.split(","))
     import robtop_amd as rt
     log(f"building {ex}x{ey}x{ez}")
@@ -157,9 +229,13 @@ def main():
     t_v = hg.time_vcycle(32, a.pre, a.post, a.vcycles)
     t_sweep = hg.time_fine_sweep(32, 10)
     B_V = algorithmic_vcycle_bytes(levels, a.pre, a.post, coarsest_dofs)
+    # a sweep is four launches, each relaxing two colours (half the rows of the box
+    # in y-z parity class, both x parities); each launch moves a quarter of the
+    # sweep's compulsory bytes: u (read + write), f, flag for its nodes and, over the
+    # sweep, rho^p once per element and u once per node as a neighbour
     sweep_bytes = levels[0]["nodes"] * (9 * 4 + 1) + 4 * levels[0]["elements"]
-    launch_bytes = sweep_bytes / 8.0            # one colour pass
-    launch_ms = t_sweep / 8.0
+    launch_bytes = sweep_bytes / 4.0
+    launch_ms = t_sweep / 4.0
     achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic_fine_gs.json")
@@ -191,7 +267,7 @@ def main():
         },
         "vcycle": {"ms": t_v, "algorithmic_bytes": B_V, "GBps": B_V / (t_v * 1e-3) / 1e9,
                    "frac_of_peak": B_V / (t_v * 1e-3) / 1e9 / HBM_PEAK_GBS, "pre": a.pre, "post": a.post},
-        "roofline": {"kernel": "k_fine_gs (one colour pass of the finest-level smoother)", "bound": "hbm",
+        "roofline": {"kernel": "k_fine_gs_pair (one colour-pair launch of the finest-level smoother)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": launch_bytes, "launch_ms": launch_ms},
     }

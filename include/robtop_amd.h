@@ -82,6 +82,11 @@ int rt_level_info(const rt_hier* h, int level, int* info);
 /* masks of the level's element box and node box (1 = solid / existing) */
 int rt_level_masks(const rt_hier* h, int level, uint8_t* eact_box, uint8_t* vact_box);
 int rt_num_active(const rt_hier* h, int level, size_t* n_elements, size_t* n_nodes);
+/* Surface-node mask of a level in its node box (1 = existing node on the
+ * surface). Replaces Grid::mark_surface_nodes_g (grid/Grid.cu:2005-2051),
+ * whose flag gates supports and loads in computeProjectionMatrix
+ * (grid/Grid.cpp:1291-1403). */
+int rt_surface_nodes_box(const rt_hier* h, int level, uint8_t* surf_box);
 
 /* --- material and boundary conditions ---------------------------------- */
 
@@ -140,6 +145,9 @@ int rt_get_level_vec(const rt_hier* h, int level, int which, double* v_box);
 int rt_gs_relax(rt_hier* h, int level, int nsweeps, int reverse);
 /* Grid::update_residual (grid/Grid.cu:1644-1678) */
 int rt_update_residual(rt_hier* h, int level);
+/* Grid::applyK (grid/Grid.cu:2519-2532): f = K(rho) u on the finest node box
+ * (FP64, 3 x n), support neighbours taken as zero; support nodes return u. */
+int rt_apply_k(rt_hier* h, const double* u_box, double* f_box);
 /* Grid::restrict_residual (grid/Grid.cu:1796-1816): level = the coarse level */
 int rt_restrict_residual(rt_hier* h, int level);
 /* Grid::prolongate_correction (grid/Grid.cu:1898-1915): level = the fine level */
@@ -186,6 +194,32 @@ int rt_update_densities(rt_hier* h, float vgoal, float step, float damp, float r
 /* mean density over solid elements */
 int rt_volume(rt_hier* h, double* v);
 
+/* --- worst-case load search --------------------------------------------- */
+
+/* Load nodes for the worst-case search. Replaces setLoadNodes
+ * (optimization/projection.cpp:129-269) for the with-support modes: load_box
+ * flags load nodes on the finest node box (supports and absent nodes are
+ * dropped); normals_box (FP64, 3 x n) gives outward normals for the
+ * constrained-direction mode wscf (only the normal component of a load is
+ * admissible), or is NULL for free loads (wsff). */
+int rt_set_load_nodes(rt_hier* h, const uint8_t* load_box, const double* normals_box);
+int rt_num_load_nodes(const rt_hier* h, int* n);
+/* forceProject (optimization/projection.cu:165-231): the finest force is
+ * cleared off the load nodes and, with normals, reduced to its normal part. */
+int rt_force_project(rt_hier* h);
+/* Grid::randForce (grid/Grid.cu:2242-2245): force components uniform in
+ * [-1, 1) on existing nodes, from a counter-based generator seeded by `seed`
+ * (the reference seeds cuRAND from the clock). */
+int rt_rand_force(rt_hier* h, uint64_t seed);
+/* modifiedPM (optimization/optimization.cpp:123-261), with-support branch,
+ * starting from the current force: project and normalise, u = 0, then per
+ * step one V(1,1) cycle (precision of rt_set_vcycle_precision) and
+ * f <- P u / |P u|, until the load change |fs_new - fs_old| / |fs_old| is at
+ * most 1e-4 and the cycle's relative residual at most 1e-2, or max_itn steps.
+ * The worst load and its displacement stay in the force / displacement;
+ * returns the worst compliance f.u and the number of V-cycles. */
+int rt_modified_pm(rt_hier* h, int max_itn, double* worst_compliance, int* vcycles);
+
 /* --- voxelizer ----------------------------------------------------------- */
 
 /* voxelize_mesh (voxelizer/voxelize_main.cpp:344-404): padded cube of side
@@ -202,8 +236,8 @@ int rt_synchronize(rt_hier* h);
  * returns the mean milliseconds per cycle measured with HIP events on the
  * handle's stream. */
 int rt_time_vcycle(rt_hier* h, int bits, int pre, int post, int n, double* ms_per_cycle);
-/* Times n full fine-level smoothing sweeps (8 colour passes) of precision
- * `bits`; returns mean ms per sweep. */
+/* Times n full fine-level smoothing sweeps (colours 0..7, run as four
+ * colour-pair launches) of precision `bits`; returns mean ms per sweep. */
 int rt_time_fine_sweep(rt_hier* h, int bits, int n, double* ms_per_sweep);
 
 #ifdef __cplusplus

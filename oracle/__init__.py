@@ -66,6 +66,9 @@ def lib():
             "or_sensitivity": ([P, P], None),
             "or_filter": ([P, F, P, P], None),
             "or_oc_update": ([P, F, F, F, F, P, P], F),
+            "or_set_load_nodes": ([P, P, P], None),
+            "or_force_project": ([P], None),
+            "or_modified_pm": ([P, I, P], D),
             "or_bitsat_rank": ([P, I, ctypes.c_long], I),
             "or_voxelize_mesh": ([P, I, P, I, I, P, I, P, P], I),
             "or_voxelize_triangles": ([P, I, P, P, I, P], None),
@@ -257,6 +260,25 @@ class Hierarchy:
         out = np.zeros_like(g)
         lib().or_filter(self.h, r, _p(g), _p(out))
         return out
+
+    def set_load_nodes(self, load, normals=None):
+        a = np.ascontiguousarray(load, dtype=np.uint8)
+        if normals is None:
+            lib().or_set_load_nodes(self.h, _p(a), None)
+        else:
+            nb = np.ascontiguousarray(normals, dtype=np.float64)
+            lib().or_set_load_nodes(self.h, _p(a), _p(nb))
+
+    def force_project(self):
+        lib().or_force_project(self.h)
+
+    def modified_pm(self, max_itn=500):
+        """(worst compliance, V-cycles) from the force in F."""
+        st = np.zeros(1, np.int32)
+        c = lib().or_modified_pm(self.h, max_itn, _p(st))
+        if np.isnan(c):
+            raise RuntimeError(lib().or_last_error().decode())
+        return c, int(st[0])
 
     def oc_update(self, Vgoal, step, damp, rhomin, g):
         g = np.ascontiguousarray(g, dtype=np.float32)

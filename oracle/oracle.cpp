@@ -65,6 +65,9 @@ struct Hier {
     std::vector<float> rho;     // finest-level element densities (box layout)
     std::vector<float> rhop;    // rho^p per element (FP32 powf, Grid.cu:834), or set directly
     std::vector<uint8_t> shell; // finest-level shell flags (box layout)
+    std::vector<uint8_t> load;  // finest-level load-node flags (box layout)
+    std::vector<double> nrm[3]; // unit outward normals at load nodes (constrained-direction mode)
+    bool normals = false;
     // coarsest dense system
     std::vector<int> lastrow;   // node -> row block, -1 if absent
     int nlastrows = 0;
@@ -582,6 +585,63 @@ double vcycle(Hier* h, int pre, int post, bool symmetric) {
     return v3norm(h->lv[0].R) / v3norm(h->lv[0].F);
 }
 
+// forceProject, with-support modes (projection.cu:165-231, projection.cpp:289-360):
+// the force is cleared off the load nodes; with normals (wscf) the tangential
+// part f - Nd^T Nd f is removed, Nd holding the orthonormal tangent pair of
+// each node (projection.cpp:176-197), which leaves (f.n) n
+void force_project(Hier* h, std::vector<double>* f) {
+    Level& L = h->lv[0];
+    for (size_t v = 0; v < L.nn(); v++) {
+        if (!h->load[v]) { for (int c = 0; c < 3; c++) f[c][v] = 0; continue; }
+        if (h->normals) {
+            const double d = f[0][v] * h->nrm[0][v] + f[1][v] * h->nrm[1][v] + f[2][v] * h->nrm[2][v];
+            for (int c = 0; c < 3; c++) f[c][v] = d * h->nrm[c][v];
+        }
+    }
+}
+
+// modifiedPM, with-support branch (optimization.cpp:123-261) from the force
+// already in F: project, unitize, u = 0; then one V(1,1) per step, F <- unit
+// projection of U, fch = |fs_new - fs_old| / |fs_old| over the load nodes
+// (Grid.cu:2255-2286), until fch <= 1e-4 and rel_res <= 1e-2 or 500 steps.
+// Returns the worst compliance F.U; *steps = number of V-cycles.
+double modified_pm(Hier* h, int max_itn, int* steps) {
+    Level& L = h->lv[0];
+    const size_t nn = L.nn();
+    auto unitize = [&]() {
+        const double n = v3norm(L.F);
+        for (int c = 0; c < 3; c++) for (size_t v = 0; v < nn; v++) L.F[c][v] /= n;
+    };
+    force_project(h, L.F);
+    unitize();
+    for (int c = 0; c < 3; c++) std::fill(L.U[c].begin(), L.U[c].end(), 0.0);
+    std::vector<double> fs[3];
+    for (int c = 0; c < 3; c++) fs[c] = L.F[c];
+    double fch = 1, rel = 1;
+    int itn = 0, nv = 0;
+    while (itn++ < max_itn && (fch > 1e-4 || rel > 1e-2)) {
+        rel = vcycle(h, 1, 1, false);
+        nv++;
+        if (rel > 1e4) throw std::runtime_error("modified power method diverged");
+        for (int c = 0; c < 3; c++) L.F[c] = L.U[c];
+        force_project(h, L.F);
+        unitize();
+        double d = 0, o = 0;
+        for (int c = 0; c < 3; c++)
+            for (size_t v = 0; v < nn; v++) {
+                if (!h->load[v]) continue;
+                d += (L.F[c][v] - fs[c][v]) * (L.F[c][v] - fs[c][v]);
+                o += fs[c][v] * fs[c][v];
+            }
+        fch = std::sqrt(d) / std::sqrt(o);
+        for (int c = 0; c < 3; c++) fs[c] = L.F[c];
+    }
+    *steps = nv;
+    double cw = 0;
+    for (int c = 0; c < 3; c++) for (size_t v = 0; v < nn; v++) cw += L.F[c][v] * L.U[c][v];
+    return cw;
+}
+
 // Sensitivity: optimization.cu:43-107, summed per element
 // g_e = -sum_v u_v . (penal * KE[v] u_e), penal = p * powf(rho, p-1) (FP32)
 void sensitivity(Hier* h, float* g) {
@@ -834,6 +894,25 @@ double or_compliance(void* hp) {   // Grid.cpp:1976-1979: F . U
     return s;
 }
 void or_sensitivity(void* hp, float* g) { sensitivity((Hier*)hp, g); }
+// load nodes (flags on the finest node box; supports excluded) and, if given,
+// their normals (3 x nn), normalised here
+void or_set_load_nodes(void* hp, const uint8_t* load, const double* normals) {
+    Hier* h = (Hier*)hp; Level& L = h->lv[0]; const size_t nn = L.nn();
+    h->load.assign(nn, 0);
+    for (size_t v = 0; v < nn; v++) h->load[v] = (load[v] && L.vact[v] && !L.vfix[v]) ? 1 : 0;
+    h->normals = normals != nullptr;
+    for (int c = 0; c < 3; c++) h->nrm[c].assign(nn, 0.0);
+    if (normals)
+        for (size_t v = 0; v < nn; v++) {
+            const double a = normals[v], b = normals[nn + v], d = normals[2 * nn + v];
+            const double l = std::sqrt(a * a + b * b + d * d);
+            if (h->load[v] && l > 0) { h->nrm[0][v] = a / l; h->nrm[1][v] = b / l; h->nrm[2][v] = d / l; }
+        }
+}
+void or_force_project(void* hp) { Hier* h = (Hier*)hp; force_project(h, h->lv[0].F); }
+double or_modified_pm(void* hp, int max_itn, int* steps) {
+    try { return modified_pm((Hier*)hp, max_itn, steps); } catch (std::exception& e) { g_err = e.what(); return NAN; }
+}
 void or_filter(void* hp, float r, const float* gin, float* gout) { filter((Hier*)hp, r, gin, gout); }
 float or_oc_update(void* hp, float Vgoal, float step, float damp, float rhomin, const float* g, int* iters) {
     return oc_update((Hier*)hp, Vgoal, step, damp, rhomin, g, iters);

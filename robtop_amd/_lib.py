@@ -46,6 +46,7 @@ SIGNATURES = {
     "rt_level_info": ([P, I, P], I),
     "rt_level_masks": ([P, I, P, P], I),
     "rt_num_active": ([P, I, P, P], I),
+    "rt_surface_nodes_box": ([P, I, P], I),
     "rt_set_material": ([P, D, D, F], I),
     "rt_get_template_matrix": ([P, P], I),
     "rt_set_support_box": ([P, P], I),
@@ -67,6 +68,7 @@ SIGNATURES = {
     "rt_get_level_vec": ([P, I, I, P], I),
     "rt_gs_relax": ([P, I, I, I], I),
     "rt_update_residual": ([P, I], I),
+    "rt_apply_k": ([P, P, P], I),
     "rt_restrict_residual": ([P, I], I),
     "rt_prolongate_correction": ([P, I], I),
     "rt_reset_displacement": ([P, I], I),
@@ -85,6 +87,11 @@ SIGNATURES = {
     "rt_update_densities": ([P, F, F, F, F, P, P], I),
     "rt_volume": ([P, P], I),
     "rt_voxelize_mesh": ([P, I, P, I, I, P, SZ, P, P], I),
+    "rt_set_load_nodes": ([P, P, P], I),
+    "rt_num_load_nodes": ([P, P], I),
+    "rt_force_project": ([P], I),
+    "rt_rand_force": ([P, ctypes.c_uint64], I),
+    "rt_modified_pm": ([P, I, P, P], I),
     "rt_synchronize": ([P], I),
     "rt_time_vcycle": ([P, I, I, I, I, P], I),
     "rt_time_fine_sweep": ([P, I, I, P], I),

@@ -5,7 +5,7 @@ FP32 instructions disabled; the voxelizer additionally with FP contraction off
 (its bit pattern depends on exact FP32 rounding). Objects go to
 robtop_amd/_build/, the shared library to robtop_amd/librobtop_amd.so.
 
-    python -m robtop_amd.build          # library + oracle
+    python -m robtop_amd.build          # library + oracle + tests/cpp programs
 """
 import concurrent.futures as cf
 import os
@@ -30,6 +30,7 @@ UNITS = {
     "k_coarse.hip": [],
     "k_galerkin.hip": [],
     "k_simp.hip": [],
+    "k_worst.hip": [],
     "k_voxel.hip": ["-ffp-contract=off"],
     "rt_host.cpp": ["-x", "hip"],
 }
@@ -66,6 +67,22 @@ def build_library(jobs=None):
     return LIB
 
 
+def build_cpp_programs():
+    """C++ programs under tests/cpp that drive the library through the C++
+    facade (include/robtop_amd.hpp); each X.cpp becomes the executable X."""
+    d = os.path.join(ROOT, "tests", "cpp")
+    out = []
+    for src in sorted(f for f in os.listdir(d) if f.endswith(".cpp")):
+        exe = os.path.join(d, os.path.splitext(src)[0])
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"),
+               os.path.join(d, src), "-L", HERE, "-lrobtop_amd", "-Wl,-rpath,$ORIGIN/../../robtop_amd", "-o", exe]
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0 or p.stderr.strip():
+            raise RuntimeError(f"g++ failed for {src}:\n{p.stderr}")
+        out.append(exe)
+    return out
+
+
 def build_oracle():
     p = subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")], capture_output=True, text=True)
     if p.returncode != 0:
@@ -76,3 +93,4 @@ def build_oracle():
 if __name__ == "__main__":
     print(build_library())
     print(build_oracle())
+    print("\n".join(build_cpp_programs()))

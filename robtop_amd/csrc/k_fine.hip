@@ -20,7 +20,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <type_traits>
 #include <utility>
 
 namespace rt {
@@ -233,7 +232,8 @@ __global__ void __launch_bounds__(256) k_fine_gs_pair(Geom g, Vec3<T> U, Vec3<co
                                                       const float* __restrict__ rhop,
                                                       const uint8_t* __restrict__ vflag, KeC<T> K) {
     const int nhy = (g.n[1] - CY + 1) >> 1;
-    const int j = blockIdx.x % nhy, k = blockIdx.x / nhy;
+    const int row = xcd_row(blockIdx.x, gridDim.x);
+    const int j = row % nhy, k = row / nhy;
     const int nbytes = g.N * (int)sizeof(T);
     const GsBufs B = {rsrc(U.c[0], nbytes), rsrc(U.c[1], nbytes), rsrc(U.c[2], nbytes), rsrc(rhop, g.N * 4),
                       rsrc(vflag, g.N)};
@@ -241,143 +241,6 @@ __global__ void __launch_bounds__(256) k_fine_gs_pair(Geom g, Vec3<T> U, Vec3<co
     gs_row_phase<T, CA>(g, j, k, U, F, vflag, K, B, REV);
     __syncthreads();
     gs_row_phase<T, CA ^ 1>(g, j, k, U, F, vflag, K, B, REV);
-}
-
-// ---------------------------------------------------------------------------
-// LDS-staged row kernel. The workgroup first loads, with independent loads,
-// the 3x3 block of node rows around its row (u with supports zeroed, flags) and
-// the 2x2 element rows (rho^p) into LDS, then relaxes the row's two colours from
-// LDS; the first colour's new values go to LDS as well as to HBM. Latency is
-// paid once per row instead of once per neighbour.
-// LDS (dynamic): u[3][9][nx+2] T, rho[4][nx+1] float, flag[9][nx+2] uint8.
-// ---------------------------------------------------------------------------
-template <typename T, int... NBs>
-__device__ __forceinline__ void row_nbs(const T* su, int W, int xi, const KeC<T>& K, T (&acc)[8][3],
-                                        std::integer_sequence<int, NBs...>) {
-    // neighbour NB at (dx,dy,dz) = (NB%3-1, NB/3%3-1, NB/9-1): row r = (dz+1)*3 + (dy+1), column xi + dx
-    auto one = [&](auto nbc) {
-        constexpr int NB = decltype(nbc)::value;
-        if constexpr (NB != 13) {
-            constexpr int r = (NB / 9) * 3 + (NB / 3 % 3);
-            constexpr int dx = NB % 3 - 1;
-            const T u0 = su[(0 * 9 + r) * W + xi + dx];
-            const T u1 = su[(1 * 9 + r) * W + xi + dx];
-            const T u2 = su[(2 * 9 + r) * W + xi + dx];
-            nb_elems<T, NB>(acc, K, u0, u1, u2, std::make_integer_sequence<int, 8>{});
-        }
-    };
-    (one(std::integral_constant<int, NBs>{}), ...);
-}
-
-template <typename T, int... Ws>
-__device__ __forceinline__ void row_elems(const float* sr, int W1, int xi, const KeC<T>& K, const T (&acc)[8][3],
-                                          T KU[3], T S[9], std::integer_sequence<int, Ws...>) {
-    auto one = [&](auto wc) {
-        constexpr int Wd = decltype(wc)::value;
-        constexpr int vi = 7 - Wd;
-        // element at (x + (w&1) - 1, y + ((w>>1)&1) - 1, z + (w>>2) - 1); rho rows (ez, ey) -> 2*ez' + ey'
-        const T pen = (T)sr[(((Wd >> 2) * 2) + ((Wd >> 1) & 1)) * W1 + xi - 1 + (Wd & 1)];
-        KU[0] += pen * acc[Wd][0];
-        KU[1] += pen * acc[Wd][1];
-        KU[2] += pen * acc[Wd][2];
-        S[0] += pen * kc<ke_code_at((3 * vi + 0) * 24 + 3 * vi + 0)>(K);
-        S[1] += pen * kc<ke_code_at((3 * vi + 0) * 24 + 3 * vi + 1)>(K);
-        S[2] += pen * kc<ke_code_at((3 * vi + 0) * 24 + 3 * vi + 2)>(K);
-        S[3] += pen * kc<ke_code_at((3 * vi + 1) * 24 + 3 * vi + 0)>(K);
-        S[4] += pen * kc<ke_code_at((3 * vi + 1) * 24 + 3 * vi + 1)>(K);
-        S[5] += pen * kc<ke_code_at((3 * vi + 1) * 24 + 3 * vi + 2)>(K);
-        S[6] += pen * kc<ke_code_at((3 * vi + 2) * 24 + 3 * vi + 0)>(K);
-        S[7] += pen * kc<ke_code_at((3 * vi + 2) * 24 + 3 * vi + 1)>(K);
-        S[8] += pen * kc<ke_code_at((3 * vi + 2) * 24 + 3 * vi + 2)>(K);
-    };
-    (one(std::integral_constant<int, Ws>{}), ...);
-}
-
-template <typename T, int C>
-__device__ __forceinline__ void gs_row_lds_phase(const Geom& g, int y, int z, T* su, const float* sr,
-                                                 const uint8_t* sf, int W, Vec3<T> U, Vec3<const T> F,
-                                                 const KeC<T>& K, bool reverse) {
-    constexpr int cx = C & 1;
-    const int nx = g.n[0];
-    for (int x = 2 * (int)threadIdx.x + cx; x < nx; x += 2 * (int)blockDim.x) {
-        const int xi = x + 1;
-        const uint8_t fl = sf[4 * W + xi];
-        if (!(fl & NF_ACTIVE)) continue;
-        const int v = oct_p(g, x, y, z);
-        const T f[3] = {F.c[0][v], F.c[1][v], F.c[2][v]};
-        if (fl & NF_FIXED) {
-            const T ne = (T)(fl >> 4);
-            U.c[0][v] = f[0] / ne; U.c[1][v] = f[1] / ne; U.c[2][v] = f[2] / ne;
-            continue;   // stays 0 in LDS: supports contribute nothing to their neighbours
-        }
-        T acc[8][3];
-#pragma unroll
-        for (int w = 0; w < 8; w++) acc[w][0] = acc[w][1] = acc[w][2] = T(0);
-        row_nbs<T>(su, W, xi, K, acc, std::make_integer_sequence<int, 27>{});
-        T KU[3] = {T(0), T(0), T(0)}, S[9];
-#pragma unroll
-        for (int i = 0; i < 9; i++) S[i] = T(0);
-        row_elems<T>(sr, W - 1, xi, K, acc, KU, S, std::make_integer_sequence<int, 8>{});
-        T u[3] = {su[(0 * 9 + 4) * W + xi], su[(1 * 9 + 4) * W + xi], su[(2 * 9 + 4) * W + xi]};
-        if (!reverse) {
-            u[0] = (f[0] - S[1] * u[1] - S[2] * u[2] - KU[0]) / S[0];
-            u[1] = (f[1] - S[3] * u[0] - S[5] * u[2] - KU[1]) / S[4];
-            u[2] = (f[2] - S[6] * u[0] - S[7] * u[1] - KU[2]) / S[8];
-        } else {
-            u[2] = (f[2] - S[6] * u[0] - S[7] * u[1] - KU[2]) / S[8];
-            u[1] = (f[1] - S[3] * u[0] - S[5] * u[2] - KU[1]) / S[4];
-            u[0] = (f[0] - S[1] * u[1] - S[2] * u[2] - KU[0]) / S[0];
-        }
-        U.c[0][v] = u[0]; U.c[1][v] = u[1]; U.c[2][v] = u[2];
-        su[(0 * 9 + 4) * W + xi] = u[0]; su[(1 * 9 + 4) * W + xi] = u[1]; su[(2 * 9 + 4) * W + xi] = u[2];
-    }
-}
-
-template <typename T, int CY, int CZ, bool REV>
-__global__ void __launch_bounds__(256) k_fine_gs_row(Geom g, Vec3<T> U, Vec3<const T> F,
-                                                     const float* __restrict__ rhop,
-                                                     const uint8_t* __restrict__ vflag, KeC<T> K) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-    const int W = nx + 2;
-    T* su = (T*)smem;
-    float* sr = (float*)(su + 27 * W);
-    uint8_t* sf = (uint8_t*)(sr + 4 * (W - 1));
-    const int nhy = (ny - CY + 1) >> 1;
-    const int y = 2 * (blockIdx.x % nhy) + CY, z = 2 * (blockIdx.x / nhy) + CZ;
-    // node rows (dy, dz) in {-1,0,1}^2, x in [-1, nx]
-    for (int idx = threadIdx.x; idx < 9 * W; idx += blockDim.x) {
-        const int r = idx / W, xi = idx - r * W;
-        const int x = xi - 1, yy = y + (r % 3) - 1, zz = z + (r / 3) - 1;
-        T a = T(0), b = T(0), c = T(0);
-        uint8_t fl = 0;
-        if (x >= 0 && x < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
-            const int s = oct_p(g, x, yy, zz);
-            fl = vflag[s];
-            if ((fl & NF_ACTIVE) && !(fl & NF_FIXED)) { a = U.c[0][s]; b = U.c[1][s]; c = U.c[2][s]; }
-        }
-        su[(0 * 9 + r) * W + xi] = a; su[(1 * 9 + r) * W + xi] = b; su[(2 * 9 + r) * W + xi] = c;
-        sf[r * W + xi] = fl;
-    }
-    // element rows (ey, ez) in {y-1, y} x {z-1, z}, x in [-1, nx-1]
-    for (int idx = threadIdx.x; idx < 4 * (W - 1); idx += blockDim.x) {
-        const int r = idx / (W - 1), xi = idx - r * (W - 1);
-        const int ex = xi - 1, ey = y - 1 + (r & 1), ez = z - 1 + (r >> 1);
-        float v = 0.f;
-        if (ex >= 0 && ex < nx - 1 && ey >= 0 && ey < ny - 1 && ez >= 0 && ez < nz - 1) v = rhop[oct_p(g, ex, ey, ez)];
-        sr[idx] = v;
-    }
-    __syncthreads();
-    constexpr int CA = (REV ? 1 : 0) | (CY << 1) | (CZ << 2);
-    gs_row_lds_phase<T, CA>(g, y, z, su, sr, sf, W, U, F, K, REV);
-    __syncthreads();
-    gs_row_lds_phase<T, CA ^ 1>(g, y, z, su, sr, sf, W, U, F, K, REV);
-}
-
-template <typename T>
-static size_t row_lds_bytes(const Geom& g) {
-    const size_t W = (size_t)g.n[0] + 2;
-    return 27 * W * sizeof(T) + 4 * (W - 1) * sizeof(float) + 9 * W;
 }
 
 // pair p = 0..3 relaxes colours (2p, 2p+1), or (2p+1, 2p) when reverse
@@ -389,18 +252,6 @@ void launch_fine_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* rhop
     if (rows <= 0) return;
     dim3 grid((unsigned)rows);
     const KeC<T> K = kec<T>();
-    const size_t lds = row_lds_bytes<T>(g);
-    if (lds <= 150 * 1024) {
-        switch (pair * 2 + (reverse ? 1 : 0)) {
-#define RT_CASE(p, r) case p * 2 + r: { auto kf = k_fine_gs_row<T, ((p) & 1), ((p) >> 1), (r) != 0>; \
-            RT_HIP(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
-            hipLaunchKernelGGL(kf, grid, dim3(256), lds, s, g, U, F, rhop, vflag, K); } break;
-            RT_CASE(0, 0) RT_CASE(0, 1) RT_CASE(1, 0) RT_CASE(1, 1) RT_CASE(2, 0) RT_CASE(2, 1) RT_CASE(3, 0) RT_CASE(3, 1)
-#undef RT_CASE
-        }
-        RT_HIP(hipGetLastError());
-        return;
-    }
     switch (pair * 2 + (reverse ? 1 : 0)) {
 #define RT_CASE(p, r) case p * 2 + r: hipLaunchKernelGGL((k_fine_gs_pair<T, ((p) & 1), ((p) >> 1), (r) != 0>), grid, dim3(256), 0, s, g, U, F, rhop, vflag, K); break;
         RT_CASE(0, 0) RT_CASE(0, 1) RT_CASE(1, 0) RT_CASE(1, 1) RT_CASE(2, 0) RT_CASE(2, 1) RT_CASE(3, 0) RT_CASE(3, 1)

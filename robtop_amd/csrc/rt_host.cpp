@@ -110,6 +110,11 @@ struct Hier {
     int n3 = 0;
     DBuf fixce;
     int nfixce = 0;
+    // worst-case load search: load-node slots, unit normals (constrained
+    // direction), projected force and the previous step's force, all [3][nload]
+    DBuf load_slots, load_nrm, pm_fs, pm_old;
+    int nload = 0;
+    bool load_normals = false;
     int vprec = 64;
     bool stencil_ready = false, material_set = false;
     ~Hier() {
@@ -458,6 +463,13 @@ static double dot(Hier& H, const TA* a, const TB* b, size_t n) {
     return v;
 }
 
+static double sum_partials_at(Hier& H, const double* p, int np) {
+    launch_sum_partials(p, np, H.scal.as<double>(), H.s);
+    double v;
+    RT_HIP(hipMemcpyAsync(&v, H.scal.p, sizeof(double), hipMemcpyDeviceToHost, H.s));
+    RT_HIP(hipStreamSynchronize(H.s));
+    return v;
+}
 static double sum_partials_host(Hier& H, int np) {
     launch_sum_partials(H.partials.as<double>(), np, H.scal.as<double>(), H.s);
     double v;
@@ -949,6 +961,39 @@ int rt_level_masks(const rt_hier* h, int level, uint8_t* eact_box, uint8_t* vact
     })
 }
 
+// mark_surface_nodes_kernel (Grid.cu:2005-2039): a node is on the surface when
+// along some axis no pair of solid elements faces each other across it
+int rt_surface_nodes_box(const rt_hier* h, int level, uint8_t* surf_box) {
+    RT_TRY({
+        need(surf_box);
+        Hier& H = H_(h);
+        Level& L = LV(H, level);
+        const int nx = L.g.n[0], ny = L.g.n[1], nz = L.g.n[2];
+        const int ex = L.e[0], ey = L.e[1], ez = L.e[2];
+        auto solid = [&](int x, int y, int z) {
+            return x >= 0 && y >= 0 && z >= 0 && x < ex && y < ey && z < ez &&
+                   L.eact[((size_t)z * ey + y) * ex + x] != 0;
+        };
+        for (int z = 0; z < nz; z++)
+            for (int y = 0; y < ny; y++)
+                for (int x = 0; x < nx; x++) {
+                    const size_t v = ((size_t)z * ny + y) * nx + x;
+                    if (!L.vact[v]) { surf_box[v] = 0; continue; }
+                    bool s[2][2][2];
+                    for (int w = 0; w < 8; w++)
+                        s[w & 1][(w >> 1) & 1][w >> 2] = solid(x + (w & 1) - 1, y + ((w >> 1) & 1) - 1, z + (w >> 2) - 1);
+                    bool ax[3] = {false, false, false};
+                    for (int i = 0; i < 2; i++)
+                        for (int j = 0; j < 2; j++) {
+                            ax[0] |= s[0][i][j] && s[1][i][j];
+                            ax[1] |= s[i][0][j] && s[i][1][j];
+                            ax[2] |= s[i][j][0] && s[i][j][1];
+                        }
+                    surf_box[v] = (!ax[0] || !ax[1] || !ax[2]) ? 1 : 0;
+                }
+    })
+}
+
 int rt_num_active(const rt_hier* h, int level, size_t* ne, size_t* nn) {
     RT_TRY({
         Hier& H = H_(h);
@@ -1296,11 +1341,171 @@ int rt_get_displacement_active(const rt_hier* h, double* u_active) {
     })
 }
 
+// Grid::applyK (Grid.cu:2519-2532, kernel :2439-2517): f = K(rho) u with support
+// neighbours taken as zero; a support node outputs its displacement (the
+// reference writes the grid's U there, which is u at every call site,
+// optimization.cpp:390-531)
+int rt_apply_k(rt_hier* h, const double* u_box, double* f_box) {
+    RT_TRY({
+        need(u_box);
+        need(f_box);
+        Hier& H = H_(h);
+        Level& L0 = H.lv[0];
+        const size_t nb = L0.nn_box();
+        std::vector<double> box(u_box, u_box + 3 * nb);
+        for (size_t v = 0; v < nb; v++)
+            if (!L0.vact[v])
+                for (int c = 0; c < 3; c++) box[c * nb + v] = 0;
+        V3<double> X, Y;
+        X.alloc(L0.g.N);
+        Y.alloc(L0.g.N);
+        upload_vec<double>(H, L0.g, box.data(), X);
+        launch_fine_apply_modal<double>(L0.g, X.cv(), Y.v(), H.rhop.as<float>(), L0.vflag.as<uint8_t>(), nullptr,
+                                        nullptr, H.s);
+        download_vec<double>(H, L0.g, Y, f_box);
+        for (size_t v = 0; v < nb; v++)
+            if (L0.vact[v] && H.fixed_box[v])
+                for (int c = 0; c < 3; c++) f_box[c * nb + v] = box[c * nb + v];
+    })
+}
+
 int rt_compliance(rt_hier* h, double* c) {
     RT_TRY({
         need(c);
         Hier& H = H_(h);
         *c = dot<double, double>(H, H.f64.data(), H.x64.data(), (size_t)3 * H.lv[0].g.N);
+    })
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// worst-case load search (modified power method)
+// ---------------------------------------------------------------------------
+// setLoadNodes (projection.cpp:129-269), with-support modes: load nodes are
+// existing, non-support nodes; normals (if given) are normalised per node
+static void set_load_nodes(Hier& H, const uint8_t* load_box, const double* nrm_box) {
+    Level& L0 = H.lv[0];
+    const size_t nb = L0.nn_box();
+    const int nx = L0.g.n[0], ny = L0.g.n[1];
+    std::vector<int> slots;
+    std::vector<double> nrm[3];
+    for (size_t v = 0; v < nb; v++) {
+        if (!load_box[v] || !L0.vact[v] || H.fixed_box[v]) continue;
+        const int x = (int)(v % nx), y = (int)(v / nx % ny), z = (int)(v / ((size_t)nx * ny));
+        if (nrm_box) {
+            const double a = nrm_box[v], b = nrm_box[nb + v], c = nrm_box[2 * nb + v];
+            const double l = std::sqrt(a * a + b * b + c * c);
+            if (!(l > 0)) throw std::invalid_argument("zero normal at a load node");
+            nrm[0].push_back(a / l); nrm[1].push_back(b / l); nrm[2].push_back(c / l);
+        }
+        slots.push_back(oct_p(L0.g, x, y, z));
+    }
+    H.nload = (int)slots.size();
+    H.load_normals = nrm_box != nullptr;
+    H.load_slots.release(); H.load_nrm.release(); H.pm_fs.release(); H.pm_old.release();
+    if (!H.nload) return;
+    const size_t nl = (size_t)H.nload;
+    H.load_slots.alloc(sizeof(int) * nl);
+    RT_HIP(hipMemcpy(H.load_slots.p, slots.data(), sizeof(int) * nl, hipMemcpyHostToDevice));
+    if (H.load_normals) {
+        H.load_nrm.alloc(sizeof(double) * 3 * nl);
+        for (int c = 0; c < 3; c++)
+            RT_HIP(hipMemcpy(H.load_nrm.as<double>() + c * nl, nrm[c].data(), sizeof(double) * nl,
+                             hipMemcpyHostToDevice));
+    }
+    H.pm_fs.alloc(sizeof(double) * 3 * nl);
+    H.pm_old.alloc(sizeof(double) * 3 * nl);
+}
+
+static const int kPmParts = 256;
+
+// fs = P x (compact); returns |fs|^2
+static double pm_gather(Hier& H, const V3<double>& X) {
+    launch_pm_gather(H.nload, H.load_slots.as<int>(), X.cv(), H.load_normals ? H.load_nrm.as<double>() : nullptr,
+                     H.pm_fs.as<double>(), H.partials.as<double>(), kPmParts, H.s);
+    return sum_partials_host(H, kPmParts);
+}
+// f64 <- scale * fs on the load nodes; returns (|new - old|, |old|)
+static std::pair<double, double> pm_commit(Hier& H, double scale) {
+    launch_pm_commit(H.nload, H.load_slots.as<int>(), H.pm_fs.as<double>(), scale, H.f64.v(), H.pm_old.as<double>(),
+                     H.partials.as<double>(), kPmParts, H.s);
+    const double d = sum_partials_at(H, H.partials.as<double>(), kPmParts);
+    const double o = sum_partials_at(H, H.partials.as<double>() + kPmParts, kPmParts);
+    return {std::sqrt(d), std::sqrt(o)};
+}
+
+// forceProject (projection.cu:165-231) of the finest force, in place
+static void force_project(Hier& H, bool scale_to_unit) {
+    if (!H.nload) throw std::logic_error("no load nodes set");
+    const double n2 = pm_gather(H, H.f64);
+    launch_fill<double>(H.f64.data(), 0.0, (size_t)3 * H.lv[0].g.N, H.s);
+    double sc = 1.0;
+    if (scale_to_unit) {
+        if (!(n2 > 0)) throw std::domain_error("projected force is zero");
+        sc = 1.0 / std::sqrt(n2);
+    }
+    pm_commit(H, sc);
+}
+
+// modifiedPM (optimization.cpp:123-261), with-support branch, from the current
+// force: project + unitize, u = 0, then one V(1,1) per step; the new force is
+// the unit projection of u; stop when |fs_new - fs_old| / |fs_old| <= 1e-4 and
+// the cycle's relative residual <= 1e-2, or after max_itn steps.
+template <typename T>
+static double modified_pm(Hier& H, int max_itn, int* steps) {
+    if (!H.stencil_ready) throw std::logic_error("update_stencil has not been called");
+    force_project(H, true);
+    launch_fill<double>(H.x64.data(), 0.0, (size_t)3 * H.lv[0].g.N, H.s);
+    double fch = 1, rel = 1;
+    int itn = 0, nv = 0;
+    while (itn++ < max_itn && (fch > 1e-4 || rel > 1e-2)) {
+        rel = vcycle_on_solution<T>(H, 1, 1);
+        nv++;
+        if (!(rel <= 1e4)) throw std::domain_error("modified power method diverged");
+        const double n2 = pm_gather(H, H.x64);
+        if (!(n2 > 0)) throw std::domain_error("projected displacement is zero");
+        const auto dn = pm_commit(H, 1.0 / std::sqrt(n2));
+        fch = dn.first / dn.second;
+    }
+    *steps = nv;
+    return dot<double, double>(H, H.f64.data(), H.x64.data(), (size_t)3 * H.lv[0].g.N);
+}
+
+extern "C" {
+
+int rt_set_load_nodes(rt_hier* h, const uint8_t* load_box, const double* normals_box) {
+    RT_TRY({
+        need(load_box);
+        set_load_nodes(H_(h), load_box, normals_box);
+    })
+}
+
+int rt_num_load_nodes(const rt_hier* h, int* n) {
+    RT_TRY({ need(n); *n = H_(h).nload; })
+}
+
+int rt_force_project(rt_hier* h) {
+    RT_TRY({ force_project(H_(h), false); })
+}
+
+int rt_rand_force(rt_hier* h, uint64_t seed) {
+    RT_TRY({
+        Hier& H = H_(h);
+        Level& L0 = H.lv[0];
+        launch_rand_force(L0.g, L0.vflag.as<uint8_t>(), seed, H.f64.v(), H.s);
+        RT_HIP(hipStreamSynchronize(H.s));
+    })
+}
+
+int rt_modified_pm(rt_hier* h, int max_itn, double* worst_compliance, int* vcycles) {
+    RT_TRY({
+        need(worst_compliance);
+        Hier& H = H_(h);
+        if (max_itn < 1) throw std::invalid_argument("max_itn must be positive");
+        int nv = 0;
+        *worst_compliance = H.vprec == 64 ? modified_pm<double>(H, max_itn, &nv) : modified_pm<float>(H, max_itn, &nv);
+        if (vcycles) *vcycles = nv;
     })
 }
 

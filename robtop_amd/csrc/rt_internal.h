@@ -61,6 +61,16 @@ __host__ __device__ inline int oct_p(const Geom& g, int x, int y, int z) { retur
 
 template <typename T> struct Vec3 { T* c[3]; };
 
+// Workgroups are handed to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8), and each XCD has its own L2. This maps workgroup b to a work index
+// so that XCD x processes one contiguous range of work items: neighbouring
+// items (rows that share halo data) then meet in the same L2. A bijection
+// on [0, n).
+__device__ __forceinline__ int xcd_row(int b, int n) {
+    const int x = b & 7, loc = b >> 3, q = n >> 3, r = n & 7;
+    return x * q + (x < r ? x : r) + loc;
+}
+
 // ---------------------------------------------------------------------------
 // kernel launchers (defined in the .hip files); all launch on stream `s`
 // ---------------------------------------------------------------------------
@@ -142,6 +152,13 @@ void launch_oc_trial(const Geom& g, const float* rho, const float* sens, const u
 void launch_maxabs(const Geom& g, const float* a, const uint8_t* eflag, double* partials, int nparts, hipStream_t s);
 void launch_sum_active(const Geom& g, const float* a, const uint8_t* eflag, double* partials, int nparts,
                        hipStream_t s);
+
+// k_worst.hip
+void launch_rand_force(const Geom& g, const uint8_t* vflag, uint64_t seed, Vec3<double> F, hipStream_t s);
+void launch_pm_gather(int nl, const int* slots, Vec3<const double> X, const double* nrm, double* fs,
+                      double* partials, int nparts, hipStream_t s);
+void launch_pm_commit(int nl, const int* slots, const double* fs, double scale, Vec3<double> F, double* fold,
+                      double* partials, int nparts, hipStream_t s);
 
 // k_voxel.hip
 void launch_voxelize(const float* tris, int ntris, const float bmin[3], const float unit[3], int R, uint32_t* bits,

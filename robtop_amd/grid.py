@@ -43,6 +43,22 @@ class Grid:
         check(lib().rt_level_masks(self._h.handle, self.index, ptr(e), ptr(v)))
         return e.astype(bool), v.astype(bool)
 
+    def surface_nodes(self):
+        """Surface-node mask of the node box (mark_surface_nodes_kernel, Grid.cu:2005-2039)."""
+        v = np.zeros(self.n_nodes_box, np.uint8)
+        check(lib().rt_surface_nodes_box(self._h.handle, self.index, ptr(v)))
+        return v.astype(bool)
+
+    def applyK(self, u):
+        """Grid::applyK (Grid.cu:2519-2532) on the finest level: K(rho) u, support
+        neighbours as zero, support nodes return u."""
+        if self.index != 0:
+            raise ValueError("applyK is defined on the finest level")
+        a = as_c(u, np.float64).reshape(3, self.n_nodes_box)
+        f = np.zeros((3, self.n_nodes_box))
+        check(lib().rt_apply_k(self._h.handle, ptr(a), ptr(f)))
+        return f
+
     def is_finest(self):
         return self.index == 0
 
@@ -272,6 +288,36 @@ class HierarchyGrid:
         v = ctypes.c_double()
         check(lib().rt_volume(self.handle, ctypes.byref(v)))
         return v.value
+
+    # worst-case load search ------------------------------------------------------
+    def set_load_nodes(self, load_box, normals_box=None):
+        """setLoadNodes (projection.cpp:129-269): load-node flags on the finest
+        node box; outward normals [3, n] select the constrained-direction mode."""
+        a = as_c(load_box, np.uint8)
+        if normals_box is None:
+            check(lib().rt_set_load_nodes(self.handle, ptr(a), None))
+        else:
+            nb = as_c(normals_box, np.float64).reshape(3, -1)
+            check(lib().rt_set_load_nodes(self.handle, ptr(a), ptr(nb)))
+        n = ctypes.c_int()
+        check(lib().rt_num_load_nodes(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def force_project(self):
+        """forceProject (projection.cu:165-231) of the finest force."""
+        check(lib().rt_force_project(self.handle))
+
+    def rand_force(self, seed):
+        """randForce (Grid.cu:2242-2245), seeded."""
+        check(lib().rt_rand_force(self.handle, ctypes.c_uint64(seed)))
+
+    def modified_pm(self, max_itn=500):
+        """modifiedPM (optimization.cpp:123-261) from the current force; returns
+        (worst compliance, V-cycles)."""
+        c = ctypes.c_double()
+        nv = ctypes.c_int()
+        check(lib().rt_modified_pm(self.handle, max_itn, ctypes.byref(c), ctypes.byref(nv)))
+        return c.value, nv.value
 
     def synchronize(self):
         check(lib().rt_synchronize(self.handle))

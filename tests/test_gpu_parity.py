@@ -100,6 +100,44 @@ def test_fine_gs(pair, bits, reverse):
     assert rel_err(g[0].getDisplacement(), o.get_vec(0, 0)) < TOL[bits]
 
 
+def test_apply_k(pair):
+    """Grid::applyK (Grid.cu:2439-2532): K u with support neighbours as zero at
+    free nodes; a support node returns its own displacement."""
+    g, o = pair.g, pair.o
+    u = pair.rand_vec(0, fixed_zero=False)
+    fx = pair.fixed.astype(bool)
+    y = g[0].applyK(u)
+    uz = u.copy()
+    uz[:, fx] = 0
+    ref = o.apply(uz)
+    ref[:, fx] = u[:, fx]
+    assert rel_err(y, ref) < TOL[64]
+
+
+def test_surface_nodes(pair):
+    """mark_surface_nodes_kernel (Grid.cu:2005-2039) by brute force: an existing
+    node is on the surface if along some axis no pair of solid elements faces
+    each other across it."""
+    lv = pair.o.levels[0]
+    ex, ey, ez = lv["e"]
+    eact = lv["eact"].reshape(ez, ey, ex)
+    pad = np.zeros((ez + 2, ey + 2, ex + 2), bool)
+    pad[1:-1, 1:-1, 1:-1] = eact
+    nx, ny, nz = ex + 1, ey + 1, ez + 1
+    # s[k][j][i][z, y, x]: element at (x + i - 1, y + j - 1, z + k - 1) of node (x, y, z)
+    s = [[[pad[k:k + nz, j:j + ny, i:i + nx] for i in range(2)] for j in range(2)] for k in range(2)]
+    axx = np.zeros((nz, ny, nx), bool)
+    axy = np.zeros_like(axx)
+    axz = np.zeros_like(axx)
+    for a in range(2):
+        for b in range(2):
+            axx |= s[b][a][0] & s[b][a][1]
+            axy |= s[b][0][a] & s[b][1][a]
+            axz |= s[0][b][a] & s[1][b][a]
+    ref = (~axx | ~axy | ~axz) & lv["vact"].reshape(nz, ny, nx)
+    assert np.array_equal(pair.g[0].surface_nodes(), ref.reshape(-1))
+
+
 def test_fine_gs_fixed_values_ignored(pair):
     """Nonzero displacement/force at supports: neighbours still treat them as 0
     (Grid.cu:981-984) and the support node itself becomes f / n_e."""

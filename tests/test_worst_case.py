@@ -1,0 +1,159 @@
+"""Worst-case load search: forceProject and modifiedPM
+(projection.cu:165-231, projection.cpp:289-360, optimization.cpp:123-261).
+
+CPU: the oracle is pinned against what the mathematics fixes.
+  - forceProject equals f - Nd^T Nd f with Nd built from an orthonormal
+    tangent pair per load node (projection.cpp:176-197), computed here by an
+    SVD null space rather than the normal-component shortcut.
+  - modifiedPM converges to the largest eigenvalue of the aggregated
+    compliance operator C = N^T K^-1 N (the reference's own cross-check,
+    test_utils.cpp:1249-1296), with K assembled by brute force and factored by
+    scipy. Its final load is the top eigenvector. The reference's stopping
+    rule (load change 1e-4, residual 1e-2) bounds the agreement.
+GPU: the HIP path reproduces the oracle step by step (FP64 V-cycle) from the
+same start: the same number of V-cycles and the same worst compliance, load
+and displacement to 1e-10.
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import bruteforce as bf
+import oracle
+from problems import active_cube, bits_of, problem_inputs, rel_err
+
+
+def _problem(kind="holes3", seed=3):
+    a = active_cube(kind, 32)
+    H = oracle.Hierarchy(32, bits_of(a))
+    H.set_material(1e6, 0.4, 3.0)
+    lv0 = H.levels[0]
+    fixed, _, rho = problem_inputs(lv0, seed)
+    rhop = (rho.astype(np.float64) ** 3).astype(np.float32)
+    H.set_fixed(fixed)
+    H.set_rho(rho)
+    H.set_rhop(rhop)
+    n = lv0["n"]
+    vact = lv0["vact"].reshape(n[2], n[1], n[0])
+    load = np.zeros_like(vact)
+    zs = np.nonzero(vact.any(axis=(1, 2)))[0]
+    load[zs[-1]] = vact[zs[-1]]                  # top face of the solid
+    load = load.reshape(-1) & ~fixed.astype(bool)
+    rng = np.random.default_rng(5)
+    nrm = np.zeros((3, lv0["nn"]))
+    nrm[2] = 1.0
+    nrm += 0.3 * rng.standard_normal(nrm.shape)   # tilted normals: every component matters
+    f0 = np.random.default_rng(1).uniform(-1, 1, (3, lv0["nn"])) * lv0["vact"][None]
+    return a, H, fixed, rho, rhop, load, nrm, f0
+
+
+@pytest.fixture(scope="module")
+def prob():
+    return _problem()
+
+
+def test_force_project_matches_tangent_basis(prob):
+    a, H, fixed, rho, rhop, load, nrm, f0 = prob
+    nn = H.levels[0]["nn"]
+    for normals in (None, nrm):
+        H.set_load_nodes(load.astype(np.uint8), normals)
+        H.set_vec(0, 1, f0)
+        H.force_project()
+        got = H.get_vec(0, 1)
+        ref = np.zeros((3, nn))
+        for v in np.nonzero(load)[0]:
+            if normals is None:
+                ref[:, v] = f0[:, v]
+            else:
+                _, _, vt = np.linalg.svd(normals[:, v][None, :])
+                Nd = vt[1:]                      # orthonormal tangent pair
+                ref[:, v] = f0[:, v] - Nd.T @ (Nd @ f0[:, v])
+        assert rel_err(got, ref) < 1e-13
+
+
+@pytest.mark.parametrize("mode", ["wsff", "wscf"])
+def test_modified_pm_finds_top_eigenpair(prob, mode):
+    a, H, fixed, rho, rhop, load, nrm, f0 = prob
+    nn = H.levels[0]["nn"]
+    normals = nrm if mode == "wscf" else None
+    H.update_stencil()
+    H.set_load_nodes(load.astype(np.uint8), normals)
+    H.set_vec(0, 1, f0)
+    c, steps = H.modified_pm(500)
+    assert 1 < steps < 500
+
+    P = bf.Problem(a, bf.from_flat(rho, H.levels[0]["e"]), bf.from_flat(fixed.astype(bool), H.levels[0]["n"]),
+                   rhop=bf.from_flat(rhop, H.levels[0]["e"]))
+    act = np.nonzero(P.active[0] & ~np.repeat(fixed.astype(bool), 3))[0]
+    lu = spla.splu(P.A_res[act][:, act].tocsc())
+    cols = []
+    for v in np.nonzero(load)[0]:
+        dirs = [np.eye(3)[k] for k in range(3)] if normals is None else [normals[:, v] / np.linalg.norm(normals[:, v])]
+        for d in dirs:
+            x = np.zeros(3 * nn)
+            x[3 * v:3 * v + 3] = d
+            cols.append(x)
+    N = np.array(cols).T
+    KN = np.zeros_like(N)
+    KN[act] = lu.solve(N[act])
+    w, V = np.linalg.eigh(N.T @ KN)
+    assert abs(c - w[-1]) / w[-1] < 5e-4
+    f = H.get_vec(0, 1).T.reshape(-1)            # interleaved dofs
+    ftop = N @ V[:, -1]
+    assert abs(abs(f @ ftop) - 1) < 1e-3
+    assert abs(np.linalg.norm(f) - 1) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["wsff", "wscf"])
+def test_gpu_modified_pm_matches_oracle(prob, mode):
+    import robtop_amd as rt
+    a, H, fixed, rho, rhop, load, nrm, f0 = prob
+    normals = nrm if mode == "wscf" else None
+    print(f"[pm] holes3 {mode}", flush=True)
+    g = rt.HierarchyGrid.genFromMesh(bits_of(a), 32)
+    g.set_material(1e6, 0.4, 3.0)
+    g.set_support(fixed)
+    g.set_density(rho)
+    assert np.array_equal(g.get_rhop(), rhop)
+    g.set_precision(64)
+    g.update_stencil()
+    nl = g.set_load_nodes(load.astype(np.uint8), normals)
+    assert nl == int(load.sum())
+    f0z = f0.copy()
+    f0z[:, fixed.astype(bool)] = 0                 # set_force clears supports; they are not load nodes
+    g.set_force(f0z)
+    cg, sg = g.modified_pm(500)
+
+    H.update_stencil()
+    H.set_load_nodes(load.astype(np.uint8), normals)
+    H.set_vec(0, 1, f0z)
+    co, so = H.modified_pm(500)
+    assert sg == so
+    assert abs(cg - co) / abs(co) < 1e-10
+    assert rel_err(g.get_force(), H.get_vec(0, 1)) < 1e-10
+    assert rel_err(g.get_displacement(), H.get_vec(0, 0)) < 1e-10
+    g.close()
+
+
+@pytest.mark.gpu
+def test_gpu_rand_force_and_project():
+    import robtop_amd as rt
+    a, H, fixed, rho, rhop, load, nrm, f0 = _problem()
+    g = rt.HierarchyGrid.genFromMesh(bits_of(a), 32)
+    g.set_material(1e6, 0.4, 3.0)
+    g.set_support(fixed)
+    g.set_load_nodes(load.astype(np.uint8), nrm)
+    g.rand_force(7)
+    f = g.get_force()
+    vact = H.levels[0]["vact"].astype(bool)
+    assert np.all(np.abs(f[:, vact]) <= 1) and not f[:, ~vact].any()
+    assert abs(f[:, vact].mean()) < 0.05 and 0.30 < f[:, vact].var() < 0.37   # uniform(-1, 1): var 1/3
+    g.rand_force(7)
+    assert np.array_equal(g.get_force(), f)
+    g.force_project()
+    H.set_load_nodes(load.astype(np.uint8), nrm)
+    H.set_vec(0, 1, f)
+    H.force_project()
+    assert rel_err(g.get_force(), H.get_vec(0, 1)) < 1e-14
+    g.close()
