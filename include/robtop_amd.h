@@ -220,6 +220,29 @@ int rt_rand_force(rt_hier* h, uint64_t seed);
  * returns the worst compliance f.u and the number of V-cycles. */
 int rt_modified_pm(rt_hier* h, int max_itn, double* worst_compliance, int* vcycles);
 
+/* --- MMA optimiser ------------------------------------------------------- */
+
+typedef struct rt_mma rt_mma;
+
+/* MMA::mma_t(ndim, nConstrain) + init(lower, upper)
+ * (optimization/mma_t.h:202-221): n FP32 design variables with bounds
+ * lower[n], upper[n] (host or device arrays), 1 <= m <= 8 constraints
+ * g(x) <= 0; a0 = 1, a = 0, c = 10, d = 1 as the reference's defaults. */
+int rt_mma_create(int n, int m, const float* lower, const float* upper, rt_mma** out);
+int rt_mma_destroy(rt_mma* h);
+/* MMA::mma_t::update(dev_df, dev_dg, dev_g) (optimization/mma_t.cpp:18-51):
+ * df[n] objective gradient, dg[m][n] constraint gradients (row-major), g[m]
+ * constraint values at the current design (host or device arrays). Solves the
+ * MMA subproblem by the reference's primal-dual iteration (mma_t.cu:330-592),
+ * moves the asymptotes (mma_t.cu:161-226); *converged = 1 once the design
+ * change has stayed below 1e-3 for more than 10 updates. */
+int rt_mma_update(rt_mma* h, const float* df, const float* dg, const float* g, int* converged);
+/* MMA::mma_t::get_x (mma_t.cu:156-159): current design (host or device array) */
+int rt_mma_get_x(const rt_mma* h, float* x);
+/* last update's multipliers lambda[m], artificial variable z, number of
+ * subproblem iterations and whether they hit the reference's limit of 80 */
+int rt_mma_stats(const rt_mma* h, float* lambda, float* z, int* sub_iters, int* overflowed);
+
 /* --- voxelizer ----------------------------------------------------------- */
 
 /* voxelize_mesh (voxelizer/voxelize_main.cpp:344-404): padded cube of side

@@ -69,6 +69,14 @@ def lib():
             "or_set_load_nodes": ([P, P, P], None),
             "or_force_project": ([P], None),
             "or_modified_pm": ([P, I, P], D),
+            "or_mma_create": ([I, I, P, P], P),
+            "or_mma_destroy": ([P], None),
+            "or_mma_update": ([P, P, P, P], I),
+            "or_mma_get_x": ([P, P], None),
+            "or_mma_set_x": ([P, P], None),
+            "or_mma_get_bounds": ([P, P], None),
+            "or_mma_stats": ([P, P, P, P, P], None),
+            "or_mma_last_error": ([], ctypes.c_char_p),
             "or_bitsat_rank": ([P, I, ctypes.c_long], I),
             "or_voxelize_mesh": ([P, I, P, I, I, P, I, P, P], I),
             "or_voxelize_triangles": ([P, I, P, P, I, P], None),
@@ -285,3 +293,51 @@ class Hierarchy:
         it = np.zeros(1, np.int32)
         thr = lib().or_oc_update(self.h, Vgoal, step, damp, rhomin, _p(g), _p(it))
         return thr, int(it[0])
+
+
+class Mma:
+    """The reference's MMA optimiser (optimization/mma_t.h:159-266): n design
+    variables in [lo, hi], m constraints g(x) <= 0."""
+
+    def __init__(self, n, m, lo, hi):
+        lo = np.broadcast_to(np.asarray(lo, np.float32), (n,)).copy()
+        hi = np.broadcast_to(np.asarray(hi, np.float32), (n,)).copy()
+        self.n, self.m = n, m
+        self.h = lib().or_mma_create(n, m, _p(lo), _p(hi))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_mma_destroy(self.h)
+            self.h = None
+
+    def x(self):
+        a = np.zeros(self.n, np.float32)
+        lib().or_mma_get_x(self.h, _p(a))
+        return a
+
+    def set_x(self, x):
+        a = np.ascontiguousarray(x, dtype=np.float32)
+        lib().or_mma_set_x(self.h, _p(a))
+
+    def bounds(self):
+        a = np.zeros((4, self.n), np.float32)
+        lib().or_mma_get_bounds(self.h, _p(a))
+        return a
+
+    def stats(self):
+        lam = np.zeros(self.m, np.float32)
+        z = np.zeros(1, np.float32)
+        it = np.zeros(1, np.int32)
+        ov = np.zeros(1, np.int32)
+        lib().or_mma_stats(self.h, _p(lam), _p(z), _p(it), _p(ov))
+        return dict(lambda_=lam, z=float(z[0]), sub_iters=int(it[0]), overflowed=bool(ov[0]))
+
+    def update(self, df, dg, g):
+        """mma_t::update (mma_t.cpp:18-51); dg is [m, n]. True once converged."""
+        df = np.ascontiguousarray(df, dtype=np.float32)
+        dg = np.ascontiguousarray(dg, dtype=np.float32).reshape(self.m, self.n)
+        g = np.ascontiguousarray(g, dtype=np.float32)
+        r = lib().or_mma_update(self.h, _p(df), _p(dg), _p(g))
+        if r < 0:
+            raise RuntimeError(lib().or_mma_last_error().decode())
+        return r == 1

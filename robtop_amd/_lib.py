@@ -87,6 +87,11 @@ SIGNATURES = {
     "rt_update_densities": ([P, F, F, F, F, P, P], I),
     "rt_volume": ([P, P], I),
     "rt_voxelize_mesh": ([P, I, P, I, I, P, SZ, P, P], I),
+    "rt_mma_create": ([I, I, P, P, P], I),
+    "rt_mma_destroy": ([P], I),
+    "rt_mma_update": ([P, P, P, P, P], I),
+    "rt_mma_get_x": ([P, P], I),
+    "rt_mma_stats": ([P, P, P, P, P], I),
     "rt_set_load_nodes": ([P, P, P], I),
     "rt_num_load_nodes": ([P, P], I),
     "rt_force_project": ([P], I),

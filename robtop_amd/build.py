@@ -1,8 +1,8 @@
 """Build recipe for librobtop_amd.so (gfx950) and the CPU oracle.
 
 Every HIP translation unit is compiled with hipcc for gfx950 only, with packed
-FP32 instructions disabled; the voxelizer additionally with FP contraction off
-(its bit pattern depends on exact FP32 rounding). Objects go to
+FP32 instructions disabled; the voxelizer and the MMA optimiser additionally
+with FP contraction off (their results depend on exact FP32 rounding). Objects go to
 robtop_amd/_build/, the shared library to robtop_amd/librobtop_amd.so.
 
     python -m robtop_amd.build          # library + oracle + tests/cpp programs
@@ -31,8 +31,10 @@ UNITS = {
     "k_galerkin.hip": [],
     "k_simp.hip": [],
     "k_worst.hip": [],
+    "k_mma.hip": ["-ffp-contract=off"],
     "k_voxel.hip": ["-ffp-contract=off"],
     "rt_host.cpp": ["-x", "hip"],
+    "rt_mma.cpp": ["-x", "hip", "-ffp-contract=off"],
 }
 
 

@@ -891,6 +891,30 @@ struct rt_hier {
         return RT_EINTERNAL;                                                         \
     }
 
+int rt::status_of_current_exception() {
+    try {
+        throw;
+    } catch (const std::invalid_argument& e) {
+        g_err = e.what();
+        return RT_EARG;
+    } catch (const std::domain_error& e) {
+        g_err = e.what();
+        return RT_ENUM;
+    } catch (const std::logic_error& e) {
+        g_err = e.what();
+        return RT_ESTATE;
+    } catch (const std::runtime_error& e) {
+        g_err = e.what();
+        return std::string(e.what()).rfind("HIP", 0) == 0 ? RT_EHIP : RT_EINTERNAL;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return RT_EINTERNAL;
+    } catch (...) {
+        g_err = "unknown error";
+        return RT_EINTERNAL;
+    }
+}
+
 static Hier& H_(rt_hier* h) {
     if (!h || !h->h) throw std::invalid_argument("null handle");
     return *h->h;

@@ -164,6 +164,10 @@ void launch_pm_commit(int nl, const int* slots, const double* fs, double scale, 
 void launch_voxelize(const float* tris, int ntris, const float bmin[3], const float unit[3], int R, uint32_t* bits,
                      hipStream_t s);
 
+// maps the exception being handled to an RT_E* status and records its message
+// for rt_last_error (call only inside a catch block)
+int status_of_current_exception();
+
 #define RT_HIP(call)                                                                                   \
     do {                                                                                               \
         hipError_t e_ = (call);                                                                        \

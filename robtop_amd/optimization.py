@@ -124,3 +124,57 @@ def ordinary_topopt(hg, params, n_iter=100, tol=1e-6, mode=RT_SOLVE_MIXED, pre=1
         if stop and done:
             break
     return recs
+
+
+class MMA:
+    """MMA::mma_t (optimization/mma_t.h:159-266) on the GPU: n FP32 design
+    variables in [lower, upper], m <= 8 constraints g(x) <= 0."""
+
+    def __init__(self, n, m, lower, upper):
+        import ctypes
+        import numpy as np
+        from ._lib import check, lib, ptr
+        lo = np.ascontiguousarray(np.broadcast_to(np.asarray(lower, np.float32), (n,)))
+        hi = np.ascontiguousarray(np.broadcast_to(np.asarray(upper, np.float32), (n,)))
+        h = ctypes.c_void_p()
+        check(lib().rt_mma_create(n, m, ptr(lo), ptr(hi), ctypes.byref(h)))
+        self.handle, self.n, self.m = h, n, m
+
+    def close(self):
+        from ._lib import lib
+        if getattr(self, "handle", None):
+            lib().rt_mma_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def x(self):
+        import numpy as np
+        from ._lib import check, lib, ptr
+        a = np.zeros(self.n, np.float32)
+        check(lib().rt_mma_get_x(self.handle, ptr(a)))
+        return a
+
+    def update(self, df, dg, g):
+        """mma_t::update (mma_t.cpp:18-51); dg is [m, n]. True once converged."""
+        import ctypes
+        import numpy as np
+        from ._lib import check, lib, ptr
+        df = np.ascontiguousarray(df, dtype=np.float32)
+        dg = np.ascontiguousarray(dg, dtype=np.float32).reshape(self.m, self.n)
+        g = np.ascontiguousarray(g, dtype=np.float32)
+        c = ctypes.c_int()
+        check(lib().rt_mma_update(self.handle, ptr(df), ptr(dg), ptr(g), ctypes.byref(c)))
+        return bool(c.value)
+
+    def stats(self):
+        import ctypes
+        import numpy as np
+        from ._lib import check, lib, ptr
+        lam = np.zeros(self.m, np.float32)
+        z = ctypes.c_float()
+        it = ctypes.c_int()
+        ov = ctypes.c_int()
+        check(lib().rt_mma_stats(self.handle, ptr(lam), ctypes.byref(z), ctypes.byref(it), ctypes.byref(ov)))
+        return dict(lambda_=lam, z=z.value, sub_iters=it.value, overflowed=bool(ov.value))

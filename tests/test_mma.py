@@ -1,0 +1,121 @@
+"""MMA optimiser (optimization/mma_t.h:159-266, mma_t.cpp:18-51, mma_t.cu).
+
+CPU: the oracle's restatement is pinned by what the mathematics fixes.
+  - The reference's own 2-D test problem (mma_t.cpp:208-300): the converged
+    design equals scipy's SLSQP optimum.
+  - A separable compliance-like problem, min mean(w / x) subject to
+    mean(x) <= V on [0.01, 1]: its optimum is known in closed form,
+    x_j = clip(sqrt(w_j / lambda)) with lambda fixed by the volume.
+  - Two constraints (m = 2): SLSQP optimum again.
+GPU: the HIP path follows the oracle update by update (same FP32 arithmetic,
+FP64 sums in another order) and reaches the same designs.
+"""
+import numpy as np
+import pytest
+from scipy.optimize import minimize
+
+import oracle
+
+
+def ref2d():
+    f = lambda x, y: x / 2 + y * np.sqrt(3) / 2 + np.exp(10 * (x - y) ** 2)
+
+    def df(x, y):
+        e = np.exp(10 * (x - y) ** 2)
+        return np.array([0.5 + 20 * (x - y) * e, np.sqrt(3) / 2 + 20 * (y - x) * e])
+
+    g = lambda x, y: np.array([x * x + y * y - 1, x * x - 0.75, y * y - 0.75, (x + y) ** 2 - 1])
+    dg = lambda x, y: np.array([[2 * x, 2 * y], [2 * x, 0], [0, 2 * y], [2 * (x + y), 2 * (x + y)]])
+    return f, df, g, dg
+
+
+def run(M, df, dg, g, maxit=200):
+    for it in range(maxit):
+        x = M.x().astype(np.float64)
+        if M.update(df(x), dg(x), g(x)):
+            return M.x(), it + 1
+    return M.x(), maxit
+
+
+def separable(n, seed):
+    # objective mean(w / x): its volume multiplier stays below the reference's
+    # default penalty c = 10 on the relaxed constraint (mma_t.h:212)
+    w = np.random.default_rng(seed).uniform(0.05, 0.5, n)
+    V = 0.4
+    df = lambda x: -w / (x * x) / n
+    dg = lambda x: np.full((1, n), 1.0 / n)
+    g = lambda x: np.array([x.mean() - V])
+    lo, hi = 0.01, 1.0
+    a, b = 1e-8, 1e8
+    for _ in range(200):        # bisection on the multiplier of the volume
+        lam = np.sqrt(a * b)
+        xs = np.clip(np.sqrt(w / lam), lo, hi)
+        if xs.mean() > V:
+            a = lam
+        else:
+            b = lam
+    return w, df, dg, g, xs
+
+
+def test_oracle_reference_2d_problem():
+    f, df, g, dg = ref2d()
+    x, it = run(oracle.Mma(2, 4, -1, 1), lambda v: df(*v), lambda v: dg(*v), lambda v: g(*v))
+    r = minimize(lambda v: f(*v), [0, 0], jac=lambda v: df(*v), bounds=[(-1, 1), (-1, 1)], method="SLSQP",
+                 constraints=[{"type": "ineq", "fun": lambda v: -g(*v), "jac": lambda v: -dg(*v)}],
+                 options={"ftol": 1e-14})
+    assert it < 200
+    assert np.max(np.abs(x - r.x)) < 1e-5
+
+
+def test_oracle_separable_closed_form():
+    n = 64
+    w, df, dg, g, xs = separable(n, 0)
+    x, it = run(oracle.Mma(n, 1, 0.01, 1.0), df, dg, g, 300)
+    assert np.max(np.abs(x - xs)) < 2e-4
+
+
+def test_oracle_two_constraints():
+    n = 6
+    c = np.array([1.0, 2.0, 0.5, 1.5, 1.0, 3.0])
+    f = lambda x: np.sum(c / x) + np.sum(x * x)
+    df = lambda x: -c / (x * x) + 2 * x
+    A = np.array([[1, 1, 1, 0, 0, 0], [0, 0, 1, 1, 1, 1]], float)
+    g = lambda x: A @ x - np.array([1.5, 2.0])
+    dg = lambda x: A
+    x, it = run(oracle.Mma(n, 2, 0.05, 2.0), df, dg, g, 300)
+    r = minimize(f, np.full(n, 0.3), jac=df, bounds=[(0.05, 2.0)] * n, method="SLSQP",
+                 constraints=[{"type": "ineq", "fun": lambda v: -g(v), "jac": lambda v: -A}], options={"ftol": 1e-14})
+    assert np.max(np.abs(x - r.x)) < 1e-4
+
+
+@pytest.mark.gpu
+def test_gpu_reference_2d_problem_matches_oracle():
+    import robtop_amd as rt
+    f, df, g, dg = ref2d()
+    print("[mma] reference 2-D problem", flush=True)
+    xo, ito = run(oracle.Mma(2, 4, -1, 1), lambda v: df(*v), lambda v: dg(*v), lambda v: g(*v))
+    M = rt.MMA(2, 4, -1, 1)
+    xg, itg = run(M, lambda v: df(*v), lambda v: dg(*v), lambda v: g(*v))
+    M.close()
+    assert np.max(np.abs(xg - xo)) < 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [64, 100003])
+def test_gpu_separable_matches_oracle_per_update(n):
+    import robtop_amd as rt
+    print(f"[mma] separable n={n}", flush=True)
+    w, df, dg, g, xs = separable(n, 1)
+    O = oracle.Mma(n, 1, 0.01, 1.0)
+    G = rt.MMA(n, 1, 0.01, 1.0)
+    for it in range(8):
+        xo, xg = O.x().astype(np.float64), G.x().astype(np.float64)
+        assert np.max(np.abs(xg - xo)) < 1e-4 * max(1, it), it
+        O.update(df(xo), dg(xo), g(xo))
+        G.update(df(xg), dg(xg), g(xg))
+    so, sg = O.stats(), G.stats()
+    assert not sg["overflowed"]
+    assert abs(sg["lambda_"][0] - so["lambda_"][0]) <= 1e-3 * abs(so["lambda_"][0])
+    x, _ = run(G, df, dg, g, 300)
+    G.close()
+    assert np.max(np.abs(x - xs)) < 5e-4
