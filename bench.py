@@ -30,6 +30,8 @@ def parse():
     ap.add_argument("--pre", type=int, default=2)
     ap.add_argument("--post", type=int, default=2)
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--coarse-mult", type=int, default=1, help="sweep multiplier on coarse levels (preconditioner)")
+    ap.add_argument("--gamma", type=int, default=1, help="preconditioner cycle index: 1 = V, 2 = W")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--vcycles", type=int, default=20, help="graphless V-cycles timed for the roofline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
@@ -189,6 +191,7 @@ def main():
     log(f"building {ex}x{ey}x{ez}")
     hg, bits, R, fixed, f, rho = build_problem(ex, ey, ez, a.seed)
     log(f"hierarchy: {hg.n_levels} levels")
+    hg.set_preconditioner(a.coarse_mult, a.gamma)
     g0 = hg[0]
     ne0, nn0 = hg.num_active(0)
     levels = []

@@ -165,6 +165,16 @@ int rt_vcycle(rt_hier* h, int pre, int post, double* rel_res);
 
 /* --- solve and SIMP step ------------------------------------------------- */
 
+/* Shape of the preconditioner cycle of rt_solve (and rt_time_vcycle): levels
+ * below the finest smooth coarse_sweep_mult times pre/post sweeps, and each
+ * coarse level is visited gamma times (1 = V-cycle, 2 = W-cycle). Default
+ * (1, 1); no reference counterpart. rt_vcycle is always the reference's V. */
+int rt_set_preconditioner(rt_hier* h, int coarse_sweep_mult, int gamma);
+
+/* Replay the preconditioner cycle of rt_solve as a captured HIP graph (on,
+ * the default) or launch its kernels one by one (off); results are identical. */
+int rt_set_graphs(rt_hier* h, int on);
+
 /* MG-preconditioned CG to ||r||/||f|| <= tol from the current displacement
  * (new; the reference iterates V-cycles: solveFEM, optimization.cpp:85-91,
  * and test_utils.cpp:633-637). Preconditioner: V(pre, post) with symmetric
@@ -255,9 +265,9 @@ int rt_voxelize_mesh(const float* coords, int nverts, const int* faces, int nfac
 /* --- timing hooks (benchmarking) ----------------------------------------- */
 
 int rt_synchronize(rt_hier* h);
-/* Runs n preconditioner V-cycles (precision `bits`) on the current force and
- * returns the mean milliseconds per cycle measured with HIP events on the
- * handle's stream. */
+/* Runs n preconditioner cycles (precision `bits`, the shape and graph mode
+ * rt_solve uses) on the current force and returns the mean milliseconds per
+ * cycle measured with HIP events on the handle's stream. */
 int rt_time_vcycle(rt_hier* h, int bits, int pre, int post, int n, double* ms_per_cycle);
 /* Times n full fine-level smoothing sweeps (colours 0..7, run as four
  * colour-pair launches) of precision `bits`; returns mean ms per sweep. */

@@ -116,8 +116,19 @@ struct Hier {
     int nload = 0;
     bool load_normals = false;
     int vprec = 64;
+    int pc_mult = 1, pc_gamma = 1;   // preconditioner cycle: coarse-level sweep multiplier, cycle index
+    // the preconditioner cycle replayed as a HIP graph (pc_key identifies its shape)
+    bool use_graphs = true;
+    hipGraphExec_t pc_graph = nullptr;
+    long long pc_key = -1;
     bool stencil_ready = false, material_set = false;
+    void drop_graph() {
+        if (pc_graph) (void)hipGraphExecDestroy(pc_graph);
+        pc_graph = nullptr;
+        pc_key = -1;
+    }
     ~Hier() {
+        drop_graph();
         if (s) (void)hipStreamDestroy(s);
     }
 };
@@ -430,26 +441,69 @@ static void coarsest(Hier& H, Vec3<const T> F, Vec3<T> U) {
 
 // V(pre, post) with caller-owned finest vectors (Grid.cpp:1068-1113 ordering).
 template <typename T>
-static void vcycle_core(Hier& H, V3<T>& U0, V3<T>& F0, V3<T>& R0, int pre, int post, bool symmetric) {
+static void vcycle_core(Hier& H, V3<T>& U0, V3<T>& F0, V3<T>& R0, int pre, int post, bool symmetric,
+                        int coarse_mult = 1, int gamma = 1) {
     const int n = (int)H.lv.size();
     auto Uv = [&](int i) { return i == 0 ? U0.v() : vecs<T>(H.lv[i]).U.v(); };
     auto Fv = [&](int i) { return i == 0 ? F0.cv() : vecs<T>(H.lv[i]).F.cv(); };
     auto Rv = [&](int i) { return i == 0 ? R0.v() : vecs<T>(H.lv[i]).R.v(); };
     auto cvt = [](Vec3<T> a) { return Vec3<const T>{{a.c[0], a.c[1], a.c[2]}}; };
-    for (int i = 0; i < n; i++) {
-        if (i > 0) {
-            residual<T>(H, i - 1, cvt(Uv(i - 1)), Fv(i - 1), Rv(i - 1));
-            LevelVecs<T>& lvv = vecs<T>(H.lv[i]);
-            restrict_to<T>(H, i, cvt(Rv(i - 1)), lvv.F.v());
-            launch_fill<T>(lvv.U.data(), T(0), (size_t)3 * lvv.U.N, H.s);
+    // level i: pre-smooth, restrict the residual, gamma coarse cycles from zero,
+    // prolong, post-smooth (Grid.cpp:1068-1113 for gamma = 1, no multiplier);
+    // levels below the finest smooth coarse_mult times as often
+    auto cyc = [&](auto&& self, int i) -> void {
+        if (i == n - 1) {
+            coarsest<T>(H, Fv(i), Uv(i));
+            return;
         }
-        if (i < n - 1) gs<T>(H, i, Uv(i), Fv(i), pre, false);
-        else coarsest<T>(H, Fv(i), Uv(i));
-    }
-    for (int i = n - 2; i >= 0; i--) {
+        const int m = i == 0 ? 1 : coarse_mult;
+        gs<T>(H, i, Uv(i), Fv(i), pre * m, false);
+        residual<T>(H, i, cvt(Uv(i)), Fv(i), Rv(i));
+        LevelVecs<T>& lvv = vecs<T>(H.lv[i + 1]);
+        restrict_to<T>(H, i + 1, cvt(Rv(i)), lvv.F.v());
+        launch_fill<T>(lvv.U.data(), T(0), (size_t)3 * lvv.U.N, H.s);
+        const int visits = i + 1 == n - 1 ? 1 : gamma;
+        for (int v = 0; v < visits; v++) self(self, i + 1);
         prolong_from<T>(H, i, Uv(i), cvt(Uv(i + 1)));
-        gs<T>(H, i, Uv(i), Fv(i), post, symmetric);
+        gs<T>(H, i, Uv(i), Fv(i), post * m, symmetric);
+    };
+    cyc(cyc, 0);
+}
+
+// z = M^-1 r for the PCG: pc.U = 0, then one preconditioner cycle on pc.F.
+// After an eager first application (which also performs lazy allocations) the
+// cycle is captured once into a HIP graph and replayed; the graph holds no
+// memset nodes (pc.U is cleared by a fill kernel) and is dropped whenever
+// stencils, material, supports or the cycle shape change.
+template <typename TC>
+static void precondition(Hier& H, LevelVecs<TC>& pc, int pre, int post) {
+    const size_t n = (size_t)3 * pc.U.N;
+    const long long key = ((((long long)sizeof(TC) * 64 + pre) * 64 + post) * 64 + H.pc_mult) * 4 + H.pc_gamma;
+    if (H.use_graphs && H.pc_graph && H.pc_key == key) {
+        RT_HIP(hipGraphLaunch(H.pc_graph, H.s));
+        return;
     }
+    launch_fill<TC>(pc.U.data(), TC(0), n, H.s);
+    vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma);
+    if (!H.use_graphs) return;
+    H.drop_graph();
+    RT_HIP(hipStreamBeginCapture(H.s, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    try {
+        launch_fill<TC>(pc.U.data(), TC(0), n, H.s);
+        vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma);
+    } catch (...) {
+        (void)hipStreamEndCapture(H.s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    RT_HIP(hipStreamEndCapture(H.s, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    RT_HIP(e);
+    H.pc_graph = ex;
+    H.pc_key = key;
 }
 
 template <typename TA, typename TB>
@@ -570,6 +624,7 @@ static void factor_coarsest(Hier& H) {
 }
 
 static void update_stencil(Hier& H) {
+    H.drop_graph();
     if (!H.material_set) throw std::logic_error("rt_set_material has not been called");
     Level& L0 = H.lv[0];
     launch_rhop(L0.g, H.rho.as<float>(), H.eflag.as<uint8_t>(), H.p, H.rhop.as<float>(), H.s);
@@ -662,8 +717,7 @@ static int pcg_core(Hier& H, V3<TV>& x, V3<TV>& b, double tol, int maxit, int pr
     int it = 0;
     double rel = nb > 0 ? std::sqrt(rr) / nb : 0.0;
     if (nb > 0 && rel > tol) {
-        launch_fill<TC>(pc.U.data(), TC(0), n, H.s);
-        vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true);
+        precondition<TC>(H, pc, pre, post);
         double rz = dot<TV, TC>(H, r.data(), pc.U.data(), n);
         launch_convert<TC, TV>(pc.U.data(), p.data(), n, H.s);
         for (it = 1; it <= maxit; it++) {
@@ -675,8 +729,7 @@ static int pcg_core(Hier& H, V3<TV>& x, V3<TV>& b, double tol, int maxit, int pr
             rr = sum_partials_host(H, np);
             rel = std::sqrt(rr) / nb;
             if (rel <= tol) break;
-            launch_fill<TC>(pc.U.data(), TC(0), n, H.s);
-            vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true);
+            precondition<TC>(H, pc, pre, post);
             const double rz_new = dot<TV, TC>(H, r.data(), pc.U.data(), n);
             const double beta = rz_new / rz;
             rz = rz_new;
@@ -767,6 +820,7 @@ static double vcycle_on_solution(Hier& H, int pre, int post) {
 // boundary conditions
 // ---------------------------------------------------------------------------
 static void set_support(Hier& H, const uint8_t* fixed_box) {
+    H.drop_graph();
     Level& L0 = H.lv[0];
     const int nx = L0.g.n[0], ny = L0.g.n[1], nz = L0.g.n[2];
     const int ex = L0.e[0], ey = L0.e[1], ez = L0.e[2];
@@ -1030,6 +1084,7 @@ int rt_num_active(const rt_hier* h, int level, size_t* ne, size_t* nn) {
 int rt_set_material(rt_hier* h, double youngs, double poisson, float power_penalty) {
     RT_TRY({
         Hier& H = H_(h);
+        H.drop_graph();
         if (!(youngs > 0) || !(poisson > -1 && poisson < 0.5)) throw std::invalid_argument("invalid material");
         H.E = youngs; H.nu = poisson; H.p = power_penalty;
         element_matrix(youngs, poisson, H.KE);
@@ -1183,6 +1238,24 @@ int rt_get_stencil_box(const rt_hier* h, int level, double* st_box) {
         const size_t nb = L.nn_box();
         for (int k = 0; k < 243; k++)
             download_box<double>(H, L.g, 0, L.st64.as<double>() + (size_t)k * L.g.N, st_box + k * nb);
+    })
+}
+
+int rt_set_graphs(rt_hier* h, int on) {
+    RT_TRY({
+        Hier& H = H_(h);
+        H.drop_graph();
+        H.use_graphs = on != 0;
+    })
+}
+
+int rt_set_preconditioner(rt_hier* h, int coarse_sweep_mult, int gamma) {
+    RT_TRY({
+        if (coarse_sweep_mult < 1 || coarse_sweep_mult > 16 || gamma < 1 || gamma > 2)
+            throw std::invalid_argument("need 1 <= coarse_sweep_mult <= 16 and gamma in {1, 2}");
+        Hier& H = H_(h);
+        H.pc_mult = coarse_sweep_mult;
+        H.pc_gamma = gamma;
     })
 }
 
@@ -1656,14 +1729,12 @@ static double time_vcycle(Hier& H, int pre, int post, int n) {
     LevelVecs<T>& pc = pcvecs<T>(H);
     const size_t nn = (size_t)3 * H.lv[0].g.N;
     launch_convert<double, T>(H.f64.data(), pc.F.data(), nn, H.s);
+    precondition<T>(H, pc, pre, post);   // eager first application (and graph capture)
     hipEvent_t a, b;
     RT_HIP(hipEventCreate(&a));
     RT_HIP(hipEventCreate(&b));
     RT_HIP(hipEventRecord(a, H.s));
-    for (int i = 0; i < n; i++) {
-        launch_fill<T>(pc.U.data(), T(0), nn, H.s);
-        vcycle_core<T>(H, pc.U, pc.F, pc.R, pre, post, true);
-    }
+    for (int i = 0; i < n; i++) precondition<T>(H, pc, pre, post);
     RT_HIP(hipEventRecord(b, H.s));
     RT_HIP(hipEventSynchronize(b));
     float ms = 0;

@@ -243,6 +243,14 @@ class HierarchyGrid:
         check(lib().rt_vcycle(self.handle, pre_relax, post_relax, ctypes.byref(r)))
         return r.value
 
+    def set_preconditioner(self, coarse_sweep_mult=1, gamma=1):
+        """Cycle shape of the MG preconditioner used by solve()."""
+        check(lib().rt_set_preconditioner(self.handle, coarse_sweep_mult, gamma))
+
+    def set_graphs(self, on=True):
+        """Replay the preconditioner cycle as a HIP graph (default) or launch eagerly."""
+        check(lib().rt_set_graphs(self.handle, int(bool(on))))
+
     def solve(self, tol=1e-6, maxit=200, mode=_lib.RT_SOLVE_MIXED, pre=1, post=1):
         st = _lib.SolveStats()
         check(lib().rt_solve(self.handle, tol, maxit, mode, pre, post, ctypes.byref(st)))
