@@ -32,6 +32,9 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--coarse-mult", type=int, default=1, help="sweep multiplier on coarse levels (preconditioner)")
     ap.add_argument("--gamma", type=int, default=1, help="preconditioner cycle index: 1 = V, 2 = W")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the preconditioner kernels one by one instead of replaying a HIP graph "
+                         "(use under rocprofv3: its kernel tracing has aborted the queue on graph replays)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--vcycles", type=int, default=20, help="graphless V-cycles timed for the roofline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
@@ -125,6 +128,7 @@ def profile_run(a):
     import torch
     ex, ey, ez = (int(v) for v in a.size.split(","))
     hg, *_ = build_problem(ex, ey, ez, a.seed)
+    hg.set_graphs(False)
     hg.update_stencil()
     log(f"profile {ex}x{ey}x{ez}: fine sweeps")
     hg.time_fine_sweep(32, 2)
@@ -192,6 +196,7 @@ def main():
     hg, bits, R, fixed, f, rho = build_problem(ex, ey, ez, a.seed)
     log(f"hierarchy: {hg.n_levels} levels")
     hg.set_preconditioner(a.coarse_mult, a.gamma)
+    hg.set_graphs(not a.no_graphs)
     g0 = hg[0]
     ne0, nn0 = hg.num_active(0)
     levels = []

@@ -7,8 +7,8 @@ mirror of the reference's entry points.
 from ._lib import (RobtopError, RT_SOLVE_FP32, RT_SOLVE_FP64, RT_SOLVE_MIXED, RT_SOLVE_FP32_IR, LIB_PATH, lib)
 from .grid import Grid, HierarchyGrid, voxelize_mesh
 from .optimization import (MMA, Parameter, converge_criteria, computeSensitivity, updateDensities, solveFEM,
-                           ordinary_topopt)
+                           ordinary_topopt, worst_case_topopt)
 
 __all__ = ["RobtopError", "RT_SOLVE_FP32", "RT_SOLVE_FP64", "RT_SOLVE_MIXED", "RT_SOLVE_FP32_IR", "LIB_PATH", "lib", "Grid",
            "HierarchyGrid", "voxelize_mesh", "Parameter", "converge_criteria", "computeSensitivity",
-           "updateDensities", "solveFEM", "ordinary_topopt", "MMA"]
+           "updateDensities", "solveFEM", "ordinary_topopt", "worst_case_topopt", "MMA"]

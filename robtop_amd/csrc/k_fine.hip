@@ -134,10 +134,45 @@ __device__ __forceinline__ void nb_step(const Geom& g, int vb, const KeC<T>& K, 
     }
 }
 
-template <typename T, int C, bool CHECK_FIX, int... NBs>
+// two consecutive values of one stream with one load (dword-aligned 8- or 16-byte
+// buffer load)
+template <typename T> struct Two { T lo, hi; };
+template <typename T> __device__ __forceinline__ Two<T> bload2(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ Two<float> bload2<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(Two<float>, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ Two<double> bload2<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(Two<double>, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+
+// The x-neighbours (dx = -1, +1) of a row (dy, dz) have the opposite x parity,
+// so both sit in one colour block at consecutive slots (offset of dx = -1, then
+// + 1): one two-wide load fetches both. The dx = 0 neighbour is a single load.
+template <typename T, int C, bool CHECK_FIX, int ROW>
+__device__ __forceinline__ void nb_row(const Geom& g, int vb, const KeC<T>& K, const GsBufs& B, T (&acc)[8][3]) {
+    constexpr int DY = ROW % 3 - 1, DZ = ROW / 3 - 1;
+    constexpr int NBM = 9 * (DZ + 1) + 3 * (DY + 1);   // dx = -1; dx = 0, +1 follow
+    const int vo = vb * (int)sizeof(T);
+    {
+        const int so = rel_off<C, -1, DY, DZ>(g), sob = so * (int)sizeof(T);
+        const Two<T> a = bload2<T>(B.u0, vo, sob), b = bload2<T>(B.u1, vo, sob), c = bload2<T>(B.u2, vo, sob);
+        T l0 = a.lo, l1 = b.lo, l2 = c.lo, r0 = a.hi, r1 = b.hi, r2 = c.hi;
+        if constexpr (CHECK_FIX) {
+            if (bload_u8(B.flag, vb, so) & NF_FIXED) { l0 = T(0); l1 = T(0); l2 = T(0); }
+            if (bload_u8(B.flag, vb, so + 1) & NF_FIXED) { r0 = T(0); r1 = T(0); r2 = T(0); }
+        }
+        nb_elems<T, NBM>(acc, K, l0, l1, l2, std::make_integer_sequence<int, 8>{});
+        nb_elems<T, NBM + 2>(acc, K, r0, r1, r2, std::make_integer_sequence<int, 8>{});
+    }
+    nb_step<T, C, CHECK_FIX, NBM + 1>(g, vb, K, B, acc);   // skips the node itself
+}
+
+template <typename T, int C, bool CHECK_FIX, int... ROWs>
 __device__ __forceinline__ void nb_all(const Geom& g, int vb, const KeC<T>& K, const GsBufs& B, T (&acc)[8][3],
-                                       std::integer_sequence<int, NBs...>) {
-    (nb_step<T, C, CHECK_FIX, NBs>(g, vb, K, B, acc), ...);
+                                       std::integer_sequence<int, ROWs...>) {
+    (nb_row<T, C, CHECK_FIX, ROWs>(g, vb, K, B, acc), ...);
 }
 
 template <typename T, int C, int W>
@@ -172,7 +207,7 @@ __device__ __forceinline__ void gs_node(const Geom& g, int base, const KeC<T>& K
     T acc[8][3];
 #pragma unroll
     for (int w = 0; w < 8; w++) acc[w][0] = acc[w][1] = acc[w][2] = T(0);
-    nb_all<T, C, CHECK_FIX>(g, vb, K, B, acc, std::make_integer_sequence<int, 27>{});
+    nb_all<T, C, CHECK_FIX>(g, vb, K, B, acc, std::make_integer_sequence<int, 9>{});
     KU[0] = KU[1] = KU[2] = T(0);
 #pragma unroll
     for (int i = 0; i < 9; i++) S[i] = T(0);
@@ -227,8 +262,15 @@ __device__ __forceinline__ void gs_row_phase(const Geom& g, int j, int k, Vec3<T
     for (int i = threadIdx.x; i < nhx; i += blockDim.x) gs_update<T, C>(g, rowbase + i, U, F, vflag, K, B, reverse);
 }
 
+// optional occupancy target (waves per SIMD) for the smoother, a build-time knob
+#if defined(RT_GS_WAVES_PER_EU) && RT_GS_WAVES_PER_EU > 0
+#define RT_GS_ATTR __attribute__((amdgpu_waves_per_eu(RT_GS_WAVES_PER_EU)))
+#else
+#define RT_GS_ATTR
+#endif
+
 template <typename T, int CY, int CZ, bool REV>
-__global__ void __launch_bounds__(256) k_fine_gs_pair(Geom g, Vec3<T> U, Vec3<const T> F,
+__global__ void __launch_bounds__(256) RT_GS_ATTR k_fine_gs_pair(Geom g, Vec3<T> U, Vec3<const T> F,
                                                       const float* __restrict__ rhop,
                                                       const uint8_t* __restrict__ vflag, KeC<T> K) {
     const int nhy = (g.n[1] - CY + 1) >> 1;

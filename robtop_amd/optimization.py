@@ -178,3 +178,39 @@ class MMA:
         ov = ctypes.c_int()
         check(lib().rt_mma_stats(self.handle, ptr(lam), ctypes.byref(z), ctypes.byref(it), ctypes.byref(ov)))
         return dict(lambda_=lam, z=z.value, sub_iters=it.value, overflowed=bool(ov.value))
+
+
+def worst_case_topopt(hg, params, n_iter=30, max_pm=500, seed=0, init_rho=None, log=None, stop=True):
+    """The worst-case SIMP loop of optimization() (optimization.cpp:583-673):
+    Vgoal schedule, update_stencil, modified power method for the worst load
+    and its compliance, stop test (converge_criteria(1, 2, 5e-3)), sensitivity
+    of the worst compliance, filter and OC update. Load nodes and normals must
+    be set (HierarchyGrid.set_load_nodes). Returns per-iteration records."""
+    hg.init_rho(params.volume_ratio if init_rho is None else init_rho)
+    vgoal = params.volume_ratio
+    stop_check = converge_criteria(1, 2, 5e-3)
+    recs = []
+    for itn in range(1, n_iter + 1):
+        t0 = time.perf_counter()
+        vgoal *= (1 - params.volume_decrease)
+        vc = vgoal - params.volume_ratio
+        if vgoal < params.volume_ratio:
+            vgoal = params.volume_ratio
+        hg.update_stencil()
+        hg.rand_force(seed + itn)
+        c, nv = hg.modified_pm(max_pm)
+        if math.isnan(c):
+            raise FloatingPointError(f"worst compliance NaN at iteration {itn}")
+        done = stop_check.update(c, [vc]) and vgoal <= params.volume_ratio + 1e-3
+        rec = dict(iter=itn, worst_compliance=c, vgoal=vgoal, vcycles=nv)
+        if not (stop and done):
+            computeSensitivity(hg, params.filter_radius)
+            rec["multiplier"] = updateDensities(hg, vgoal, params)
+        hg.synchronize()
+        rec["seconds"] = time.perf_counter() - t0
+        recs.append(rec)
+        if log:
+            log(rec)
+        if stop and done:
+            break
+    return recs

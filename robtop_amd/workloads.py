@@ -90,3 +90,54 @@ def smoothed_random_density(eact_box, shape_xyz, seed, sigma=4.0, mean=0.4, scal
     rho = np.clip(base + 0.5 * (lo + hi), rho_min, 1.0).astype(np.float32)
     rho[~act] = 0
     return rho.reshape(-1)
+
+
+def _torus_sdf(px, py, pz, n, R_major, r_minor):
+    """Signed distance to a torus about the z axis through the centre of the
+    cube [0, n]^3; radii are fractions of the edge n."""
+    c = n / 2.0
+    q = np.hypot(px - c, py - c) - R_major * n
+    return np.hypot(q, pz - c) - r_minor * n
+
+
+def torus_bits(n, R_major=0.35, r_minor=0.12):
+    """Analytic torus voxelized at n^3 by the sign of its distance at voxel
+    centres, placed at the origin of the smallest power-of-two cube R >= n.
+    Returns (bits, R, solid [n, n, n] indexed [x, y, z])."""
+    c = np.arange(n) + 0.5
+    px, py, pz = np.meshgrid(c, c, c, indexing="ij")
+    solid = _torus_sdf(px, py, pz, n, R_major, r_minor) < 0
+    R = cube_side(n, n, n)
+    a = np.zeros((R, R, R), bool)
+    a[:n, :n, :n] = solid
+    return cube_bits(a), R, solid
+
+
+def torus_worst_case_bc(hg, n, R_major=0.35, r_minor=0.12, fixed_fraction=0.05):
+    """Supports, load nodes and load normals (finest node box) for the
+    worst-case torus: nodes of the solid within the bottom `fixed_fraction` of
+    its height are fixed; every other surface node is a load node whose
+    admissible load is along the outward normal, the normalised gradient of the
+    signed distance at the node."""
+    g0 = hg[0]
+    _, vact = g0.masks()
+    surf = g0.surface_nodes()
+    nx, ny, nz = g0.nodes
+    ox, oy, oz = g0.origin
+    zs = np.arange(nz) + oz
+    zsolid = zs[np.nonzero(vact.reshape(nz, ny, nx).any(axis=(1, 2)))[0]]
+    zcut = zsolid.min() + fixed_fraction * (zsolid.max() - zsolid.min())
+    Z = np.broadcast_to(zs[:, None, None], (nz, ny, nx)).reshape(-1)
+    fixed = vact & (Z <= zcut)
+    load = surf & ~fixed
+    X = np.broadcast_to((np.arange(nx) + ox)[None, None, :], (nz, ny, nx)).reshape(-1).astype(np.float64)
+    Y = np.broadcast_to((np.arange(ny) + oy)[None, :, None], (nz, ny, nx)).reshape(-1).astype(np.float64)
+    Zf = Z.astype(np.float64)
+    h = 1e-3
+    grad = np.stack([
+        _torus_sdf(X + h, Y, Zf, n, R_major, r_minor) - _torus_sdf(X - h, Y, Zf, n, R_major, r_minor),
+        _torus_sdf(X, Y + h, Zf, n, R_major, r_minor) - _torus_sdf(X, Y - h, Zf, n, R_major, r_minor),
+        _torus_sdf(X, Y, Zf + h, n, R_major, r_minor) - _torus_sdf(X, Y, Zf - h, n, R_major, r_minor)])
+    normals = grad / np.maximum(np.linalg.norm(grad, axis=0), 1e-30)[None]
+    normals[:, ~load] = 0.0
+    return fixed, load, normals
