@@ -32,6 +32,9 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--coarse-mult", type=int, default=1, help="sweep multiplier on coarse levels (preconditioner)")
     ap.add_argument("--gamma", type=int, default=1, help="preconditioner cycle index: 1 = V, 2 = W")
+    ap.add_argument("--mode", default="mixed", choices=["mixed", "reliable", "fp64"],
+                    help="MG-PCG variant: FP64 Krylov vectors (mixed), FP32 vectors with FP64 residual "
+                         "replacement (reliable), or all FP64")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the preconditioner kernels one by one instead of replaying a HIP graph "
                          "(use under rocprofv3: its kernel tracing has aborted the queue on graph replays)")
@@ -206,10 +209,11 @@ def main():
     coarsest_dofs = 3 * levels[-1]["nodes"]
 
     vgoal, step, damp, rmin, rfilt = 0.4, 0.03, 0.5, 1e-3, 2
+    mode = {"mixed": rt.RT_SOLVE_MIXED, "reliable": rt.RT_SOLVE_RELIABLE, "fp64": rt.RT_SOLVE_FP64}[a.mode]
 
     def simp_step():
         hg.update_stencil()
-        st = hg.solve(tol=a.tol, maxit=500, mode=rt.RT_SOLVE_MIXED, pre=a.pre, post=a.post)
+        st = hg.solve(tol=a.tol, maxit=500, mode=mode, pre=a.pre, post=a.post)
         c = hg.compliance()
         hg.compute_sensitivity()
         hg.filter_sensitivity(rfilt)
@@ -268,7 +272,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"cantilever {ex}x{ey}x{ez} voxels (R={R}), seeded smoothed-random density, "
-                        f"MG-PCG mixed V({a.pre},{a.post}) to {a.tol:g}, filter r=2, OC",
+                        f"MG-PCG {a.mode} V({a.pre},{a.post}) to {a.tol:g}, filter r=2, OC",
             "active_elements": ne0, "active_nodes": nn0, "dofs": 3 * nn0, "levels": hg.n_levels,
             "pcg_iters_per_step": float(np.mean(iters)), "rel_res": float(np.max(rels)),
             "compliance_last": comps[-1],

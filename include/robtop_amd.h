@@ -43,7 +43,9 @@ enum {
     RT_SOLVE_FP64 = 0,      /* FP64 PCG, FP64 V-cycle preconditioner */
     RT_SOLVE_FP32 = 1,      /* FP32 PCG vectors (FP64 dots), FP32 V-cycle */
     RT_SOLVE_MIXED = 2,     /* FP64 PCG vectors and dots, FP32 V-cycle preconditioner */
-    RT_SOLVE_FP32_IR = 3    /* FP64 iterative refinement (true residual) around FP32 MG-PCG */
+    RT_SOLVE_FP32_IR = 3,   /* FP64 iterative refinement (true residual) around FP32 MG-PCG */
+    RT_SOLVE_RELIABLE = 4   /* FP32 Krylov vectors and operator, FP64 solution, FP64 residual
+                               replacement whenever the FP32 residual has dropped 10x */
 };
 
 /* vector selectors for the per-level accessors */

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "librobtop_amd.so")
 _LIB = None
 
 RT_OK, RT_EARG, RT_EHIP, RT_ESTATE, RT_ENUM, RT_EINTERNAL = 0, -1, -2, -3, -4, -5
-RT_SOLVE_FP64, RT_SOLVE_FP32, RT_SOLVE_MIXED, RT_SOLVE_FP32_IR = 0, 1, 2, 3
+RT_SOLVE_FP64, RT_SOLVE_FP32, RT_SOLVE_MIXED, RT_SOLVE_FP32_IR, RT_SOLVE_RELIABLE = 0, 1, 2, 3, 4
 RT_VEC_U, RT_VEC_F, RT_VEC_R = 0, 1, 2
 
 

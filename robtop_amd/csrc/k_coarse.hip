@@ -377,6 +377,27 @@ void launch_pcg_update_x_r(T* x, T* r, const T* p, const T* q, double alpha, siz
     RT_HIP(hipGetLastError());
 }
 
+// reliable-update PCG step: x (FP64) += a p, r (FP32) -= a q; partials of r.r
+__global__ void __launch_bounds__(256) k_pcg_ru_xr(double* __restrict__ x, float* __restrict__ r,
+                                                   const float* __restrict__ p, const float* __restrict__ q,
+                                                   double alpha, size_t n, double* __restrict__ partials) {
+    const float a = (float)alpha;
+    double s = 0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        x[i] += alpha * (double)p[i];
+        const float ri = r[i] - a * q[i];
+        r[i] = ri;
+        s += (double)ri * (double)ri;
+    }
+    s = blk_sum(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+void launch_pcg_ru_update_x_r(double* x, float* r, const float* p, const float* q, double alpha, size_t n,
+                              double* partials, int nparts, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_ru_xr, dim3(nparts), dim3(256), 0, s, x, r, p, q, alpha, n, partials);
+    RT_HIP(hipGetLastError());
+}
+
 // p = z + b p
 template <typename T, typename TZ>
 __global__ void k_pcg_p(T* __restrict__ p, const TZ* __restrict__ z, double beta, size_t n) {

@@ -799,6 +799,75 @@ static void pcg_ir(Hier& H, double tol, int maxit, int pre, int post, double inn
     }
 }
 
+// PCG with FP32 Krylov vectors and FP32 operator, the solution accumulated in
+// FP64, and reliable updates (residual replacement): whenever the FP32
+// recurrence residual has dropped by 10x since the last replacement, and
+// before stopping, r is recomputed as f - K x in FP64 and the recurrence
+// continues from it. Converges like the FP64 PCG while streaming half the
+// bytes per iteration outside the V-cycle.
+static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_solve_stats* st) {
+    if (!H.stencil_ready) throw std::logic_error("update_stencil has not been called");
+    auto t0 = std::chrono::steady_clock::now();
+    Level& L0 = H.lv[0];
+    const int N = L0.g.N;
+    const size_t n = (size_t)3 * N;
+    if (!H.r64.ok()) H.r64.alloc(N);
+    if (!H.q64.ok()) H.q64.alloc(N);
+    if (!H.p32.ok()) H.p32.alloc(N);
+    if (!H.q32.ok()) H.q32.alloc(N);
+    LevelVecs<float>& pc = pcvecs<float>(H);   // pc.F holds the FP32 residual, pc.U the preconditioned one
+    const int np = 1024;
+    const double xAx = apply_dot<double>(H, H.x64, H.q64);
+    if (xAx > 0) {
+        const double xb = dot<double, double>(H, H.x64.data(), H.f64.data(), n);
+        const double alpha = xb > 0 ? xb / xAx : 0.0;
+        launch_scale<double>(H.x64.data(), alpha, n, H.s);
+        launch_scale<double>(H.q64.data(), alpha, n, H.s);
+    }
+    launch_sub_conv<double, float>(H.f64.data(), H.q64.data(), H.r64.data(), pc.F.data(), n, H.partials.as<double>(),
+                                   np, H.s);
+    double rr = sum_partials_host(H, np);
+    const double nb = std::sqrt(dot<double, double>(H, H.f64.data(), H.f64.data(), n));
+    double rel = nb > 0 ? std::sqrt(rr) / nb : 0.0, rel_true = rel, rmax = std::sqrt(rr);
+    int it = 0;
+    if (nb > 0 && rel > tol) {
+        precondition<float>(H, pc, pre, post);
+        double rz = dot<float, float>(H, pc.F.data(), pc.U.data(), n);
+        launch_convert<float, float>(pc.U.data(), H.p32.data(), n, H.s);
+        for (it = 1; it <= maxit; it++) {
+            const double pq = apply_dot<float>(H, H.p32, H.q32);
+            if (!(pq > 0)) throw std::domain_error("PCG breakdown: p.Ap <= 0");
+            const double alpha = rz / pq;
+            launch_pcg_ru_update_x_r(H.x64.data(), pc.F.data(), H.p32.data(), H.q32.data(), alpha, n,
+                                     H.partials.as<double>(), np, H.s);
+            const double rn = std::sqrt(sum_partials_host(H, np));
+            rel = rn / nb;
+            if (rel <= tol || rn < 0.1 * rmax) {
+                rel_true = true_rel_residual(H);      // r64 = f - K x64
+                launch_convert<double, float>(H.r64.data(), pc.F.data(), n, H.s);
+                rmax = rel_true * nb;
+                rel = rel_true;
+                if (rel_true <= tol) break;
+            }
+            precondition<float>(H, pc, pre, post);
+            const double rz_new = dot<float, float>(H, pc.F.data(), pc.U.data(), n);
+            const double beta = rz_new / rz;
+            rz = rz_new;
+            launch_pcg_update_p<float, float>(H.p32.data(), pc.U.data(), beta, n, H.s);
+        }
+        if (it > maxit) it = maxit;
+    }
+    RT_HIP(hipStreamSynchronize(H.s));
+    const double tr = true_rel_residual(H);
+    auto t1 = std::chrono::steady_clock::now();
+    if (st) {
+        st->iterations = it;
+        st->rel_res = tr;
+        st->rel_res_pcg = rel;
+        st->seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+}
+
 // reference-semantics V-cycle on the finest displacement / force
 template <typename T>
 static double vcycle_on_solution(Hier& H, int pre, int post) {
@@ -1406,6 +1475,7 @@ int rt_solve(rt_hier* h, double tol, int maxit, int mode, int pre, int post, rt_
         else if (mode == RT_SOLVE_FP32) pcg<float, float>(H, tol, maxit, pre, post, stats);
         else if (mode == RT_SOLVE_MIXED) pcg<double, float>(H, tol, maxit, pre, post, stats);
         else if (mode == RT_SOLVE_FP32_IR) pcg_ir(H, tol, maxit, pre, post, 1e-4, stats);
+        else if (mode == RT_SOLVE_RELIABLE) pcg_reliable(H, tol, maxit, pre, post, stats);
         else throw std::invalid_argument("unknown solver mode");
     })
 }

@@ -117,6 +117,8 @@ void launch_sum_partials(const double* partials, int nparts, double* out, hipStr
 template <typename T, typename TZ>
 void launch_pcg_update_x_r(T* x, T* r, const T* p, const T* q, double alpha, size_t n, double* partials, int nparts,
                            TZ* rz_out, hipStream_t s);
+void launch_pcg_ru_update_x_r(double* x, float* r, const float* p, const float* q, double alpha, size_t n,
+                              double* partials, int nparts, hipStream_t s);
 template <typename T, typename TZ>
 void launch_pcg_update_p(T* p, const TZ* z, double beta, size_t n, hipStream_t s);
 template <typename T, typename TZ>

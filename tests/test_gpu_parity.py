@@ -215,7 +215,8 @@ def test_vcycle(pair, bits):
     g.set_precision(64)
 
 
-@pytest.mark.parametrize("mode", [rt.RT_SOLVE_FP64, rt.RT_SOLVE_MIXED, rt.RT_SOLVE_FP32_IR, rt.RT_SOLVE_FP32])
+@pytest.mark.parametrize("mode", [rt.RT_SOLVE_FP64, rt.RT_SOLVE_MIXED, rt.RT_SOLVE_FP32_IR, rt.RT_SOLVE_FP32,
+                                  rt.RT_SOLVE_RELIABLE])
 def test_pcg_solution(pair, mode):
     """MG-PCG to 1e-8 against the oracle's V-cycle iteration converged to 1e-12.
     Plain FP32 PCG stops on its FP32 recurrence; its true residual is only
