@@ -2,6 +2,8 @@
 // deterministic FP64 reductions and the PCG vector updates.
 #include "rt_internal.h"
 
+#include <utility>
+
 namespace rt {
 
 // ---------------------------------------------------------------------------
@@ -36,9 +38,63 @@ static inline unsigned colour_blocks(const Geom& g, int colour, int bs) {
 template <typename TS> __device__ __forceinline__ double ld_st(const TS* p) { return (double)*p; }
 
 // ---------------------------------------------------------------------------
+// stored-stencil operator of the coarse levels
+// ---------------------------------------------------------------------------
+// Au over the stencil blocks of node v (colour C, half-index base), with raw
+// buffer loads at wave-uniform scalar offsets. The 27 neighbours are taken in
+// three planes of nine (dz = -1, 0, 1): a plane's 9 x (3 + 9) loads are issued
+// together into registers, a scheduling barrier keeps them ahead of the
+// plane's FMAs, so each plane costs one memory latency rather than one per
+// load. CENTRE selects whether the centre block (nb = 13) is included
+// (residual) or left to the caller (smoother).
+template <typename T, typename TS, bool CENTRE, int DZ>
+__device__ __forceinline__ void coarse_apply_plane(const Geom& g, int C, int vbu, int vbs, int sh,
+                                                   const __amdgpu_buffer_rsrc_t (&ru)[3], const TS* __restrict__ st,
+                                                   T (&Au)[3]) {
+    const int cx = C & 1, cy = (C >> 1) & 1, cz = C >> 2;
+    const int N = g.N;
+    T u[9][3];
+    TS a[9][9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int nb = 9 * (DZ + 1) + k;
+        if (!CENTRE && nb == 13) continue;
+        const int dx = k % 3 - 1, dy = k / 3 - 1;
+        const int col = C ^ ((dx & 1) | ((dy & 1) << 1) | ((DZ & 1) << 2));
+        const int so = (col * g.cs + (((cz + DZ) >> 1) * g.h[1] + ((cy + dy) >> 1)) * g.h[0] + ((cx + dx) >> 1) + sh) *
+                       (int)sizeof(T);
+#pragma unroll
+        for (int c = 0; c < 3; c++) u[k][c] = bload<T>(ru[c], vbu, so);
+        const __amdgpu_buffer_rsrc_t rs = rsrc(st + (size_t)nb * 9 * N, 9 * N * (int)sizeof(TS));
+#pragma unroll
+        for (int q = 0; q < 9; q++) a[k][q] = bload<TS>(rs, vbs, q * N * (int)sizeof(TS));
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int nb = 9 * (DZ + 1) + k;
+        if (!CENTRE && nb == 13) continue;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            Au[r] += (T)a[k][3 * r + 0] * u[k][0] + (T)a[k][3 * r + 1] * u[k][1] + (T)a[k][3 * r + 2] * u[k][2];
+    }
+}
+
+template <typename T, typename TS, bool CENTRE = true>
+__device__ __forceinline__ void coarse_apply(const Geom& g, int C, int base, int v, Vec3<const T> U,
+                                             const TS* __restrict__ st, T (&Au)[3]) {
+    const int N = g.N;
+    const int sh = g.h[0] * g.h[1] + g.h[0] + 1;                 // keeps the neighbour offsets >= 0
+    const int vbu = (base - sh) * (int)sizeof(T), vbs = v * (int)sizeof(TS);
+    const __amdgpu_buffer_rsrc_t ru[3] = {rsrc(U.c[0], N * (int)sizeof(T)), rsrc(U.c[1], N * (int)sizeof(T)),
+                                          rsrc(U.c[2], N * (int)sizeof(T))};
+    coarse_apply_plane<T, TS, CENTRE, -1>(g, C, vbu, vbs, sh, ru, st, Au);
+    coarse_apply_plane<T, TS, CENTRE, 0>(g, C, vbu, vbs, sh, ru, st, Au);
+    coarse_apply_plane<T, TS, CENTRE, 1>(g, C, vbu, vbs, sh, ru, st, Au);
+}
+
 // coarse smoother: Grid.cu:678-790. Au over the 26 off-centre stencil blocks,
 // then component-wise GS with the centre block (stencil index (nb*9 + 3r + q)).
-// ---------------------------------------------------------------------------
 template <typename T, typename TS, int C>
 __device__ __forceinline__ void coarse_gs_body(const Geom& g, Vec3<T> U, Vec3<const T> F, const TS* __restrict__ st,
                                                const uint8_t* __restrict__ vflag, bool reverse) {
@@ -48,19 +104,7 @@ __device__ __forceinline__ void coarse_gs_body(const Geom& g, Vec3<T> U, Vec3<co
     if (!(vflag[v] & NF_ACTIVE)) return;
     const size_t N = (size_t)g.N;
     T Au[3] = {T(0), T(0), T(0)};
-#pragma unroll
-    for (int nb = 0; nb < 27; nb++) {
-        if (nb == 13) continue;
-        const int s = c_nb_slot<C>(g, base, nb);
-        const T u0 = U.c[0][s], u1 = U.c[1][s], u2 = U.c[2][s];
-        const TS* p = st + (size_t)nb * 9 * N + v;
-#pragma unroll
-        for (int r = 0; r < 3; r++) {
-            Au[r] += (T)p[(3 * r + 0) * N] * u0;
-            Au[r] += (T)p[(3 * r + 1) * N] * u1;
-            Au[r] += (T)p[(3 * r + 2) * N] * u2;
-        }
-    }
+    coarse_apply<T, TS, false>(g, C, base, v, Vec3<const T>{{U.c[0], U.c[1], U.c[2]}}, st, Au);
     const TS* sc = st + (size_t)13 * 9 * N + v;
     T S[9];
 #pragma unroll
@@ -98,41 +142,27 @@ void launch_coarse_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const TS* st, c
     RT_HIP(hipGetLastError());
 }
 
-// coarse residual: Grid.cu:1349-1448
-template <typename T, typename TS, int C>
-__device__ __forceinline__ void coarse_res_body(const Geom& g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R,
-                                                const TS* __restrict__ st, const uint8_t* __restrict__ vflag) {
-    int base;
-    if (!c_colour_thread<C>(g, blockIdx.x * blockDim.x + threadIdx.x, base)) return;
-    const int v = C * g.cs + base;
-    if (!(vflag[v] & NF_ACTIVE)) return;
-    const size_t N = (size_t)g.N;
-    T Au[3] = {T(0), T(0), T(0)};
-#pragma unroll
-    for (int nb = 0; nb < 27; nb++) {
-        const int s = c_nb_slot<C>(g, base, nb);
-        const T u0 = U.c[0][s], u1 = U.c[1][s], u2 = U.c[2][s];
-        const TS* p = st + (size_t)nb * 9 * N + v;
-#pragma unroll
-        for (int r = 0; r < 3; r++) {
-            Au[r] += (T)p[(3 * r + 0) * N] * u0;
-            Au[r] += (T)p[(3 * r + 1) * N] * u1;
-            Au[r] += (T)p[(3 * r + 2) * N] * u2;
-        }
-    }
-    R.c[0][v] = F.c[0][v] - Au[0];
-    R.c[1][v] = F.c[1][v] - Au[1];
-    R.c[2][v] = F.c[2][v] - Au[2];
-}
-
+// coarse residual: Grid.cu:1349-1448. All colours in one launch (blockIdx.y =
+// colour). Loads are raw buffer loads whose neighbour / stencil-component
+// offsets are wave-uniform scalars, so all 27 x 12 loads of a node are issued
+// back to back.
 template <typename T, typename TS>
 __global__ void __launch_bounds__(256) k_coarse_res(Geom g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R,
                                                     const TS* __restrict__ st, const uint8_t* __restrict__ vflag) {
-    switch (blockIdx.y) {
-#define RT_CASE(c) case c: coarse_res_body<T, TS, c>(g, U, F, R, st, vflag); break;
-        RT_CASE(0) RT_CASE(1) RT_CASE(2) RT_CASE(3) RT_CASE(4) RT_CASE(5) RT_CASE(6) RT_CASE(7)
-#undef RT_CASE
-    }
+    const int C = blockIdx.y;
+    const int cx = C & 1, cy = (C >> 1) & 1, cz = C >> 2;
+    const int nhx = (g.n[0] - cx + 1) >> 1, nhy = (g.n[1] - cy + 1) >> 1, nhz = (g.n[2] - cz + 1) >> 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nhx * nhy * nhz) return;
+    const int rr = t / nhx;
+    const int base = ((rr / nhy + 1) * g.h[1] + (rr % nhy + 1)) * g.h[0] + (t % nhx + 1);
+    const int v = C * g.cs + base;
+    if (!(vflag[v] & NF_ACTIVE)) return;
+    T Au[3] = {T(0), T(0), T(0)};
+    coarse_apply<T, TS>(g, C, base, v, U, st, Au);
+    R.c[0][v] = F.c[0][v] - Au[0];
+    R.c[1][v] = F.c[1][v] - Au[1];
+    R.c[2][v] = F.c[2][v] - Au[2];
 }
 
 template <typename T, typename TS>
@@ -204,13 +234,14 @@ void launch_restrict(const Geom& gf, const int of[3], Vec3<const T> Rf, const Ge
 template <typename T, int RATIO>
 __global__ void __launch_bounds__(256) k_prolong(Geom gf, int ofx, int ofy, int ofz, Vec3<T> Uf,
                                                  const uint8_t* __restrict__ vflag_f, Geom gc, int ocx, int ocy,
-                                                 int ocz, Vec3<const T> Uc) {
+                                                 int ocz, Vec3<const T> Uc, int skip_fixed) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nx = gf.n[0], ny = gf.n[1], nz = gf.n[2];
     if (t >= nx * ny * nz) return;
     const int x = t % nx, y = (t / nx) % ny, z = t / (nx * ny);
     const int v = oct_p(gf, x, y, z);
-    if (!(vflag_f[v] & NF_ACTIVE)) return;
+    const uint8_t fl = vflag_f[v];
+    if (!(fl & NF_ACTIVE) || (skip_fixed && (fl & NF_FIXED))) return;
     const int ax = ofx + x, ay = ofy + y, az = ofz + z;
     const int px = ax % RATIO, py = ay % RATIO, pz = az % RATIO;
     const int bx = ax / RATIO - ocx, by = ay / RATIO - ocy, bz = az / RATIO - ocz;
@@ -234,15 +265,15 @@ __global__ void __launch_bounds__(256) k_prolong(Geom gf, int ofx, int ofy, int 
 
 template <typename T>
 void launch_prolong(const Geom& gf, const int of[3], Vec3<T> Uf, const uint8_t* vflag_f, const Geom& gc,
-                    const int oc[3], Vec3<const T> Uc, int ratio, bool, hipStream_t s) {
+                    const int oc[3], Vec3<const T> Uc, int ratio, bool skip_fixed, hipStream_t s) {
     long long tot = (long long)gf.n[0] * gf.n[1] * gf.n[2];
     dim3 grid((unsigned)((tot + 255) / 256));
     if (ratio == 4)
         hipLaunchKernelGGL((k_prolong<T, 4>), grid, dim3(256), 0, s, gf, of[0], of[1], of[2], Uf, vflag_f, gc, oc[0],
-                           oc[1], oc[2], Uc);
+                           oc[1], oc[2], Uc, (int)skip_fixed);
     else
         hipLaunchKernelGGL((k_prolong<T, 2>), grid, dim3(256), 0, s, gf, of[0], of[1], of[2], Uf, vflag_f, gc, oc[0],
-                           oc[1], oc[2], Uc);
+                           oc[1], oc[2], Uc, (int)skip_fixed);
     RT_HIP(hipGetLastError());
 }
 

@@ -52,22 +52,6 @@ static KeC<T> kec() {
     return k;
 }
 
-template <typename T> __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, int voff, int soff);
-template <>
-__device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-}
-template <>
-__device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-}
-__device__ __forceinline__ uint8_t bload_u8(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-    return __builtin_amdgcn_raw_buffer_load_b8(r, voff, soff, 0);
-}
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, int bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes, 0x00020000);
-}
-
 // offset (in slots, shifted to be >= 0) of the point at (DX,DY,DZ) in {-1,0,1}^3
 // from a colour-C node whose half-index base is b: slot = (b - hxy - hx - 1) + off
 template <int C, int DX, int DY, int DZ>
@@ -215,7 +199,10 @@ __device__ __forceinline__ void gs_node(const Geom& g, int base, const KeC<T>& K
 }
 
 // GS update of the colour-C node with half-index base (Grid.cu:1007-1062)
-template <typename T, int C>
+// ZS ("zero supports"): the caller guarantees u = 0 at every support node for
+// the whole sweep (f = 0 there, so the relaxation keeps it 0, and the
+// prolongation skips supports), so a neighbour never needs its flag checked.
+template <typename T, int C, bool ZS>
 __device__ __forceinline__ void gs_update(const Geom& g, int base, Vec3<T> U, Vec3<const T> F,
                                           const uint8_t* __restrict__ vflag, const KeC<T>& K, const GsBufs& B,
                                           bool reverse) {
@@ -230,7 +217,7 @@ __device__ __forceinline__ void gs_update(const Geom& g, int base, Vec3<T> U, Ve
         return;
     }
     T KU[3], S[9];
-    if (fl & NF_NEARFIX) gs_node<T, C, true>(g, base, K, B, KU, S);
+    if (!ZS && (fl & NF_NEARFIX)) gs_node<T, C, true>(g, base, K, B, KU, S);
     else gs_node<T, C, false>(g, base, K, B, KU, S);
     T u[3] = {U.c[0][v], U.c[1][v], U.c[2][v]};
     // component-wise Gauss-Seidel with the 3x3 self block (Grid.cu:1056-1062)
@@ -253,14 +240,22 @@ __device__ __forceinline__ void gs_update(const Geom& g, int base, Vec3<T> U, Ve
 // Every other neighbour belongs to a row of another colour pair and is
 // read-only during the launch. Four launches make a sweep, in the reference's
 // colour order 0..7 (or 7..0).
-template <typename T, int C>
+template <typename T, int C, bool ZS>
 __device__ __forceinline__ void gs_row_phase(const Geom& g, int j, int k, Vec3<T> U, Vec3<const T> F,
                                              const uint8_t* __restrict__ vflag, const KeC<T>& K, const GsBufs& B,
                                              bool reverse) {
     const int nhx = (g.n[0] - (C & 1) + 1) >> 1;
     const int rowbase = ((k + 1) * g.h[1] + (j + 1)) * g.h[0] + 1;
-    for (int i = threadIdx.x; i < nhx; i += blockDim.x) gs_update<T, C>(g, rowbase + i, U, F, vflag, K, B, reverse);
+    for (int i = threadIdx.x; i < nhx; i += blockDim.x)
+        gs_update<T, C, ZS>(g, rowbase + i, U, F, vflag, K, B, reverse);
 }
+
+// rows relaxed one after another by each workgroup of the smoother: rows of one
+// launch never neighbour each other, and consecutive rows (y, y + 2) share their
+// neighbour row y + 1, which the second row then finds in cache
+#ifndef RT_GS_ROWS
+#define RT_GS_ROWS 2
+#endif
 
 // optional occupancy target (waves per SIMD) for the smoother, a build-time knob
 #if defined(RT_GS_WAVES_PER_EU) && RT_GS_WAVES_PER_EU > 0
@@ -269,42 +264,48 @@ __device__ __forceinline__ void gs_row_phase(const Geom& g, int j, int k, Vec3<T
 #define RT_GS_ATTR
 #endif
 
-template <typename T, int CY, int CZ, bool REV>
+template <typename T, int CY, int CZ, bool REV, bool ZS>
 __global__ void __launch_bounds__(256) RT_GS_ATTR k_fine_gs_pair(Geom g, Vec3<T> U, Vec3<const T> F,
                                                       const float* __restrict__ rhop,
-                                                      const uint8_t* __restrict__ vflag, KeC<T> K) {
+                                                      const uint8_t* __restrict__ vflag, KeC<T> K, int nrows) {
     const int nhy = (g.n[1] - CY + 1) >> 1;
-    const int row = xcd_row(blockIdx.x, gridDim.x);
-    const int j = row % nhy, k = row / nhy;
+    const int first = xcd_row(blockIdx.x, gridDim.x) * RT_GS_ROWS;
     const int nbytes = g.N * (int)sizeof(T);
     const GsBufs B = {rsrc(U.c[0], nbytes), rsrc(U.c[1], nbytes), rsrc(U.c[2], nbytes), rsrc(rhop, g.N * 4),
                       rsrc(vflag, g.N)};
     constexpr int CA = (REV ? 1 : 0) | (CY << 1) | (CZ << 2);
-    gs_row_phase<T, CA>(g, j, k, U, F, vflag, K, B, REV);
-    __syncthreads();
-    gs_row_phase<T, CA ^ 1>(g, j, k, U, F, vflag, K, B, REV);
+    for (int r = 0; r < RT_GS_ROWS; r++) {
+        const int row = first + r;   // uniform across the workgroup
+        if (row >= nrows) break;
+        const int j = row % nhy, k = row / nhy;
+        gs_row_phase<T, CA, ZS>(g, j, k, U, F, vflag, K, B, REV);
+        __syncthreads();
+        gs_row_phase<T, CA ^ 1, ZS>(g, j, k, U, F, vflag, K, B, REV);
+    }
 }
 
 // pair p = 0..3 relaxes colours (2p, 2p+1), or (2p+1, 2p) when reverse
 template <typename T>
 void launch_fine_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* rhop, const uint8_t* vflag, int pair,
-                    bool reverse, hipStream_t s) {
+                    bool reverse, bool zero_support, hipStream_t s) {
     const int cy = pair & 1, cz = pair >> 1;
     const long long rows = (long long)((g.n[1] - cy + 1) >> 1) * ((g.n[2] - cz + 1) >> 1);
     if (rows <= 0) return;
-    dim3 grid((unsigned)rows);
+    dim3 grid((unsigned)((rows + RT_GS_ROWS - 1) / RT_GS_ROWS));
     const KeC<T> K = kec<T>();
-    switch (pair * 2 + (reverse ? 1 : 0)) {
-#define RT_CASE(p, r) case p * 2 + r: hipLaunchKernelGGL((k_fine_gs_pair<T, ((p) & 1), ((p) >> 1), (r) != 0>), grid, dim3(256), 0, s, g, U, F, rhop, vflag, K); break;
-        RT_CASE(0, 0) RT_CASE(0, 1) RT_CASE(1, 0) RT_CASE(1, 1) RT_CASE(2, 0) RT_CASE(2, 1) RT_CASE(3, 0) RT_CASE(3, 1)
+    switch (pair * 4 + (reverse ? 2 : 0) + (zero_support ? 1 : 0)) {
+#define RT_CASE(p, r, z) case p * 4 + r * 2 + z: hipLaunchKernelGGL((k_fine_gs_pair<T, ((p) & 1), ((p) >> 1), (r) != 0, (z) != 0>), grid, dim3(256), 0, s, g, U, F, rhop, vflag, K, (int)rows); break;
+        RT_CASE(0, 0, 0) RT_CASE(0, 1, 0) RT_CASE(1, 0, 0) RT_CASE(1, 1, 0) RT_CASE(2, 0, 0) RT_CASE(2, 1, 0)
+        RT_CASE(3, 0, 0) RT_CASE(3, 1, 0) RT_CASE(0, 0, 1) RT_CASE(0, 1, 1) RT_CASE(1, 0, 1) RT_CASE(1, 1, 1)
+        RT_CASE(2, 0, 1) RT_CASE(2, 1, 1) RT_CASE(3, 0, 1) RT_CASE(3, 1, 1)
 #undef RT_CASE
     }
     RT_HIP(hipGetLastError());
 }
 
 template void launch_fine_gs<float>(const Geom&, Vec3<float>, Vec3<const float>, const float*, const uint8_t*, int,
-                                    bool, hipStream_t);
+                                    bool, bool, hipStream_t);
 template void launch_fine_gs<double>(const Geom&, Vec3<double>, Vec3<const double>, const float*, const uint8_t*, int,
-                                     bool, hipStream_t);
+                                     bool, bool, hipStream_t);
 
 }  // namespace rt

@@ -394,14 +394,16 @@ template <> LevelVecs<float>& pcvecs<float>(Hier& H) { if (!H.pc32.ok()) H.pc32.
 // multigrid operations (level index li into H.lv; the finest level is 0)
 // ---------------------------------------------------------------------------
 template <typename T>
-static void gs(Hier& H, int li, Vec3<T> U, Vec3<const T> F, int n, bool reverse) {
+// zero_support: u and f are 0 at every support node of the finest level (the
+// preconditioner's case), so the smoother skips neighbour support checks
+static void gs(Hier& H, int li, Vec3<T> U, Vec3<const T> F, int n, bool reverse, bool zero_support = false) {
     Level& L = H.lv[li];
     for (int it = 0; it < n; it++) {
         if (li == 0) {
             // four launches, each relaxing the colour pair (2p, 2p+1) row by row
             for (int pp = 0; pp < 4; pp++)
                 launch_fine_gs<T>(L.g, U, F, H.rhop.as<float>(), L.vflag.as<uint8_t>(), reverse ? 3 - pp : pp,
-                                  reverse, H.s);
+                                  reverse, zero_support, H.s);
         } else {
             for (int cc = 0; cc < 8; cc++)
                 launch_coarse_gs<T, T>(L.g, U, F, stencil<T>(L), L.vflag.as<uint8_t>(), reverse ? 7 - cc : cc,
@@ -427,10 +429,13 @@ static void restrict_to(Hier& H, int li, Vec3<const T> Rf, Vec3<T> Fc) {
 }
 
 template <typename T>
-static void prolong_from(Hier& H, int li, Vec3<T> Uf, Vec3<const T> Uc) {
+// skip_fixed leaves support nodes untouched; the smoother that follows resets
+// them to f / n_e and their neighbours ignore them, so the cycle's result is
+// the same either way (Grid.cu:1818-1895 corrects them too)
+static void prolong_from(Hier& H, int li, Vec3<T> Uf, Vec3<const T> Uc, bool skip_fixed = false) {
     Level& Fl = H.lv[li];
     Level& C = H.lv[li + 1];
-    launch_prolong<T>(Fl.g, Fl.o, Uf, Fl.vflag.as<uint8_t>(), C.g, C.o, Uc, C.scale / Fl.scale, false, H.s);
+    launch_prolong<T>(Fl.g, Fl.o, Uf, Fl.vflag.as<uint8_t>(), C.g, C.o, Uc, C.scale / Fl.scale, skip_fixed, H.s);
 }
 
 template <typename T>
@@ -441,8 +446,11 @@ static void coarsest(Hier& H, Vec3<const T> F, Vec3<T> U) {
 
 // V(pre, post) with caller-owned finest vectors (Grid.cpp:1068-1113 ordering).
 template <typename T>
+// zero_support: F0 and U0 vanish at support nodes (the PCG preconditioner), so
+// the finest smoother runs without neighbour support checks and the
+// prolongation keeps supports at 0
 static void vcycle_core(Hier& H, V3<T>& U0, V3<T>& F0, V3<T>& R0, int pre, int post, bool symmetric,
-                        int coarse_mult = 1, int gamma = 1) {
+                        int coarse_mult = 1, int gamma = 1, bool zero_support = false) {
     const int n = (int)H.lv.size();
     auto Uv = [&](int i) { return i == 0 ? U0.v() : vecs<T>(H.lv[i]).U.v(); };
     auto Fv = [&](int i) { return i == 0 ? F0.cv() : vecs<T>(H.lv[i]).F.cv(); };
@@ -457,15 +465,16 @@ static void vcycle_core(Hier& H, V3<T>& U0, V3<T>& F0, V3<T>& R0, int pre, int p
             return;
         }
         const int m = i == 0 ? 1 : coarse_mult;
-        gs<T>(H, i, Uv(i), Fv(i), pre * m, false);
+        const bool zs = i == 0 && zero_support;
+        gs<T>(H, i, Uv(i), Fv(i), pre * m, false, zs);
         residual<T>(H, i, cvt(Uv(i)), Fv(i), Rv(i));
         LevelVecs<T>& lvv = vecs<T>(H.lv[i + 1]);
         restrict_to<T>(H, i + 1, cvt(Rv(i)), lvv.F.v());
         launch_fill<T>(lvv.U.data(), T(0), (size_t)3 * lvv.U.N, H.s);
         const int visits = i + 1 == n - 1 ? 1 : gamma;
         for (int v = 0; v < visits; v++) self(self, i + 1);
-        prolong_from<T>(H, i, Uv(i), cvt(Uv(i + 1)));
-        gs<T>(H, i, Uv(i), Fv(i), post * m, symmetric);
+        prolong_from<T>(H, i, Uv(i), cvt(Uv(i + 1)), zs);
+        gs<T>(H, i, Uv(i), Fv(i), post * m, symmetric, zs);
     };
     cyc(cyc, 0);
 }
@@ -484,14 +493,14 @@ static void precondition(Hier& H, LevelVecs<TC>& pc, int pre, int post) {
         return;
     }
     launch_fill<TC>(pc.U.data(), TC(0), n, H.s);
-    vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma);
+    vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma, true);
     if (!H.use_graphs) return;
     H.drop_graph();
     RT_HIP(hipStreamBeginCapture(H.s, hipStreamCaptureModeThreadLocal));
     hipGraph_t g = nullptr;
     try {
         launch_fill<TC>(pc.U.data(), TC(0), n, H.s);
-        vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma);
+        vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma, true);
     } catch (...) {
         (void)hipStreamEndCapture(H.s, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -1833,7 +1842,7 @@ static double time_sweep(Hier& H, int n) {
     RT_HIP(hipEventCreate(&a));
     RT_HIP(hipEventCreate(&b));
     RT_HIP(hipEventRecord(a, H.s));
-    gs<T>(H, 0, pc.U.v(), pc.F.cv(), n, false);
+    gs<T>(H, 0, pc.U.v(), pc.F.cv(), n, false, true);   // the preconditioner's smoother (f = 0 at supports)
     RT_HIP(hipEventRecord(b, H.s));
     RT_HIP(hipEventSynchronize(b));
     float ms = 0;

@@ -71,6 +71,26 @@ __device__ __forceinline__ int xcd_row(int b, int n) {
     return x * q + (x < r ? x : r) + loc;
 }
 
+// Raw buffer loads: voff is the per-lane byte offset, soff a wave-uniform byte
+// offset held in a scalar register, so a kernel's many neighbour loads need no
+// per-load address arithmetic. Both offsets must be non-negative.
+template <typename T> __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ uint8_t bload_u8(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_amdgcn_raw_buffer_load_b8(r, voff, soff, 0);
+}
+// buffer resource over `bytes` bytes from p (loads beyond them return 0)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes, 0x00020000);
+}
+
 // ---------------------------------------------------------------------------
 // kernel launchers (defined in the .hip files); all launch on stream `s`
 // ---------------------------------------------------------------------------
@@ -78,7 +98,7 @@ __device__ __forceinline__ int xcd_row(int b, int n) {
 void set_ke_constants(const double* ke576, hipStream_t s);
 template <typename T>
 void launch_fine_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* rhop, const uint8_t* vflag, int pair,
-                    bool reverse, hipStream_t s);
+                    bool reverse, bool zero_support, hipStream_t s);
 
 // k_fine_modal.hip (element-centric residual / apply in the sum-difference basis)
 void set_modal_constants(const double* ke576);
