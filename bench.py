@@ -51,24 +51,31 @@ def parse():
     return ap.parse_args()
 
 
-def algorithmic_vcycle_bytes(levels, pre, post, coarsest_dofs, P=4):
-    """Compulsory bytes of one V(pre, post): each array element an operation
-    must read or write, counted once per operation; u/f/r in FP32, stencils in
-    FP32, 1 flag byte per node, rho^p 4 B per element."""
+def algorithmic_vcycle_bytes(levels, pre, post, coarsest_dofs, P=4, mult=1, gamma=1):
+    """Compulsory bytes of one preconditioner cycle: each array element an
+    operation must read or write, counted once per operation; u/f/r in FP32,
+    stencils in FP32, 1 flag byte per node, rho^p 4 B per element. Coarse
+    levels smooth `mult` times pre/post sweeps and level i > 0 is visited
+    gamma^(i-1) times (the coarsest once per visit of its parent)."""
     n0, e0 = levels[0]["nodes"], levels[0]["elements"]
     sweep0 = n0 * (9 * P + 1) + 4 * e0
     res0 = sweep0
     total = (pre + post) * sweep0 + res0
     nlev = len(levels)
+    visits = 1
     for i in range(1, nlev):
         nf, nc = levels[i - 1]["nodes"], levels[i]["nodes"]
-        total += 3 * P * nf + 3 * P * nc            # restrict (read r_f, write f_c)
-        total += 3 * P * nc + nf * (6 * P + 1)      # prolong (read u_c, read+write u_f, flag)
+        per_visit = 3 * P * nf + 3 * P * nc            # restrict (read r_f, write f_c)
+        per_visit += 3 * P * nc + nf * (6 * P + 1)     # prolong (read u_c, read+write u_f, flag)
         if i < nlev - 1:
             nl = levels[i]["nodes"]
             per = nl * (243 * P + 9 * P + 1)
-            total += (pre + post) * per + per       # sweeps + residual
-    total += coarsest_dofs ** 2 * 8
+            per_visit += mult * (pre + post) * per + per   # sweeps + residual
+        else:
+            per_visit += coarsest_dofs ** 2 * 8
+        total += visits * per_visit
+        if i < nlev - 2:
+            visits *= gamma
     return total
 
 
@@ -240,7 +247,7 @@ def main():
     hg.update_stencil()
     t_v = hg.time_vcycle(32, a.pre, a.post, a.vcycles)
     t_sweep = hg.time_fine_sweep(32, 10)
-    B_V = algorithmic_vcycle_bytes(levels, a.pre, a.post, coarsest_dofs)
+    B_V = algorithmic_vcycle_bytes(levels, a.pre, a.post, coarsest_dofs, mult=a.coarse_mult, gamma=a.gamma)
     # a sweep is four launches, each relaxing two colours (half the rows of the box
     # in y-z parity class, both x parities); each launch moves a quarter of the
     # sweep's compulsory bytes: u (read + write), f, flag for its nodes and, over the
@@ -278,7 +285,8 @@ def main():
             "compliance_last": comps[-1],
         },
         "vcycle": {"ms": t_v, "algorithmic_bytes": B_V, "GBps": B_V / (t_v * 1e-3) / 1e9,
-                   "frac_of_peak": B_V / (t_v * 1e-3) / 1e9 / HBM_PEAK_GBS, "pre": a.pre, "post": a.post},
+                   "frac_of_peak": B_V / (t_v * 1e-3) / 1e9 / HBM_PEAK_GBS, "pre": a.pre, "post": a.post,
+                   "coarse_mult": a.coarse_mult, "gamma": a.gamma},
         "roofline": {"kernel": "k_fine_gs_pair (one colour-pair launch of the finest-level smoother)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": launch_bytes, "launch_ms": launch_ms},
