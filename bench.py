@@ -32,7 +32,7 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--coarse-mult", type=int, default=1, help="sweep multiplier on coarse levels (preconditioner)")
     ap.add_argument("--gamma", type=int, default=1, help="preconditioner cycle index: 1 = V, 2 = W")
-    ap.add_argument("--mode", default="mixed", choices=["mixed", "reliable", "fp64"],
+    ap.add_argument("--mode", default="reliable", choices=["mixed", "reliable", "fp64"],
                     help="MG-PCG variant: FP64 Krylov vectors (mixed), FP32 vectors with FP64 residual "
                          "replacement (reliable), or all FP64")
     ap.add_argument("--no-graphs", action="store_true",
@@ -96,12 +96,13 @@ def build_problem(ex, ey, ez, seed):
     return hg, bits, R, fixed, f, rho
 
 
-def cpu_baseline(bits, R, fixed, f, rho, pre, post, n_vcycles, n_applies):
+def cpu_baseline(bits, R, fixed, f, rho, pre, post, n_vcycles, n_applies, reps=2):
     """Oracle (FP64 C++/OpenMP restatement of the reference kernels) on the host
-    cores: one fine-level GS sweep and one fine residual at the full size; the
-    SIMP-iteration time is extrapolated from the GPU run's V-cycle and operator
-    counts (fine level only; coarse levels, Galerkin and OC excluded, so this is a
-    lower bound of the CPU path)."""
+    cores: the finest level's share of `reps` V(pre, post) cycles (pre + post
+    Gauss-Seidel sweeps and one residual each) at the full size; the SIMP
+    iteration is extrapolated from the GPU run's V-cycle and operator counts.
+    Coarse levels, Galerkin and OC are left out, so this is a lower bound of
+    the CPU path."""
     import oracle
     o = oracle.Hierarchy(R, bits)
     o.set_material(1e6, 0.4, 3.0)
@@ -110,18 +111,25 @@ def cpu_baseline(bits, R, fixed, f, rho, pre, post, n_vcycles, n_applies):
     nn = o.levels[0]["nn"]
     o.set_vec(0, 1, f)
     o.set_vec(0, 0, np.zeros((3, nn)))
-    t0 = time.perf_counter()
-    o.gs_relax(0, 1)
-    t1 = time.perf_counter()
-    o.update_residual(0)
-    t2 = time.perf_counter()
-    t_sweep, t_res = t1 - t0, t2 - t1
+    t_sweep = t_res = 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        o.gs_relax(0, pre + post)
+        t1 = time.perf_counter()
+        o.update_residual(0)
+        t2 = time.perf_counter()
+        t_sweep += (t1 - t0) / (pre + post)
+        t_res += t2 - t1
+    t_sweep /= reps
+    t_res /= reps
     est = n_vcycles * ((pre + post) * t_sweep + t_res) + n_applies * t_res
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(value=est, unit="s/iter", cores=cores, kind="port",
-                sample=(f"oracle FP64 fine-level GS sweep ({t_sweep:.2f} s) and residual ({t_res:.2f} s) at "
-                        f"full size on {cores} threads, scaled by the GPU run's {n_vcycles:.1f} V-cycles and "
-                        f"{n_applies:.1f} operator applies per SIMP iteration; fine level only (lower bound)"))
+                sample=(f"oracle FP64 finest-level work of {reps} V({pre},{post}) cycles at full size on {cores} "
+                        f"threads ({reps * (pre + post)} GS sweeps, {t_sweep:.2f} s each; {reps} residuals, "
+                        f"{t_res:.2f} s each), scaled by the GPU run's {n_vcycles:.1f} V-cycles and "
+                        f"{n_applies:.1f} operator applies per SIMP iteration; coarse levels, Galerkin and OC "
+                        f"excluded (lower bound)"))
 
 
 def log(msg):
@@ -218,13 +226,25 @@ def main():
     vgoal, step, damp, rmin, rfilt = 0.4, 0.03, 0.5, 1e-3, 2
     mode = {"mixed": rt.RT_SOLVE_MIXED, "reliable": rt.RT_SOLVE_RELIABLE, "fp64": rt.RT_SOLVE_FP64}[a.mode]
 
-    def simp_step():
+    phase = {"update_stencil": 0.0, "solve": 0.0, "sensitivity_filter_oc": 0.0}
+
+    def simp_step(record=False):
+        t0 = time.perf_counter()
         hg.update_stencil()
+        hg.synchronize()
+        t1 = time.perf_counter()
         st = hg.solve(tol=a.tol, maxit=500, mode=mode, pre=a.pre, post=a.post)
         c = hg.compliance()
+        t2 = time.perf_counter()
         hg.compute_sensitivity()
         hg.filter_sensitivity(rfilt)
         hg.update_densities(vgoal, step, damp, rmin)
+        hg.synchronize()
+        t3 = time.perf_counter()
+        if record:
+            phase["update_stencil"] += t1 - t0
+            phase["solve"] += t2 - t1
+            phase["sensitivity_filter_oc"] += t3 - t2
         return st, c
 
     for w in range(a.warmup):
@@ -234,7 +254,7 @@ def main():
     iters, rels, comps = [], [], []
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        st, c = simp_step()
+        st, c = simp_step(record=True)
         log(f"step: {st.iterations} PCG iterations, rel_res {st.rel_res:.2e}, c {c:.6e}")
         iters.append(st.iterations)
         rels.append(st.rel_res)
@@ -283,6 +303,7 @@ def main():
             "active_elements": ne0, "active_nodes": nn0, "dofs": 3 * nn0, "levels": hg.n_levels,
             "pcg_iters_per_step": float(np.mean(iters)), "rel_res": float(np.max(rels)),
             "compliance_last": comps[-1],
+            "seconds_per_step": {k: v / max(1, a.steps) for k, v in phase.items()},
         },
         "vcycle": {"ms": t_v, "algorithmic_bytes": B_V, "GBps": B_V / (t_v * 1e-3) / 1e9,
                    "frac_of_peak": B_V / (t_v * 1e-3) / 1e9 / HBM_PEAK_GBS, "pre": a.pre, "post": a.post,

@@ -61,7 +61,7 @@ def build_library(jobs=None):
                 print(f"[build] {futs[f]}: {err}", file=sys.stderr)
             objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + sorted(objs)
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + sorted(objs) + ["-lrocsolver", "-lrocblas"]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError("link failed:\n" + p.stderr)

@@ -311,6 +311,18 @@ void launch_coarsest(const double* Ainv, int n3, const int* rowslot, int, Vec3<c
     RT_HIP(hipGetLastError());
 }
 
+// A[r][c] = A[c][r] for c < r (row-major n x n): completes a symmetric matrix
+// whose upper triangle holds the values
+__global__ void k_mirror_upper(double* __restrict__ A, int n) {
+    const int r = blockIdx.y;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < r; c += gridDim.x * blockDim.x)
+        A[(size_t)r * n + c] = A[(size_t)c * n + r];
+}
+void launch_mirror_upper(double* A, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_mirror_upper, dim3((n + 255) / 256, n), dim3(256), 0, s, A, n);
+    RT_HIP(hipGetLastError());
+}
+
 // ---------------------------------------------------------------------------
 // vector helpers and deterministic two-stage FP64 reductions
 // ---------------------------------------------------------------------------

@@ -8,10 +8,12 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
-#include <thread>
 
 #include "../../include/robtop_amd.h"
 #include "rt_internal.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 namespace rt {
 
@@ -105,8 +107,10 @@ struct Hier {
     LevelVecs<float> pc32;
     DBuf partials, scal, fbuf, stage;
     int nparts_max = 0;
-    // coarsest
-    DBuf Ainv, rowslot;
+    // coarsest: explicit inverse of the dense system (FP64, n3 x n3), factored
+    // with rocSOLVER on the handle's stream
+    DBuf Ainv, rowslot, dinfo;
+    rocblas_handle blas = nullptr;
     int n3 = 0;
     DBuf fixce;
     int nfixce = 0;
@@ -129,6 +133,7 @@ struct Hier {
     }
     ~Hier() {
         drop_graph();
+        if (blas) (void)rocblas_destroy_handle(blas);
         if (s) (void)hipStreamDestroy(s);
     }
 };
@@ -578,56 +583,34 @@ static void factor_coarsest(Hier& H) {
                             K[(size_t)(3 * r + a) * n + 3 * c + b] = st[((size_t)nb * 9 + 3 * a + b) * N + s];
                 }
             }
-    // Cholesky (lower, in place), rows in parallel
-    const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
-    auto par = [&](int lo, int hi, auto fn) {
-        if (hi - lo < 64 || nth == 1) { for (int i = lo; i < hi; i++) fn(i); return; }
-        std::vector<std::thread> ts;
-        const int chunk = (hi - lo + nth - 1) / nth;
-        for (int t = 0; t < nth; t++) {
-            const int a = lo + t * chunk, b = std::min(hi, a + chunk);
-            if (a >= b) break;
-            ts.emplace_back([=]() { for (int i = a; i < b; i++) fn(i); });
-        }
-        for (auto& th : ts) th.join();
-    };
-    for (int j = 0; j < n; j++) {
-        double d = K[(size_t)j * n + j];
-        for (int k = 0; k < j; k++) d -= K[(size_t)j * n + k] * K[(size_t)j * n + k];
-        if (!(d > 0)) throw std::domain_error("coarsest system is not positive definite (no supports?)");
-        d = std::sqrt(d);
-        K[(size_t)j * n + j] = d;
-        par(j + 1, n, [&](int i) {
-            double s = K[(size_t)i * n + j];
-            const double* ri = &K[(size_t)i * n];
-            const double* rj = &K[(size_t)j * n];
-            for (int k = 0; k < j; k++) s -= ri[k] * rj[k];
-            K[(size_t)i * n + j] = s / d;
-        });
-    }
-    // inverse: columns of (L L^T)^-1
-    std::vector<double> Ainv((size_t)n * n);
-    par(0, n, [&](int col) {
-        std::vector<double> b(n, 0.0);
-        b[col] = 1.0;
-        for (int i = col; i < n; i++) {
-            double s = b[i];
-            const double* ri = &K[(size_t)i * n];
-            for (int k = col; k < i; k++) s -= ri[k] * b[k];
-            b[i] = s / ri[i];
-        }
-        for (int i = n - 1; i >= 0; i--) {
-            double s = b[i];
-            for (int k = i + 1; k < n; k++) s -= K[(size_t)k * n + i] * b[k];
-            b[i] = s / K[(size_t)i * n + i];
-        }
-        for (int i = 0; i < n; i++) Ainv[(size_t)i * n + col] = b[i];
-    });
-    H.n3 = n;
+    // Cholesky and inverse on the device (rocSOLVER, FP64), then mirror the
+    // triangle the inverse was written to so the GEMV can read whole rows
     H.Ainv.ensure(sizeof(double) * (size_t)n * n);
+    H.dinfo.ensure(sizeof(int));
+    RT_HIP(hipMemcpyAsync(H.Ainv.p, K.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice, H.s));
+    if (!H.blas) {
+        if (rocblas_create_handle(&H.blas) != rocblas_status_success) throw std::runtime_error("rocblas_create_handle failed");
+    }
+    if (rocblas_set_stream(H.blas, H.s) != rocblas_status_success) throw std::runtime_error("rocblas_set_stream failed");
+    auto info_ok = [&](const char* what) {
+        int info = 0;
+        RT_HIP(hipMemcpyAsync(&info, H.dinfo.p, sizeof(int), hipMemcpyDeviceToHost, H.s));
+        RT_HIP(hipStreamSynchronize(H.s));
+        if (info != 0)
+            throw std::domain_error(std::string("coarsest system: ") + what + " failed (not positive definite; no supports?)");
+    };
+    if (rocsolver_dpotrf(H.blas, rocblas_fill_lower, n, H.Ainv.as<double>(), n, H.dinfo.as<int>()) !=
+        rocblas_status_success)
+        throw std::runtime_error("rocsolver_dpotrf failed");
+    info_ok("Cholesky factorisation");
+    if (rocsolver_dpotri(H.blas, rocblas_fill_lower, n, H.Ainv.as<double>(), n, H.dinfo.as<int>()) !=
+        rocblas_status_success)
+        throw std::runtime_error("rocsolver_dpotri failed");
+    info_ok("inverse");
+    launch_mirror_upper(H.Ainv.as<double>(), n, H.s);
+    H.n3 = n;
     H.rowslot.ensure(sizeof(int) * (size_t)nr);
     H.fbuf.ensure(sizeof(double) * (size_t)n);
-    RT_HIP(hipMemcpyAsync(H.Ainv.p, Ainv.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice, H.s));
     RT_HIP(hipMemcpyAsync(H.rowslot.p, slots.data(), sizeof(int) * (size_t)nr, hipMemcpyHostToDevice, H.s));
     RT_HIP(hipStreamSynchronize(H.s));
 }

@@ -126,6 +126,7 @@ void launch_prolong(const Geom& gf, const int of[3], Vec3<T> Uf, const uint8_t* 
 template <typename T>
 void launch_coarsest(const double* Ainv, int n3, const int* rowslot, int unused, Vec3<const T> F, Vec3<T> U,
                      double* fbuf, hipStream_t s);
+void launch_mirror_upper(double* A, int n, hipStream_t s);
 template <typename TA, typename TB>
 void launch_convert(const TA* a, TB* b, size_t n, hipStream_t s);
 template <typename T>
