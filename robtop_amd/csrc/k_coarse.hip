@@ -129,9 +129,103 @@ __global__ void __launch_bounds__(256) k_coarse_gs(Geom g, Vec3<T> U, Vec3<const
     coarse_gs_body<T, TS, C>(g, U, F, st, vflag, reverse != 0);
 }
 
+// Sliced form of the same smoother, for levels whose colour passes are too
+// short to hide memory latency with one thread per node: a workgroup of nine
+// waves owns 64 nodes, wave w reads the three neighbours of neighbour row
+// (dy, dz) = (w % 3 - 1, w / 3 - 1) (12 loads per lane, one round trip) and
+// the partial products meet in LDS; the wave holding the centre block does the
+// component-wise update. Lanes of a wave take consecutive nodes, so every load
+// is a unit-stride run, and the row's offsets are wave-uniform scalars.
+template <typename T, typename TS, int C>
+__global__ void __launch_bounds__(576) k_coarse_gs9(Geom g, Vec3<T> U, Vec3<const T> F, const TS* __restrict__ st,
+                                                    const uint8_t* __restrict__ vflag, int reverse) {
+    __shared__ T part[9][3][64];
+    constexpr int cx = C & 1, cy = (C >> 1) & 1, cz = C >> 2;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int dy = w % 3 - 1, dz = w / 3 - 1;
+    int base = 0;
+    bool act = c_colour_thread<C>(g, blockIdx.x * 64 + lane, base);
+    const int v = C * g.cs + base;
+    if (act) act = (vflag[v] & NF_ACTIVE) != 0;
+    const int N = g.N;
+    const int sh = g.h[0] * g.h[1] + g.h[0] + 1;
+    const int vbu = act ? (base - sh) * (int)sizeof(T) : 0, vbs = act ? v * (int)sizeof(TS) : 0;
+    T Au[3] = {T(0), T(0), T(0)};
+    T S[9], uc[3] = {T(0), T(0), T(0)};
+    if (act) {
+        T u[3][3];
+        TS a[3][9];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int dx = k - 1;
+            const int nb = 9 * (dz + 1) + 3 * (dy + 1) + k;
+            const int col = C ^ ((dx & 1) | ((dy & 1) << 1) | ((dz & 1) << 2));
+            const int so = (col * g.cs + (((cz + dz) >> 1) * g.h[1] + ((cy + dy) >> 1)) * g.h[0] + ((cx + dx) >> 1) +
+                            sh) * (int)sizeof(T);
+            const __amdgpu_buffer_rsrc_t rs = rsrc(st + (size_t)nb * 9 * N, 9 * N * (int)sizeof(TS));
+#pragma unroll
+            for (int c = 0; c < 3; c++) u[k][c] = bload<T>(rsrc(U.c[c], N * (int)sizeof(T)), vbu, so);
+#pragma unroll
+            for (int q = 0; q < 9; q++) a[k][q] = bload<TS>(rs, vbs, q * N * (int)sizeof(TS));
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            if (w == 4 && k == 1) {          // the node itself: centre block and own value
+#pragma unroll
+                for (int q = 0; q < 9; q++) S[q] = (T)a[k][q];
+#pragma unroll
+                for (int c = 0; c < 3; c++) uc[c] = u[k][c];
+                continue;
+            }
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                Au[r] += (T)a[k][3 * r + 0] * u[k][0] + (T)a[k][3 * r + 1] * u[k][1] + (T)a[k][3 * r + 2] * u[k][2];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++) part[w][r][lane] = Au[r];
+    __syncthreads();
+    if (w != 4 || !act) return;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        T acc = T(0);
+#pragma unroll
+        for (int q = 0; q < 9; q++) acc += part[q][r][lane];
+        Au[r] = acc;
+    }
+    T u[3] = {uc[0], uc[1], uc[2]};
+    const T f[3] = {F.c[0][v], F.c[1][v], F.c[2][v]};
+    if (!reverse) {
+        u[0] = (f[0] - Au[0] - (S[1] * u[1] + S[2] * u[2])) / S[0];
+        u[1] = (f[1] - Au[1] - (S[3] * u[0] + S[5] * u[2])) / S[4];
+        u[2] = (f[2] - Au[2] - (S[6] * u[0] + S[7] * u[1])) / S[8];
+    } else {
+        u[2] = (f[2] - Au[2] - (S[6] * u[0] + S[7] * u[1])) / S[8];
+        u[1] = (f[1] - Au[1] - (S[3] * u[0] + S[5] * u[2])) / S[4];
+        u[0] = (f[0] - Au[0] - (S[1] * u[1] + S[2] * u[2])) / S[0];
+    }
+    U.c[0][v] = u[0]; U.c[1][v] = u[1]; U.c[2][v] = u[2];
+}
+
+#ifndef RT_COARSE_SLICED
+#define RT_COARSE_SLICED 1
+#endif
+
 template <typename T, typename TS>
 void launch_coarse_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const TS* st, const uint8_t* vflag, int colour,
                       bool reverse, hipStream_t s) {
+    if (RT_COARSE_SLICED) {
+        unsigned nb = colour_blocks(g, colour, 64);
+        if (!nb) return;
+        switch (colour) {
+#define RT_CASE(c) case c: hipLaunchKernelGGL((k_coarse_gs9<T, TS, c>), dim3(nb), dim3(576), 0, s, g, U, F, st, vflag, (int)reverse); break;
+            RT_CASE(0) RT_CASE(1) RT_CASE(2) RT_CASE(3) RT_CASE(4) RT_CASE(5) RT_CASE(6) RT_CASE(7)
+#undef RT_CASE
+        }
+        RT_HIP(hipGetLastError());
+        return;
+    }
     unsigned nb = colour_blocks(g, colour, 256);
     if (!nb) return;
     switch (colour) {
@@ -165,11 +259,75 @@ __global__ void __launch_bounds__(256) k_coarse_res(Geom g, Vec3<const T> U, Vec
     R.c[2][v] = F.c[2][v] - Au[2];
 }
 
+// sliced residual (same partition as k_coarse_gs9): colour = blockIdx.y
+template <typename T, typename TS>
+__global__ void __launch_bounds__(576) k_coarse_res9(Geom g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R,
+                                                     const TS* __restrict__ st, const uint8_t* __restrict__ vflag) {
+    __shared__ T part[9][3][64];
+    const int C = blockIdx.y;
+    const int cx = C & 1, cy = (C >> 1) & 1, cz = C >> 2;
+    const int nhx = (g.n[0] - cx + 1) >> 1, nhy = (g.n[1] - cy + 1) >> 1, nhz = (g.n[2] - cz + 1) >> 1;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int dy = w % 3 - 1, dz = w / 3 - 1;
+    const int t = blockIdx.x * 64 + lane;
+    bool act = t < nhx * nhy * nhz;
+    int base = 0, v = 0;
+    if (act) {
+        const int rr = t / nhx;
+        base = ((rr / nhy + 1) * g.h[1] + (rr % nhy + 1)) * g.h[0] + (t % nhx + 1);
+        v = C * g.cs + base;
+        act = (vflag[v] & NF_ACTIVE) != 0;
+    }
+    const int N = g.N;
+    const int sh = g.h[0] * g.h[1] + g.h[0] + 1;
+    T Au[3] = {T(0), T(0), T(0)};
+    if (act) {
+        const int vbu = (base - sh) * (int)sizeof(T), vbs = v * (int)sizeof(TS);
+        T u[3][3];
+        TS a[3][9];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int dx = k - 1;
+            const int nb = 9 * (dz + 1) + 3 * (dy + 1) + k;
+            const int col = C ^ ((dx & 1) | ((dy & 1) << 1) | ((dz & 1) << 2));
+            const int so = (col * g.cs + (((cz + dz) >> 1) * g.h[1] + ((cy + dy) >> 1)) * g.h[0] + ((cx + dx) >> 1) +
+                            sh) * (int)sizeof(T);
+            const __amdgpu_buffer_rsrc_t rs = rsrc(st + (size_t)nb * 9 * N, 9 * N * (int)sizeof(TS));
+#pragma unroll
+            for (int c = 0; c < 3; c++) u[k][c] = bload<T>(rsrc(U.c[c], N * (int)sizeof(T)), vbu, so);
+#pragma unroll
+            for (int q = 0; q < 9; q++) a[k][q] = bload<TS>(rs, vbs, q * N * (int)sizeof(TS));
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                Au[r] += (T)a[k][3 * r + 0] * u[k][0] + (T)a[k][3 * r + 1] * u[k][1] + (T)a[k][3 * r + 2] * u[k][2];
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++) part[w][r][lane] = Au[r];
+    __syncthreads();
+    if (w != 0 || !act) return;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        T acc = T(0);
+#pragma unroll
+        for (int q = 0; q < 9; q++) acc += part[q][r][lane];
+        R.c[r][v] = F.c[r][v] - acc;
+    }
+}
+
 template <typename T, typename TS>
 void launch_coarse_residual(const Geom& g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R, const TS* st,
                             const uint8_t* vflag, hipStream_t s) {
-    dim3 grid(colour_blocks(g, 0, 256), 8);
-    hipLaunchKernelGGL((k_coarse_res<T, TS>), grid, dim3(256), 0, s, g, U, F, R, st, vflag);
+    if (RT_COARSE_SLICED) {
+        dim3 grid(colour_blocks(g, 0, 64), 8);
+        hipLaunchKernelGGL((k_coarse_res9<T, TS>), grid, dim3(576), 0, s, g, U, F, R, st, vflag);
+    } else {
+        dim3 grid(colour_blocks(g, 0, 256), 8);
+        hipLaunchKernelGGL((k_coarse_res<T, TS>), grid, dim3(256), 0, s, g, U, F, R, st, vflag);
+    }
     RT_HIP(hipGetLastError());
 }
 
@@ -216,12 +374,87 @@ __global__ void __launch_bounds__(256) k_restrict(Geom gf, int ofx, int ofy, int
     Fc.c[0][vc] = s0; Fc.c[1][vc] = s1; Fc.c[2][vc] = s2;
 }
 
+// x4 restriction in two separable stages (the weights factor per axis):
+//   stage 1: W(xc, yc, z) = sum_{dx,dy} (4 - |dx|)(4 - |dy|) r_f(fx + dx, fy + dy, z)
+//            for every fine plane z, into a plain [z][yc][xc] scratch array;
+//   stage 2: F_c = sum_dz (4 - |dz|) W(xc, yc, fz + dz) / 64.
+// Stage 1 reads each fine value from HBM once (its 7 x 7 window is shared by
+// neighbouring lanes through L1/L2); stage 2 reads the 1/16-size scratch.
+template <typename T, int RATIO>
+__global__ void __launch_bounds__(256) k_restrict_xy(Geom gf, int ofx, int ofy, Vec3<const T> Rf, int ncx, int ncy,
+                                                     int ocx, int ocy, T* __restrict__ W) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nz = gf.n[2];
+    const int tot = ncx * ncy * nz;
+    if (t >= tot) return;
+    const int xc = t % ncx, r = t / ncx, yc = r % ncy, z = r / ncy;
+    const int fx = (ocx + xc) * RATIO - ofx, fy = (ocy + yc) * RATIO - ofy;
+    constexpr int Wd = RATIO - 1;
+    T s0 = T(0), s1 = T(0), s2 = T(0);
+    for (int dy = -Wd; dy <= Wd; dy++) {
+        const int py = fy + dy;
+        if (py < 0 || py >= gf.n[1]) continue;
+        const int wy = RATIO - (dy < 0 ? -dy : dy);
+#pragma unroll
+        for (int dx = -Wd; dx <= Wd; dx++) {
+            const int px = fx + dx;
+            if (px < 0 || px >= gf.n[0]) continue;
+            const T w = (T)(wy * (RATIO - (dx < 0 ? -dx : dx)));
+            const int sl = oct_p(gf, px, py, z);
+            s0 += w * Rf.c[0][sl];
+            s1 += w * Rf.c[1][sl];
+            s2 += w * Rf.c[2][sl];
+        }
+    }
+    W[t] = s0;
+    W[(size_t)tot + t] = s1;
+    W[2 * (size_t)tot + t] = s2;
+}
+
+template <typename T, int RATIO>
+__global__ void __launch_bounds__(256) k_restrict_z(int nfz, int ofz, const T* __restrict__ W, Geom gc, int ocz,
+                                                    Vec3<T> Fc, const uint8_t* __restrict__ vflag_c) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncx = gc.n[0], ncy = gc.n[1];
+    if (t >= ncx * ncy * gc.n[2]) return;
+    const int xy = t % (ncx * ncy), zc = t / (ncx * ncy);
+    const int vc = oct_p(gc, xy % ncx, xy / ncx, zc);
+    if (!(vflag_c[vc] & NF_ACTIVE)) return;
+    const int fz = (ocz + zc) * RATIO - ofz;
+    const size_t tot = (size_t)ncx * ncy * nfz;
+    constexpr int Wd = RATIO - 1;
+    constexpr T inv = T(1) / T(RATIO * RATIO * RATIO);
+    T s0 = T(0), s1 = T(0), s2 = T(0);
+#pragma unroll
+    for (int dz = -Wd; dz <= Wd; dz++) {
+        const int pz = fz + dz;
+        if (pz < 0 || pz >= nfz) continue;
+        const T w = (T)(RATIO - (dz < 0 ? -dz : dz));
+        const size_t i = (size_t)pz * ncx * ncy + xy;
+        s0 += w * W[i];
+        s1 += w * W[tot + i];
+        s2 += w * W[2 * tot + i];
+    }
+    Fc.c[0][vc] = s0 * inv; Fc.c[1][vc] = s1 * inv; Fc.c[2][vc] = s2 * inv;
+}
+
+size_t restrict_scratch_elems(const Geom& gf, const Geom& gc) {
+    return 3 * (size_t)gc.n[0] * gc.n[1] * gf.n[2];
+}
+
 template <typename T>
 void launch_restrict(const Geom& gf, const int of[3], Vec3<const T> Rf, const Geom& gc, const int oc[3], Vec3<T> Fc,
-                     const uint8_t* vflag_c, int ratio, hipStream_t s) {
+                     const uint8_t* vflag_c, int ratio, T* scratch, hipStream_t s) {
     long long tot = (long long)gc.n[0] * gc.n[1] * gc.n[2];
     dim3 grid((unsigned)((tot + 255) / 256));
-    if (ratio == 4)
+    if (ratio == 4 && scratch) {
+        const long long t1 = (long long)gc.n[0] * gc.n[1] * gf.n[2];
+        hipLaunchKernelGGL((k_restrict_xy<T, 4>), dim3((unsigned)((t1 + 255) / 256)), dim3(256), 0, s, gf, of[0],
+                           of[1], Rf, gc.n[0], gc.n[1], oc[0], oc[1], scratch);
+        RT_HIP(hipGetLastError());
+        hipLaunchKernelGGL((k_restrict_z<T, 4>), grid, dim3(256), 0, s, gf.n[2], of[2], (const T*)scratch, gc, oc[2],
+                           Fc, vflag_c);
+    } else if (ratio == 4)
         hipLaunchKernelGGL((k_restrict<T, 4>), grid, dim3(256), 0, s, gf, of[0], of[1], of[2], Rf, gc, oc[0], oc[1],
                            oc[2], Fc, vflag_c);
     else
@@ -263,17 +496,67 @@ __global__ void __launch_bounds__(256) k_prolong(Geom gf, int ofx, int ofy, int 
     Uf.c[0][v] += c0; Uf.c[1][v] += c1; Uf.c[2][v] += c2;
 }
 
+// x4 prolongation, one thread per coarse x-interval of a fine row: the thread
+// interpolates the coarse values in y and z at the interval's two ends (8
+// coarse nodes) and then in x for the RATIO fine nodes of the interval, so a
+// fine node costs 6 coarse loads instead of 24. Interval gxl covers absolute
+// fine x in [RATIO (gx0 + gxl), RATIO (gx0 + gxl + 1)).
+template <typename T, int RATIO>
+__global__ void __launch_bounds__(256) k_prolong_iv(Geom gf, int ofx, int ofy, int ofz, Vec3<T> Uf,
+                                                    const uint8_t* __restrict__ vflag_f, Geom gc, int ocx, int ocy,
+                                                    int ocz, Vec3<const T> Uc, int skip_fixed, int gx0, int ngx) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nx = gf.n[0], ny = gf.n[1], nz = gf.n[2];
+    if (t >= ngx * ny * nz) return;
+    const int gxl = t % ngx, r = t / ngx, y = r % ny, z = r / ny;
+    const int bxa = gx0 + gxl;
+    const int xs = bxa * RATIO - ofx;                 // fine x of the interval start
+    const int ilo = xs < 0 ? -xs : 0, ihi = nx - xs < RATIO ? nx - xs : RATIO;
+    const int ay = ofy + y, az = ofz + z;
+    const int py = ay % RATIO, pz = az % RATIO;
+    const int bx = bxa - ocx, by = ay / RATIO - ocy, bz = az / RATIO - ocz;
+    const bool right = ihi > 1;                       // some node of the interval has a weight on bx + 1
+    T a0 = T(0), a1 = T(0), a2 = T(0), b0 = T(0), b1 = T(0), b2 = T(0);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int wy = (j & 1) ? py : RATIO - py;
+        const int wz = (j & 2) ? pz : RATIO - pz;
+        if (wy * wz == 0) continue;
+        const T w = (T)(wy * wz);
+        const int sa = oct_p(gc, bx, by + (j & 1), bz + (j >> 1));
+        a0 += w * Uc.c[0][sa]; a1 += w * Uc.c[1][sa]; a2 += w * Uc.c[2][sa];
+        if (right) {
+            const int sb = oct_p(gc, bx + 1, by + (j & 1), bz + (j >> 1));
+            b0 += w * Uc.c[0][sb]; b1 += w * Uc.c[1][sb]; b2 += w * Uc.c[2][sb];
+        }
+    }
+    constexpr T inv = T(1) / T(RATIO * RATIO * RATIO);
+#pragma unroll
+    for (int i = 0; i < RATIO; i++) {
+        if (i < ilo || i >= ihi) continue;
+        const int v = oct_p(gf, xs + i, y, z);
+        const uint8_t fl = vflag_f[v];
+        if (!(fl & NF_ACTIVE) || (skip_fixed && (fl & NF_FIXED))) continue;
+        const T wl = (T)(RATIO - i), wr = (T)i;
+        Uf.c[0][v] += (wl * a0 + wr * b0) * inv;
+        Uf.c[1][v] += (wl * a1 + wr * b1) * inv;
+        Uf.c[2][v] += (wl * a2 + wr * b2) * inv;
+    }
+}
+
 template <typename T>
 void launch_prolong(const Geom& gf, const int of[3], Vec3<T> Uf, const uint8_t* vflag_f, const Geom& gc,
                     const int oc[3], Vec3<const T> Uc, int ratio, bool skip_fixed, hipStream_t s) {
-    long long tot = (long long)gf.n[0] * gf.n[1] * gf.n[2];
-    dim3 grid((unsigned)((tot + 255) / 256));
-    if (ratio == 4)
-        hipLaunchKernelGGL((k_prolong<T, 4>), grid, dim3(256), 0, s, gf, of[0], of[1], of[2], Uf, vflag_f, gc, oc[0],
-                           oc[1], oc[2], Uc, (int)skip_fixed);
-    else
-        hipLaunchKernelGGL((k_prolong<T, 2>), grid, dim3(256), 0, s, gf, of[0], of[1], of[2], Uf, vflag_f, gc, oc[0],
-                           oc[1], oc[2], Uc, (int)skip_fixed);
+    if (ratio == 4) {
+        const int gx0 = of[0] / 4, ngx = (of[0] + gf.n[0] - 1) / 4 - gx0 + 1;
+        long long tot = (long long)ngx * gf.n[1] * gf.n[2];
+        hipLaunchKernelGGL((k_prolong_iv<T, 4>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, gf, of[0],
+                           of[1], of[2], Uf, vflag_f, gc, oc[0], oc[1], oc[2], Uc, (int)skip_fixed, gx0, ngx);
+    } else {
+        long long tot = (long long)gf.n[0] * gf.n[1] * gf.n[2];
+        hipLaunchKernelGGL((k_prolong<T, 2>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, gf, of[0], of[1],
+                           of[2], Uf, vflag_f, gc, oc[0], oc[1], oc[2], Uc, (int)skip_fixed);
+    }
     RT_HIP(hipGetLastError());
 }
 
@@ -421,13 +704,10 @@ void launch_pcg_update_x_r(T* x, T* r, const T* p, const T* q, double alpha, siz
 }
 
 // reliable-update PCG step: x (FP64) += a p, r (FP32) -= a q; partials of r.r
-__global__ void __launch_bounds__(256) k_pcg_ru_xr(double* __restrict__ x, float* __restrict__ r,
-                                                   const float* __restrict__ p, const float* __restrict__ q,
-                                                   double alpha, size_t n, double* __restrict__ partials) {
-    const float a = (float)alpha;
+__global__ void __launch_bounds__(256) k_pcg_r_update(float* __restrict__ r, const float* __restrict__ q, float a,
+                                                      size_t n, double* __restrict__ partials) {
     double s = 0;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        x[i] += alpha * (double)p[i];
         const float ri = r[i] - a * q[i];
         r[i] = ri;
         s += (double)ri * (double)ri;
@@ -435,9 +715,23 @@ __global__ void __launch_bounds__(256) k_pcg_ru_xr(double* __restrict__ x, float
     s = blk_sum(s);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
-void launch_pcg_ru_update_x_r(double* x, float* r, const float* p, const float* q, double alpha, size_t n,
-                              double* partials, int nparts, hipStream_t s) {
-    hipLaunchKernelGGL(k_pcg_ru_xr, dim3(nparts), dim3(256), 0, s, x, r, p, q, alpha, n, partials);
+void launch_pcg_r_update(float* r, const float* q, double alpha, size_t n, double* partials, int nparts,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_r_update, dim3(nparts), dim3(256), 0, s, r, q, (float)alpha, n, partials);
+    RT_HIP(hipGetLastError());
+}
+
+// x += dx + alpha p (FP64), dx = 0
+__global__ void k_pcg_flush_x(double* __restrict__ x, float* __restrict__ dx, const float* __restrict__ p,
+                              double alpha, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        x[i] += (double)dx[i] + alpha * (double)p[i];
+        dx[i] = 0.f;
+    }
+}
+void launch_pcg_flush_x(double* x, float* dx, const float* p, double alpha, size_t n, hipStream_t s) {
+    unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_pcg_flush_x, dim3(nb), dim3(256), 0, s, x, dx, p, alpha, n);
     RT_HIP(hipGetLastError());
 }
 
@@ -554,7 +848,7 @@ RT_INST_CS(float, float)
 
 #define RT_INST_T(T)                                                                                                  \
     template void launch_restrict<T>(const Geom&, const int*, Vec3<const T>, const Geom&, const int*, Vec3<T>,        \
-                                     const uint8_t*, int, hipStream_t);                                               \
+                                     const uint8_t*, int, T*, hipStream_t);                                           \
     template void launch_prolong<T>(const Geom&, const int*, Vec3<T>, const uint8_t*, const Geom&, const int*,        \
                                     Vec3<const T>, int, bool, hipStream_t);                                           \
     template void launch_coarsest<T>(const double*, int, const int*, int, Vec3<const T>, Vec3<T>, double*,            \

@@ -81,12 +81,25 @@ __device__ __forceinline__ double blk_sum_m(double v) {
     return (threadIdx.x == 0) ? (red[0] + red[1]) + (red[2] + red[3]) : 0.0;
 }
 
-// APPLY = false: R = F - K U;  APPLY = true: R = K U and per-block partials of U.R
-template <typename T, bool APPLY>
+// MODE 0: R = F - K U.  MODE 1: R = K U and per-block partials of U.R.
+// MODE 2 (one PCG step of the reliable-update solver, FP32): the direction
+// p' = Z + beta U is formed while the planes are staged and R = K p', with
+// partials of p'.R; the owner of each node also writes p' to PN (a buffer
+// other than U: neighbouring workgroups still read U) and adds alpha U to the
+// deferred solution increment DX.
+template <typename T> struct PcgStep {
+    const T* z[3];
+    T* pn[3];
+    T* dx[3];
+    T beta, alpha;
+};
+
+template <typename T, int MODE>
 __global__ void __launch_bounds__(256) k_fine_res_modal(Geom g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R,
                                                         const float* __restrict__ rhop,
                                                         const uint8_t* __restrict__ vflag, ModalK<T> K,
-                                                        double* __restrict__ partials) {
+                                                        double* __restrict__ partials, PcgStep<T> ps) {
+    constexpr bool APPLY = MODE != 0;
     __shared__ T su[2][3][PLANE];
     __shared__ uint8_t sf[2][PLANE];
     __shared__ T sy[24][256];
@@ -122,7 +135,14 @@ __global__ void __launch_bounds__(256) k_fine_res_modal(Geom g, Vec3<const T> U,
                 const int s = oct_p(g, px, py, z);
                 const uint8_t fl = vflag[s];
                 pfl[q] = fl;
-                if ((fl & NF_ACTIVE) && !(fl & NF_FIXED)) { pu[q][0] = U.c[0][s]; pu[q][1] = U.c[1][s]; pu[q][2] = U.c[2][s]; }
+                if ((fl & NF_ACTIVE) && !(fl & NF_FIXED)) {
+                    if (MODE == 2) {
+#pragma unroll
+                        for (int c = 0; c < 3; c++) pu[q][c] = ps.z[c][s] + ps.beta * U.c[c][s];
+                    } else {
+                        pu[q][0] = U.c[0][s]; pu[q][1] = U.c[1][s]; pu[q][2] = U.c[2][s];
+                    }
+                }
             }
         }
     };
@@ -198,6 +218,13 @@ __global__ void __launch_bounds__(256) k_fine_res_modal(Geom g, Vec3<const T> U,
                         R.c[0][s] = KU[0]; R.c[1][s] = KU[1]; R.c[2][s] = KU[2];
                         dotacc += (double)KU[0] * (double)su[lo][0][p] + (double)KU[1] * (double)su[lo][1][p] +
                                   (double)KU[2] * (double)su[lo][2][p];
+                        if (MODE == 2) {
+#pragma unroll
+                            for (int c = 0; c < 3; c++) {
+                                ps.dx[c][s] += ps.alpha * U.c[c][s];
+                                ps.pn[c][s] = su[lo][c][p];
+                            }
+                        }
                     } else {
                         if (fl & NF_FIXED) {
                             R.c[0][s] = F.c[0][s]; R.c[1][s] = F.c[1][s]; R.c[2][s] = F.c[2][s];
@@ -266,16 +293,29 @@ int fine_apply_partials_modal(const Geom& g) {
 template <typename T>
 void launch_fine_residual_modal(const Geom& g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R, const float* rhop,
                                 const uint8_t* vflag, hipStream_t s) {
-    hipLaunchKernelGGL((k_fine_res_modal<T, false>), modal_grid(g), dim3(256), 0, s, g, U, F, R, rhop, vflag,
-                       modal_values<T>(g_modal), (double*)nullptr);
+    hipLaunchKernelGGL((k_fine_res_modal<T, 0>), modal_grid(g), dim3(256), 0, s, g, U, F, R, rhop, vflag,
+                       modal_values<T>(g_modal), (double*)nullptr, PcgStep<T>{});
     RT_HIP(hipGetLastError());
 }
 
 template <typename T>
 void launch_fine_apply_modal(const Geom& g, Vec3<const T> X, Vec3<T> Y, const float* rhop, const uint8_t* vflag,
                              double* partials, int* nparts, hipStream_t s) {
-    hipLaunchKernelGGL((k_fine_res_modal<T, true>), modal_grid(g), dim3(256), 0, s, g, X, X, Y, rhop, vflag,
-                       modal_values<T>(g_modal), partials);
+    hipLaunchKernelGGL((k_fine_res_modal<T, 1>), modal_grid(g), dim3(256), 0, s, g, X, X, Y, rhop, vflag,
+                       modal_values<T>(g_modal), partials, PcgStep<T>{});
+    RT_HIP(hipGetLastError());
+    if (nparts) *nparts = fine_apply_partials_modal(g);
+}
+
+void launch_fine_pcg_step_modal(const Geom& g, Vec3<const float> P, Vec3<const float> Z, float beta, Vec3<float> PN,
+                                Vec3<float> DX, float alpha, Vec3<float> Q, const float* rhop, const uint8_t* vflag,
+                                double* partials, int* nparts, hipStream_t s) {
+    PcgStep<float> ps;
+    for (int c = 0; c < 3; c++) { ps.z[c] = Z.c[c]; ps.pn[c] = PN.c[c]; ps.dx[c] = DX.c[c]; }
+    ps.beta = beta;
+    ps.alpha = alpha;
+    hipLaunchKernelGGL((k_fine_res_modal<float, 2>), modal_grid(g), dim3(256), 0, s, g, P, P, Q, rhop, vflag,
+                       modal_values<float>(g_modal), partials, ps);
     RT_HIP(hipGetLastError());
     if (nparts) *nparts = fine_apply_partials_modal(g);
 }

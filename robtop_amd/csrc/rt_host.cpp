@@ -96,6 +96,7 @@ struct Hier {
     double E = 1e6, nu = 0.4;
     float p = 3.f;
     DBuf dM;                 // galerkin x4 M_k [64][576]
+    DBuf xfer;               // scratch of the two-stage x4 restriction (sized for FP64)
     // finest level
     DBuf rho, rhop, eflag, sens, sens_tmp;
     std::vector<uint8_t> eflag_box, fixed_box;
@@ -103,6 +104,7 @@ struct Hier {
     // PCG work
     V3<double> r64, p64, q64;
     V3<float> x32, b32, r32, p32, q32;
+    V3<float> p32b, dx32;    // reliable-update PCG: second direction buffer, FP32 solution increment
     LevelVecs<double> pc64;  // preconditioner level-0 vectors
     LevelVecs<float> pc32;
     DBuf partials, scal, fbuf, stage;
@@ -381,6 +383,8 @@ static void setup_device(Hier& H) {
         L.KEc.alloc(sizeof(double) * 576 * L.ne_box());
         L.st64.alloc(sizeof(double) * 243 * (size_t)L.g.N);
         L.st32.alloc(sizeof(float) * 243 * (size_t)L.g.N);
+        if (L.scale / H.lv[li - 1].scale == 4)
+            H.xfer.ensure(sizeof(double) * restrict_scratch_elems(H.lv[li - 1].g, L.g));
     }
     ensure_partials(H);
 }
@@ -430,7 +434,10 @@ template <typename T>
 static void restrict_to(Hier& H, int li, Vec3<const T> Rf, Vec3<T> Fc) {
     Level& C = H.lv[li];
     Level& Fl = H.lv[li - 1];
-    launch_restrict<T>(Fl.g, Fl.o, Rf, C.g, C.o, Fc, C.vflag.as<uint8_t>(), C.scale / Fl.scale, H.s);
+    const int ratio = C.scale / Fl.scale;
+    T* scratch = nullptr;
+    if (ratio == 4 && H.xfer.bytes >= sizeof(T) * restrict_scratch_elems(Fl.g, C.g)) scratch = H.xfer.as<T>();
+    launch_restrict<T>(Fl.g, Fl.o, Rf, C.g, C.o, Fc, C.vflag.as<uint8_t>(), ratio, scratch, H.s);
 }
 
 template <typename T>
@@ -806,7 +813,9 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
     if (!H.r64.ok()) H.r64.alloc(N);
     if (!H.q64.ok()) H.q64.alloc(N);
     if (!H.p32.ok()) H.p32.alloc(N);
+    if (!H.p32b.ok()) H.p32b.alloc(N);
     if (!H.q32.ok()) H.q32.alloc(N);
+    if (!H.dx32.ok()) H.dx32.alloc(N);
     LevelVecs<float>& pc = pcvecs<float>(H);   // pc.F holds the FP32 residual, pc.U the preconditioned one
     const int np = 1024;
     const double xAx = apply_dot<double>(H, H.x64, H.q64);
@@ -823,30 +832,52 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
     double rel = nb > 0 ? std::sqrt(rr) / nb : 0.0, rel_true = rel, rmax = std::sqrt(rr);
     int it = 0;
     if (nb > 0 && rel > tol) {
+        // The solution increment since the last residual replacement is kept
+        // in FP32 (dx) and flushed into x64 at each replacement; alpha_k p_k is
+        // added to dx by the next step's fused apply (which reads p_k to form
+        // p_k+1 = z + beta p_k), or by the flush. p ping-pongs between two
+        // buffers because the fused apply must not overwrite the p it reads.
+        launch_fill<float>(H.dx32.data(), 0.f, n, H.s);
         precondition<float>(H, pc, pre, post);
         double rz = dot<float, float>(H, pc.F.data(), pc.U.data(), n);
-        launch_convert<float, float>(pc.U.data(), H.p32.data(), n, H.s);
+        V3<float>* P = &H.p32;
+        V3<float>* PN = &H.p32b;
+        launch_convert<float, float>(pc.U.data(), P->data(), n, H.s);
+        double pq = apply_dot<float>(H, *P, H.q32);
+        double pending = 0.0;                       // alpha of the p not yet added to dx
+        bool flushed = false;
         for (it = 1; it <= maxit; it++) {
-            const double pq = apply_dot<float>(H, H.p32, H.q32);
             if (!(pq > 0)) throw std::domain_error("PCG breakdown: p.Ap <= 0");
             const double alpha = rz / pq;
-            launch_pcg_ru_update_x_r(H.x64.data(), pc.F.data(), H.p32.data(), H.q32.data(), alpha, n,
-                                     H.partials.as<double>(), np, H.s);
+            launch_pcg_r_update(pc.F.data(), H.q32.data(), alpha, n, H.partials.as<double>(), np, H.s);
+            pending = alpha;
+            flushed = false;
             const double rn = std::sqrt(sum_partials_host(H, np));
             rel = rn / nb;
             if (rel <= tol || rn < 0.1 * rmax) {
+                launch_pcg_flush_x(H.x64.data(), H.dx32.data(), P->data(), pending, n, H.s);
+                pending = 0.0;
+                flushed = true;
                 rel_true = true_rel_residual(H);      // r64 = f - K x64
                 launch_convert<double, float>(H.r64.data(), pc.F.data(), n, H.s);
                 rmax = rel_true * nb;
                 rel = rel_true;
                 if (rel_true <= tol) break;
             }
+            if (it == maxit) break;
             precondition<float>(H, pc, pre, post);
             const double rz_new = dot<float, float>(H, pc.F.data(), pc.U.data(), n);
             const double beta = rz_new / rz;
             rz = rz_new;
-            launch_pcg_update_p<float, float>(H.p32.data(), pc.U.data(), beta, n, H.s);
+            int nparts = 0;
+            launch_fine_pcg_step_modal(L0.g, P->cv(), pc.U.cv(), (float)beta, PN->v(), H.dx32.v(), (float)pending,
+                                       H.q32.v(), H.rhop.as<float>(), L0.vflag.as<uint8_t>(), H.partials.as<double>(),
+                                       &nparts, H.s);
+            pq = sum_partials_host(H, nparts);
+            pending = 0.0;
+            std::swap(P, PN);
         }
+        if (!flushed) launch_pcg_flush_x(H.x64.data(), H.dx32.data(), P->data(), pending, n, H.s);
         if (it > maxit) it = maxit;
     }
     RT_HIP(hipStreamSynchronize(H.s));

@@ -119,7 +119,9 @@ void launch_coarse_residual(const Geom& g, Vec3<const T> U, Vec3<const T> F, Vec
                             const uint8_t* vflag, hipStream_t s);
 template <typename T>
 void launch_restrict(const Geom& gf, const int of[3], Vec3<const T> Rf, const Geom& gc, const int oc[3], Vec3<T> Fc,
-                     const uint8_t* vflag_c, int ratio, hipStream_t s);
+                     const uint8_t* vflag_c, int ratio, T* scratch, hipStream_t s);
+// elements of T the two-stage x4 restriction needs as scratch
+size_t restrict_scratch_elems(const Geom& gf, const Geom& gc);
 template <typename T>
 void launch_prolong(const Geom& gf, const int of[3], Vec3<T> Uf, const uint8_t* vflag_f, const Geom& gc,
                     const int oc[3], Vec3<const T> Uc, int ratio, bool skip_fixed, hipStream_t s);
@@ -138,8 +140,12 @@ void launch_sum_partials(const double* partials, int nparts, double* out, hipStr
 template <typename T, typename TZ>
 void launch_pcg_update_x_r(T* x, T* r, const T* p, const T* q, double alpha, size_t n, double* partials, int nparts,
                            TZ* rz_out, hipStream_t s);
-void launch_pcg_ru_update_x_r(double* x, float* r, const float* p, const float* q, double alpha, size_t n,
-                              double* partials, int nparts, hipStream_t s);
+void launch_pcg_r_update(float* r, const float* q, double alpha, size_t n, double* partials, int nparts,
+                         hipStream_t s);
+void launch_pcg_flush_x(double* x, float* dx, const float* p, double alpha, size_t n, hipStream_t s);
+void launch_fine_pcg_step_modal(const Geom& g, Vec3<const float> P, Vec3<const float> Z, float beta, Vec3<float> PN,
+                                Vec3<float> DX, float alpha, Vec3<float> Q, const float* rhop, const uint8_t* vflag,
+                                double* partials, int* nparts, hipStream_t s);
 template <typename T, typename TZ>
 void launch_pcg_update_p(T* p, const TZ* z, double beta, size_t n, hipStream_t s);
 template <typename T, typename TZ>
