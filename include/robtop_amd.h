@@ -274,6 +274,9 @@ int rt_time_vcycle(rt_hier* h, int bits, int pre, int post, int n, double* ms_pe
 /* Times n full fine-level smoothing sweeps (colours 0..7, run as four
  * colour-pair launches) of precision `bits`; returns mean ms per sweep. */
 int rt_time_fine_sweep(rt_hier* h, int bits, int n, double* ms_per_sweep);
+/* Times n finest-level residual evaluations r = f - K u (precision `bits`,
+ * u = the current solution, f = the current force); returns mean ms each. */
+int rt_time_fine_residual(rt_hier* h, int bits, int n, double* ms_per_residual);
 
 #ifdef __cplusplus
 }

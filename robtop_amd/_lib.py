@@ -102,6 +102,7 @@ SIGNATURES = {
     "rt_synchronize": ([P], I),
     "rt_time_vcycle": ([P, I, I, I, I, P], I),
     "rt_time_fine_sweep": ([P, I, I, P], I),
+    "rt_time_fine_residual": ([P, I, I, P], I),
 }
 
 

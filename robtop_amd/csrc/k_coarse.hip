@@ -2,6 +2,7 @@
 // deterministic FP64 reductions and the PCG vector updates.
 #include "rt_internal.h"
 
+#include <type_traits>
 #include <utility>
 
 namespace rt {
@@ -649,6 +650,20 @@ __device__ __forceinline__ double blk_sum(double v) {
     return t;
 }
 
+// partials of a.b, four FP32 values per lane
+__global__ void __launch_bounds__(256) k_dot4_partials(const float* __restrict__ a, const float* __restrict__ b,
+                                                       size_t n4, double* __restrict__ partials) {
+    const float4* a4 = reinterpret_cast<const float4*>(a);
+    const float4* b4 = reinterpret_cast<const float4*>(b);
+    double s = 0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
+        const float4 x = a4[i], y = b4[i];
+        s += ((double)x.x * y.x + (double)x.y * y.y) + ((double)x.z * y.z + (double)x.w * y.w);
+    }
+    s = blk_sum(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
 template <typename TA, typename TB>
 __global__ void __launch_bounds__(256) k_dot_partials(const TA* __restrict__ a, const TB* __restrict__ b, size_t n,
                                                       double* __restrict__ partials) {
@@ -661,6 +676,13 @@ __global__ void __launch_bounds__(256) k_dot_partials(const TA* __restrict__ a, 
 
 template <typename TA, typename TB>
 void launch_dot_partials_t(const TA* a, const TB* b, size_t n, double* partials, int nparts, hipStream_t s) {
+    if constexpr (std::is_same<TA, float>::value && std::is_same<TB, float>::value) {
+        if (n % 4 == 0) {
+            hipLaunchKernelGGL(k_dot4_partials, dim3(nparts), dim3(256), 0, s, a, b, n / 4, partials);
+            RT_HIP(hipGetLastError());
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_dot_partials<TA, TB>), dim3(nparts), dim3(256), 0, s, a, b, n, partials);
     RT_HIP(hipGetLastError());
 }
@@ -704,36 +726,64 @@ void launch_pcg_update_x_r(T* x, T* r, const T* p, const T* q, double alpha, siz
 }
 
 // reliable-update PCG step: x (FP64) += a p, r (FP32) -= a q; partials of r.r
-__global__ void __launch_bounds__(256) k_pcg_r_update(float* __restrict__ r, const float* __restrict__ q, float a,
-                                                      size_t n, double* __restrict__ partials) {
+// The FP32 streaming kernels of the reliable-update PCG take four values per
+// lane per step (16 B loads) when n is a multiple of 4 (it is: n = 3 * 8 cs),
+// over kStreamBlocks workgroups, so enough bytes are in flight to approach
+// the HBM rate.
+__global__ void __launch_bounds__(256) k_pcg_ru_dxr(float* __restrict__ r, const float* __restrict__ q,
+                                                    float* __restrict__ dx, const float* __restrict__ p, float a,
+                                                    size_t n4, double* __restrict__ partials) {
+    float4* r4 = reinterpret_cast<float4*>(r);
+    float4* d4 = reinterpret_cast<float4*>(dx);
+    const float4* q4 = reinterpret_cast<const float4*>(q);
+    const float4* p4 = reinterpret_cast<const float4*>(p);
     double s = 0;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const float ri = r[i] - a * q[i];
-        r[i] = ri;
-        s += (double)ri * (double)ri;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
+        float4 ri = r4[i];
+        const float4 qi = q4[i], pi = p4[i];
+        float4 di = d4[i];
+        ri.x -= a * qi.x; ri.y -= a * qi.y; ri.z -= a * qi.z; ri.w -= a * qi.w;
+        di.x += a * pi.x; di.y += a * pi.y; di.z += a * pi.z; di.w += a * pi.w;
+        r4[i] = ri;
+        d4[i] = di;
+        s += ((double)ri.x * ri.x + (double)ri.y * ri.y) + ((double)ri.z * ri.z + (double)ri.w * ri.w);
     }
     s = blk_sum(s);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
-void launch_pcg_r_update(float* r, const float* q, double alpha, size_t n, double* partials, int nparts,
-                         hipStream_t s) {
-    hipLaunchKernelGGL(k_pcg_r_update, dim3(nparts), dim3(256), 0, s, r, q, (float)alpha, n, partials);
+void launch_pcg_ru_dxr(float* r, const float* q, float* dx, const float* p, double alpha, size_t n, double* partials,
+                       int nparts, hipStream_t s) {
+    if (n % 4) throw std::logic_error("launch_pcg_ru_dxr: n must be a multiple of 4");
+    hipLaunchKernelGGL(k_pcg_ru_dxr, dim3(nparts), dim3(256), 0, s, r, q, dx, p, (float)alpha, n / 4, partials);
     RT_HIP(hipGetLastError());
 }
 
-// x += dx + alpha p (FP64), dx = 0
-__global__ void k_pcg_flush_x(double* __restrict__ x, float* __restrict__ dx, const float* __restrict__ p,
-                              double alpha, size_t n) {
+// x += dx (FP64), dx = 0
+__global__ void k_pcg_flush_x(double* __restrict__ x, float* __restrict__ dx, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        x[i] += (double)dx[i] + alpha * (double)p[i];
+        x[i] += (double)dx[i];
         dx[i] = 0.f;
     }
 }
-void launch_pcg_flush_x(double* x, float* dx, const float* p, double alpha, size_t n, hipStream_t s) {
+void launch_pcg_flush_x(double* x, float* dx, size_t n, hipStream_t s) {
     unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
-    hipLaunchKernelGGL(k_pcg_flush_x, dim3(nb), dim3(256), 0, s, x, dx, p, alpha, n);
+    hipLaunchKernelGGL(k_pcg_flush_x, dim3(nb), dim3(256), 0, s, x, dx, n);
     RT_HIP(hipGetLastError());
 }
+
+// p = z + b p, four FP32 values per lane
+__global__ void __launch_bounds__(256) k_pcg_p4(float* __restrict__ p, const float* __restrict__ z, float b,
+                                                size_t n4) {
+    float4* p4 = reinterpret_cast<float4*>(p);
+    const float4* z4 = reinterpret_cast<const float4*>(z);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
+        const float4 zi = z4[i];
+        float4 pi = p4[i];
+        pi.x = zi.x + b * pi.x; pi.y = zi.y + b * pi.y; pi.z = zi.z + b * pi.z; pi.w = zi.w + b * pi.w;
+        p4[i] = pi;
+    }
+}
+
 
 // p = z + b p
 template <typename T, typename TZ>
@@ -744,6 +794,13 @@ __global__ void k_pcg_p(T* __restrict__ p, const TZ* __restrict__ z, double beta
 }
 template <typename T, typename TZ>
 void launch_pcg_update_p(T* p, const TZ* z, double beta, size_t n, hipStream_t s) {
+    if constexpr (std::is_same<T, float>::value && std::is_same<TZ, float>::value) {
+        if (n % 4 == 0) {
+            hipLaunchKernelGGL(k_pcg_p4, dim3(kStreamBlocks), dim3(256), 0, s, p, z, (float)beta, n / 4);
+            RT_HIP(hipGetLastError());
+            return;
+        }
+    }
     unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
     hipLaunchKernelGGL((k_pcg_p<T, TZ>), dim3(nb), dim3(256), 0, s, p, z, beta, n);
     RT_HIP(hipGetLastError());

@@ -20,6 +20,10 @@
 
 namespace rt {
 
+#ifndef RT_MODAL_XCD
+#define RT_MODAL_XCD 1
+#endif
+
 namespace {
 constexpr int TX = 31, TY = 7;             // node tile
 constexpr int EX = TX + 1, EY = TY + 1;    // element tile (= 256 threads)
@@ -82,23 +86,11 @@ __device__ __forceinline__ double blk_sum_m(double v) {
 }
 
 // MODE 0: R = F - K U.  MODE 1: R = K U and per-block partials of U.R.
-// MODE 2 (one PCG step of the reliable-update solver, FP32): the direction
-// p' = Z + beta U is formed while the planes are staged and R = K p', with
-// partials of p'.R; the owner of each node also writes p' to PN (a buffer
-// other than U: neighbouring workgroups still read U) and adds alpha U to the
-// deferred solution increment DX.
-template <typename T> struct PcgStep {
-    const T* z[3];
-    T* pn[3];
-    T* dx[3];
-    T beta, alpha;
-};
-
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256) k_fine_res_modal(Geom g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R,
                                                         const float* __restrict__ rhop,
                                                         const uint8_t* __restrict__ vflag, ModalK<T> K,
-                                                        double* __restrict__ partials, PcgStep<T> ps) {
+                                                        double* __restrict__ partials) {
     constexpr bool APPLY = MODE != 0;
     __shared__ T su[2][3][PLANE];
     __shared__ uint8_t sf[2][PLANE];
@@ -106,7 +98,13 @@ __global__ void __launch_bounds__(256) k_fine_res_modal(Geom g, Vec3<const T> U,
 
     const int tid = threadIdx.x;
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * ZC;
+    // Tile of this workgroup: the flattened block index is remapped so that
+    // each XCD marches one contiguous range of tiles (x fastest), so the tiles
+    // that share halo rows and partly used cache lines meet in one L2.
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int bl = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int bm = RT_MODAL_XCD ? xcd_row(bl, gx * gy * (int)gridDim.z) : bl;
+    const int x0 = (bm % gx) * TX, y0 = ((bm / gx) % gy) * TY, z0 = (bm / (gx * gy)) * ZC;
     const int z1 = min(z0 + ZC, nz);
     // element owned by this thread (tile-local i, j -> global x0 - 1 + i, y0 - 1 + j)
     const int ei = tid % EX, ej = tid / EX;
@@ -118,54 +116,53 @@ __global__ void __launch_bounds__(256) k_fine_res_modal(Geom g, Vec3<const T> U,
     const int gnx = x0 + na, gny = y0 + nb;
     const bool n_in = own && gnx < nx && gny < ny;
 
-    // Each thread stages plane points tid and tid + 256 (PLANE < 512). Loads for
-    // the plane after next are issued before the element work of the current
-    // step, so their latency overlaps it.
-    T pu[2][3];
-    uint8_t pfl[2];
-    auto fetch = [&](int z) {
+    // Each thread stages plane points tid and tid + 256 (PLANE < 512). A plane's
+    // flags and values are loaded together (absent and ghost slots hold zeros;
+    // support values are zeroed once the flag is in), and the loads for plane
+    // ez + 3 are issued at step ez, two steps before the plane is committed
+    // to LDS, so their latency overlaps two steps of element work.
+    struct Stage { T u[2][3]; uint8_t fl[2]; };
+    auto fetch = [&](Stage& st, int z) {
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const int p = tid + 256 * q;
-            pu[q][0] = pu[q][1] = pu[q][2] = T(0);
-            pfl[q] = 0;
+            st.u[q][0] = st.u[q][1] = st.u[q][2] = T(0);
+            st.fl[q] = 0;
             if (p >= PLANE) continue;
             const int px = x0 - 1 + p % PX, py = y0 - 1 + p / PX;
             if (px >= 0 && py >= 0 && z >= 0 && px < nx && py < ny && z < nz) {
-                const int s = oct_p(g, px, py, z);
-                const uint8_t fl = vflag[s];
-                pfl[q] = fl;
-                if ((fl & NF_ACTIVE) && !(fl & NF_FIXED)) {
-                    if (MODE == 2) {
-#pragma unroll
-                        for (int c = 0; c < 3; c++) pu[q][c] = ps.z[c][s] + ps.beta * U.c[c][s];
-                    } else {
-                        pu[q][0] = U.c[0][s]; pu[q][1] = U.c[1][s]; pu[q][2] = U.c[2][s];
-                    }
-                }
+                const int sl = oct_p(g, px, py, z);
+                st.fl[q] = vflag[sl];
+                st.u[q][0] = U.c[0][sl]; st.u[q][1] = U.c[1][sl]; st.u[q][2] = U.c[2][sl];
             }
         }
     };
-    auto commit = [&](int slot) {
+    auto commit = [&](const Stage& st, int slot) {
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const int p = tid + 256 * q;
             if (p >= PLANE) continue;
-            su[slot][0][p] = pu[q][0]; su[slot][1][p] = pu[q][1]; su[slot][2][p] = pu[q][2];
-            sf[slot][p] = pfl[q];
+            const bool live = (st.fl[q] & NF_ACTIVE) && !(st.fl[q] & NF_FIXED);
+            su[slot][0][p] = live ? st.u[q][0] : T(0);
+            su[slot][1][p] = live ? st.u[q][1] : T(0);
+            su[slot][2][p] = live ? st.u[q][2] : T(0);
+            sf[slot][p] = st.fl[q];
         }
     };
 
     T part[3] = {T(0), T(0), T(0)};
     double dotacc = 0.0;
     int lo = 0;
-    fetch(z0 - 1);
-    commit(lo);
-    fetch(z0);
+    Stage sa, sb;
+    fetch(sa, z0 - 1);
+    commit(sa, lo);
+    fetch(sa, z0);
+    fetch(sb, z0 + 1);
     for (int ez = z0 - 1; ez < z1; ez++) {
         const int hi = lo ^ 1;
-        commit(hi);            // plane ez + 1
-        fetch(ez + 2);         // in flight during this step
+        commit(sa, hi);        // plane ez + 1
+        sa = sb;
+        fetch(sb, ez + 3);     // in flight during this step and the next
         __syncthreads();
         // element (gex, gey, ez) between planes lo (kz = 0) and hi (kz = 1)
         T y[24];
@@ -218,13 +215,6 @@ __global__ void __launch_bounds__(256) k_fine_res_modal(Geom g, Vec3<const T> U,
                         R.c[0][s] = KU[0]; R.c[1][s] = KU[1]; R.c[2][s] = KU[2];
                         dotacc += (double)KU[0] * (double)su[lo][0][p] + (double)KU[1] * (double)su[lo][1][p] +
                                   (double)KU[2] * (double)su[lo][2][p];
-                        if (MODE == 2) {
-#pragma unroll
-                            for (int c = 0; c < 3; c++) {
-                                ps.dx[c][s] += ps.alpha * U.c[c][s];
-                                ps.pn[c][s] = su[lo][c][p];
-                            }
-                        }
                     } else {
                         if (fl & NF_FIXED) {
                             R.c[0][s] = F.c[0][s]; R.c[1][s] = F.c[1][s]; R.c[2][s] = F.c[2][s];
@@ -242,7 +232,7 @@ __global__ void __launch_bounds__(256) k_fine_res_modal(Geom g, Vec3<const T> U,
     }
     if (APPLY && partials) {
         const double t = blk_sum_m(dotacc);
-        if (tid == 0) partials[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = t;
+        if (tid == 0) partials[bl] = t;
     }
 }
 
@@ -294,7 +284,7 @@ template <typename T>
 void launch_fine_residual_modal(const Geom& g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R, const float* rhop,
                                 const uint8_t* vflag, hipStream_t s) {
     hipLaunchKernelGGL((k_fine_res_modal<T, 0>), modal_grid(g), dim3(256), 0, s, g, U, F, R, rhop, vflag,
-                       modal_values<T>(g_modal), (double*)nullptr, PcgStep<T>{});
+                       modal_values<T>(g_modal), (double*)nullptr);
     RT_HIP(hipGetLastError());
 }
 
@@ -302,20 +292,7 @@ template <typename T>
 void launch_fine_apply_modal(const Geom& g, Vec3<const T> X, Vec3<T> Y, const float* rhop, const uint8_t* vflag,
                              double* partials, int* nparts, hipStream_t s) {
     hipLaunchKernelGGL((k_fine_res_modal<T, 1>), modal_grid(g), dim3(256), 0, s, g, X, X, Y, rhop, vflag,
-                       modal_values<T>(g_modal), partials, PcgStep<T>{});
-    RT_HIP(hipGetLastError());
-    if (nparts) *nparts = fine_apply_partials_modal(g);
-}
-
-void launch_fine_pcg_step_modal(const Geom& g, Vec3<const float> P, Vec3<const float> Z, float beta, Vec3<float> PN,
-                                Vec3<float> DX, float alpha, Vec3<float> Q, const float* rhop, const uint8_t* vflag,
-                                double* partials, int* nparts, hipStream_t s) {
-    PcgStep<float> ps;
-    for (int c = 0; c < 3; c++) { ps.z[c] = Z.c[c]; ps.pn[c] = PN.c[c]; ps.dx[c] = DX.c[c]; }
-    ps.beta = beta;
-    ps.alpha = alpha;
-    hipLaunchKernelGGL((k_fine_res_modal<float, 2>), modal_grid(g), dim3(256), 0, s, g, P, P, Q, rhop, vflag,
-                       modal_values<float>(g_modal), partials, ps);
+                       modal_values<T>(g_modal), partials);
     RT_HIP(hipGetLastError());
     if (nparts) *nparts = fine_apply_partials_modal(g);
 }

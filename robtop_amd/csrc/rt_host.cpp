@@ -104,7 +104,7 @@ struct Hier {
     // PCG work
     V3<double> r64, p64, q64;
     V3<float> x32, b32, r32, p32, q32;
-    V3<float> p32b, dx32;    // reliable-update PCG: second direction buffer, FP32 solution increment
+    V3<float> dx32;          // reliable-update PCG: FP32 solution increment since the last replacement
     LevelVecs<double> pc64;  // preconditioner level-0 vectors
     LevelVecs<float> pc32;
     DBuf partials, scal, fbuf, stage;
@@ -529,7 +529,7 @@ static void precondition(Hier& H, LevelVecs<TC>& pc, int pre, int post) {
 
 template <typename TA, typename TB>
 static double dot(Hier& H, const TA* a, const TB* b, size_t n) {
-    const int np = 1024;
+    const int np = kStreamBlocks;
     launch_dot_partials_t<TA, TB>(a, b, n, H.partials.as<double>(), np, H.s);
     launch_sum_partials(H.partials.as<double>(), np, H.scal.as<double>(), H.s);
     double v;
@@ -813,7 +813,6 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
     if (!H.r64.ok()) H.r64.alloc(N);
     if (!H.q64.ok()) H.q64.alloc(N);
     if (!H.p32.ok()) H.p32.alloc(N);
-    if (!H.p32b.ok()) H.p32b.alloc(N);
     if (!H.q32.ok()) H.q32.alloc(N);
     if (!H.dx32.ok()) H.dx32.alloc(N);
     LevelVecs<float>& pc = pcvecs<float>(H);   // pc.F holds the FP32 residual, pc.U the preconditioned one
@@ -833,30 +832,25 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
     int it = 0;
     if (nb > 0 && rel > tol) {
         // The solution increment since the last residual replacement is kept
-        // in FP32 (dx) and flushed into x64 at each replacement; alpha_k p_k is
-        // added to dx by the next step's fused apply (which reads p_k to form
-        // p_k+1 = z + beta p_k), or by the flush. p ping-pongs between two
-        // buffers because the fused apply must not overwrite the p it reads.
+        // in FP32 (dx, updated with r) and flushed into x64 at each
+        // replacement and at exit.
+        const int ns = kStreamBlocks;
         launch_fill<float>(H.dx32.data(), 0.f, n, H.s);
         precondition<float>(H, pc, pre, post);
         double rz = dot<float, float>(H, pc.F.data(), pc.U.data(), n);
-        V3<float>* P = &H.p32;
-        V3<float>* PN = &H.p32b;
-        launch_convert<float, float>(pc.U.data(), P->data(), n, H.s);
-        double pq = apply_dot<float>(H, *P, H.q32);
-        double pending = 0.0;                       // alpha of the p not yet added to dx
+        launch_convert<float, float>(pc.U.data(), H.p32.data(), n, H.s);
         bool flushed = false;
         for (it = 1; it <= maxit; it++) {
+            const double pq = apply_dot<float>(H, H.p32, H.q32);
             if (!(pq > 0)) throw std::domain_error("PCG breakdown: p.Ap <= 0");
             const double alpha = rz / pq;
-            launch_pcg_r_update(pc.F.data(), H.q32.data(), alpha, n, H.partials.as<double>(), np, H.s);
-            pending = alpha;
+            launch_pcg_ru_dxr(pc.F.data(), H.q32.data(), H.dx32.data(), H.p32.data(), alpha, n,
+                              H.partials.as<double>(), ns, H.s);
             flushed = false;
-            const double rn = std::sqrt(sum_partials_host(H, np));
+            const double rn = std::sqrt(sum_partials_host(H, ns));
             rel = rn / nb;
             if (rel <= tol || rn < 0.1 * rmax) {
-                launch_pcg_flush_x(H.x64.data(), H.dx32.data(), P->data(), pending, n, H.s);
-                pending = 0.0;
+                launch_pcg_flush_x(H.x64.data(), H.dx32.data(), n, H.s);
                 flushed = true;
                 rel_true = true_rel_residual(H);      // r64 = f - K x64
                 launch_convert<double, float>(H.r64.data(), pc.F.data(), n, H.s);
@@ -869,15 +863,9 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
             const double rz_new = dot<float, float>(H, pc.F.data(), pc.U.data(), n);
             const double beta = rz_new / rz;
             rz = rz_new;
-            int nparts = 0;
-            launch_fine_pcg_step_modal(L0.g, P->cv(), pc.U.cv(), (float)beta, PN->v(), H.dx32.v(), (float)pending,
-                                       H.q32.v(), H.rhop.as<float>(), L0.vflag.as<uint8_t>(), H.partials.as<double>(),
-                                       &nparts, H.s);
-            pq = sum_partials_host(H, nparts);
-            pending = 0.0;
-            std::swap(P, PN);
+            launch_pcg_update_p<float, float>(H.p32.data(), pc.U.data(), beta, n, H.s);
         }
-        if (!flushed) launch_pcg_flush_x(H.x64.data(), H.dx32.data(), P->data(), pending, n, H.s);
+        if (!flushed) launch_pcg_flush_x(H.x64.data(), H.dx32.data(), n, H.s);
         if (it > maxit) it = maxit;
     }
     RT_HIP(hipStreamSynchronize(H.s));
@@ -1872,6 +1860,36 @@ int rt_time_fine_sweep(rt_hier* h, int bits, int n, double* ms) {
         need(ms);
         Hier& H = H_(h);
         *ms = bits == 64 ? time_sweep<double>(H, n) : time_sweep<float>(H, n);
+    })
+}
+
+}  // extern "C"
+template <typename T>
+static double time_residual(Hier& H, int n) {
+    LevelVecs<T>& pc = pcvecs<T>(H);
+    const size_t nn = (size_t)3 * H.lv[0].g.N;
+    launch_convert<double, T>(H.f64.data(), pc.F.data(), nn, H.s);
+    launch_convert<double, T>(H.x64.data(), pc.U.data(), nn, H.s);
+    hipEvent_t a, b;
+    RT_HIP(hipEventCreate(&a));
+    RT_HIP(hipEventCreate(&b));
+    RT_HIP(hipEventRecord(a, H.s));
+    for (int i = 0; i < n; i++) residual<T>(H, 0, pc.U.cv(), pc.F.cv(), pc.R.v());
+    RT_HIP(hipEventRecord(b, H.s));
+    RT_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    RT_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms / std::max(1, n);
+}
+extern "C" {
+
+int rt_time_fine_residual(rt_hier* h, int bits, int n, double* ms) {
+    RT_TRY({
+        need(ms);
+        Hier& H = H_(h);
+        *ms = bits == 64 ? time_residual<double>(H, n) : time_residual<float>(H, n);
     })
 }
 

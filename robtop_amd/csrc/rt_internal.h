@@ -140,12 +140,13 @@ void launch_sum_partials(const double* partials, int nparts, double* out, hipStr
 template <typename T, typename TZ>
 void launch_pcg_update_x_r(T* x, T* r, const T* p, const T* q, double alpha, size_t n, double* partials, int nparts,
                            TZ* rz_out, hipStream_t s);
-void launch_pcg_r_update(float* r, const float* q, double alpha, size_t n, double* partials, int nparts,
-                         hipStream_t s);
-void launch_pcg_flush_x(double* x, float* dx, const float* p, double alpha, size_t n, hipStream_t s);
-void launch_fine_pcg_step_modal(const Geom& g, Vec3<const float> P, Vec3<const float> Z, float beta, Vec3<float> PN,
-                                Vec3<float> DX, float alpha, Vec3<float> Q, const float* rhop, const uint8_t* vflag,
-                                double* partials, int* nparts, hipStream_t s);
+// reliable-update PCG (FP32 vectors): r -= alpha q, dx += alpha p, partials of r.r
+void launch_pcg_ru_dxr(float* r, const float* q, float* dx, const float* p, double alpha, size_t n, double* partials,
+                       int nparts, hipStream_t s);
+// x64 += dx, dx = 0
+void launch_pcg_flush_x(double* x, float* dx, size_t n, hipStream_t s);
+// streaming kernels' grid size (partials written by the reducing ones)
+constexpr int kStreamBlocks = 2048;
 template <typename T, typename TZ>
 void launch_pcg_update_p(T* p, const TZ* z, double beta, size_t n, hipStream_t s);
 template <typename T, typename TZ>

@@ -340,6 +340,11 @@ class HierarchyGrid:
         check(lib().rt_time_fine_sweep(self.handle, bits, n, ctypes.byref(ms)))
         return ms.value
 
+    def time_fine_residual(self, bits=32, n=10):
+        ms = ctypes.c_double()
+        check(lib().rt_time_fine_residual(self.handle, bits, n, ctypes.byref(ms)))
+        return ms.value
+
 
 def voxelize_mesh(coords, faces, prefer_reso):
     """voxelize_mesh (voxelizer/voxelize_main.cpp:344-404) on the GPU.
