@@ -232,6 +232,23 @@ int rt_rand_force(rt_hier* h, uint64_t seed);
  * returns the worst compliance f.u and the number of V-cycles. */
 int rt_modified_pm(rt_hier* h, int max_itn, double* worst_compliance, int* vcycles);
 
+/* Worst-case load search by block Lanczos (no counterpart in the reference,
+ * whose modifiedPM is rt_modified_pm): the top eigenpair of the compliance
+ * operator C = P N^T K^-1 N P on the admissible loads of rt_set_load_nodes,
+ * with block size k (<= 64) from a seeded random block, full
+ * reorthogonalisation, and one rt_solve (mode, V(pre, post), relative residual
+ * solve_tol, zero start) per operator application. Stops when the top Ritz
+ * pair's residual is below tol * theta or after max_blocks blocks. On return
+ * the force is the unit Ritz load, the displacement its solution, and
+ * *worst_compliance = f . u. */
+typedef struct rt_lanczos_stats {
+    int blocks;             /* Lanczos blocks built */
+    int solves;             /* operator applications (PCG solves), the final one included */
+    int pcg_iterations;     /* PCG iterations over all solves */
+} rt_lanczos_stats;
+int rt_block_lanczos(rt_hier* h, int k, int max_blocks, double tol, double solve_tol, int mode, int pre, int post,
+                     uint64_t seed, double* worst_compliance, rt_lanczos_stats* stats);
+
 /* --- MMA optimiser ------------------------------------------------------- */
 
 typedef struct rt_mma rt_mma;

@@ -28,6 +28,10 @@ class SolveStats(ctypes.Structure):
                 ("rel_res_pcg", ctypes.c_double), ("seconds", ctypes.c_double)]
 
 
+class LanczosStats(ctypes.Structure):
+    _fields_ = [("blocks", ctypes.c_int), ("solves", ctypes.c_int), ("pcg_iterations", ctypes.c_int)]
+
+
 P = ctypes.c_void_p
 I = ctypes.c_int
 D = ctypes.c_double
@@ -99,6 +103,7 @@ SIGNATURES = {
     "rt_force_project": ([P], I),
     "rt_rand_force": ([P, ctypes.c_uint64], I),
     "rt_modified_pm": ([P, I, P, P], I),
+    "rt_block_lanczos": ([P, I, I, D, D, I, I, I, ctypes.c_uint64, P, P], I),
     "rt_synchronize": ([P], I),
     "rt_time_vcycle": ([P, I, I, I, I, P], I),
     "rt_time_fine_sweep": ([P, I, I, P], I),

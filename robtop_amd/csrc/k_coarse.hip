@@ -30,6 +30,8 @@ __device__ __forceinline__ int c_nb_slot(const Geom& g, int base, int nb) {
     return col * g.cs + base + (oz * g.h[1] + oy) * g.h[0] + ox;
 }
 
+constexpr int kProlongRowMax = 1026;        // longest fine row of the row-per-workgroup transfer kernels
+
 static inline unsigned colour_blocks(const Geom& g, int colour, int bs) {
     const int cx = colour & 1, cy = (colour >> 1) & 1, cz = colour >> 2;
     long long tot = (long long)((g.n[0] - cx + 1) >> 1) * ((g.n[1] - cy + 1) >> 1) * ((g.n[2] - cz + 1) >> 1);
@@ -381,35 +383,66 @@ __global__ void __launch_bounds__(256) k_restrict(Geom gf, int ofx, int ofy, int
 //   stage 2: F_c = sum_dz (4 - |dz|) W(xc, yc, fz + dz) / 64.
 // Stage 1 reads each fine value from HBM once (its 7 x 7 window is shared by
 // neighbouring lanes through L1/L2); stage 2 reads the 1/16-size scratch.
+// Stage 1 as one workgroup per (coarse y, fine z): the 7 fine rows of the y
+// window are staged in LDS one after another with unit-stride loads of each
+// parity block, and each thread accumulates the (x, y) window of its coarse x
+// from LDS. Workgroups are mapped so that each XCD takes a contiguous run of
+// coarse y (consecutive windows share 3 fine rows in its L2).
 template <typename T, int RATIO>
 __global__ void __launch_bounds__(256) k_restrict_xy(Geom gf, int ofx, int ofy, Vec3<const T> Rf, int ncx, int ncy,
                                                      int ocx, int ocy, T* __restrict__ W) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nz = gf.n[2];
-    const int tot = ncx * ncy * nz;
-    if (t >= tot) return;
-    const int xc = t % ncx, r = t / ncx, yc = r % ncy, z = r / ncy;
-    const int fx = (ocx + xc) * RATIO - ofx, fy = (ocy + yc) * RATIO - ofy;
+    __shared__ T row[3][kProlongRowMax];
+    const int nx = gf.n[0], ny = gf.n[1], nz = gf.n[2];
+    const int b = xcd_row(blockIdx.x, gridDim.x);
+    const int yc = b % ncy, z = b / ncy;
+    const int fy = (ocy + yc) * RATIO - ofy;
     constexpr int Wd = RATIO - 1;
-    T s0 = T(0), s1 = T(0), s2 = T(0);
+    constexpr int MAXC = (kProlongRowMax + 255) / 256;
+    T s[MAXC][3];
+#pragma unroll
+    for (int m = 0; m < MAXC; m++) s[m][0] = s[m][1] = s[m][2] = T(0);
+    const int nh = (nx + 1) >> 1;
     for (int dy = -Wd; dy <= Wd; dy++) {
         const int py = fy + dy;
-        if (py < 0 || py >= gf.n[1]) continue;
+        if (py < 0 || py >= ny) continue;           // uniform across the workgroup
+        __syncthreads();                             // the previous row has been consumed
+        for (int k = threadIdx.x; k < nh; k += blockDim.x) {
+#pragma unroll
+            for (int par = 0; par < 2; par++) {
+                const int x = 2 * k + par;
+                if (x >= nx) continue;
+                const int sl = oct_p(gf, x, py, z);
+                row[0][x] = Rf.c[0][sl]; row[1][x] = Rf.c[1][sl]; row[2][x] = Rf.c[2][sl];
+            }
+        }
+        __syncthreads();
         const int wy = RATIO - (dy < 0 ? -dy : dy);
 #pragma unroll
-        for (int dx = -Wd; dx <= Wd; dx++) {
-            const int px = fx + dx;
-            if (px < 0 || px >= gf.n[0]) continue;
-            const T w = (T)(wy * (RATIO - (dx < 0 ? -dx : dx)));
-            const int sl = oct_p(gf, px, py, z);
-            s0 += w * Rf.c[0][sl];
-            s1 += w * Rf.c[1][sl];
-            s2 += w * Rf.c[2][sl];
+        for (int m = 0; m < MAXC; m++) {
+            const int xc = threadIdx.x + 256 * m;
+            if (xc >= ncx) continue;
+            const int fx = (ocx + xc) * RATIO - ofx;
+#pragma unroll
+            for (int dx = -Wd; dx <= Wd; dx++) {
+                const int px = fx + dx;
+                if (px < 0 || px >= nx) continue;
+                const T w = (T)(wy * (RATIO - (dx < 0 ? -dx : dx)));
+                s[m][0] += w * row[0][px];
+                s[m][1] += w * row[1][px];
+                s[m][2] += w * row[2][px];
+            }
         }
     }
-    W[t] = s0;
-    W[(size_t)tot + t] = s1;
-    W[2 * (size_t)tot + t] = s2;
+    const size_t tot = (size_t)ncx * ncy * nz;
+#pragma unroll
+    for (int m = 0; m < MAXC; m++) {
+        const int xc = threadIdx.x + 256 * m;
+        if (xc >= ncx) continue;
+        const size_t t = ((size_t)z * ncy + yc) * ncx + xc;
+        W[t] = s[m][0];
+        W[tot + t] = s[m][1];
+        W[2 * tot + t] = s[m][2];
+    }
 }
 
 template <typename T, int RATIO>
@@ -448,10 +481,10 @@ void launch_restrict(const Geom& gf, const int of[3], Vec3<const T> Rf, const Ge
                      const uint8_t* vflag_c, int ratio, T* scratch, hipStream_t s) {
     long long tot = (long long)gc.n[0] * gc.n[1] * gc.n[2];
     dim3 grid((unsigned)((tot + 255) / 256));
-    if (ratio == 4 && scratch) {
-        const long long t1 = (long long)gc.n[0] * gc.n[1] * gf.n[2];
-        hipLaunchKernelGGL((k_restrict_xy<T, 4>), dim3((unsigned)((t1 + 255) / 256)), dim3(256), 0, s, gf, of[0],
-                           of[1], Rf, gc.n[0], gc.n[1], oc[0], oc[1], scratch);
+    if (ratio == 4 && scratch && gf.n[0] <= kProlongRowMax) {
+        const long long wgs = (long long)gc.n[1] * gf.n[2];
+        hipLaunchKernelGGL((k_restrict_xy<T, 4>), dim3((unsigned)wgs), dim3(256), 0, s, gf, of[0], of[1], Rf,
+                           gc.n[0], gc.n[1], oc[0], oc[1], scratch);
         RT_HIP(hipGetLastError());
         hipLaunchKernelGGL((k_restrict_z<T, 4>), grid, dim3(256), 0, s, gf.n[2], of[2], (const T*)scratch, gc, oc[2],
                            Fc, vflag_c);
@@ -497,62 +530,84 @@ __global__ void __launch_bounds__(256) k_prolong(Geom gf, int ofx, int ofy, int 
     Uf.c[0][v] += c0; Uf.c[1][v] += c1; Uf.c[2][v] += c2;
 }
 
-// x4 prolongation, one thread per coarse x-interval of a fine row: the thread
-// interpolates the coarse values in y and z at the interval's two ends (8
-// coarse nodes) and then in x for the RATIO fine nodes of the interval, so a
-// fine node costs 6 coarse loads instead of 24. Interval gxl covers absolute
-// fine x in [RATIO (gx0 + gxl), RATIO (gx0 + gxl + 1)).
+// x4 prolongation, one workgroup per fine row (y, z). Each thread takes coarse
+// x-intervals of the row: it interpolates the coarse values in y and z at the
+// interval's two ends (8 coarse nodes) and then in x for the RATIO fine nodes
+// of the interval, so a fine node costs 6 coarse loads instead of 24. The
+// increments meet in LDS by (parity, x / 2), and the row is then updated with
+// unit-stride loads and stores of each parity block. Interval gxl covers
+// absolute fine x in [RATIO (gx0 + gxl), RATIO (gx0 + gxl + 1)).
 template <typename T, int RATIO>
-__global__ void __launch_bounds__(256) k_prolong_iv(Geom gf, int ofx, int ofy, int ofz, Vec3<T> Uf,
-                                                    const uint8_t* __restrict__ vflag_f, Geom gc, int ocx, int ocy,
-                                                    int ocz, Vec3<const T> Uc, int skip_fixed, int gx0, int ngx) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nx = gf.n[0], ny = gf.n[1], nz = gf.n[2];
-    if (t >= ngx * ny * nz) return;
-    const int gxl = t % ngx, r = t / ngx, y = r % ny, z = r / ny;
-    const int bxa = gx0 + gxl;
-    const int xs = bxa * RATIO - ofx;                 // fine x of the interval start
-    const int ilo = xs < 0 ? -xs : 0, ihi = nx - xs < RATIO ? nx - xs : RATIO;
+__global__ void __launch_bounds__(256) k_prolong_row(Geom gf, int ofx, int ofy, int ofz, Vec3<T> Uf,
+                                                     const uint8_t* __restrict__ vflag_f, Geom gc, int ocx, int ocy,
+                                                     int ocz, Vec3<const T> Uc, int skip_fixed, int gx0, int ngx) {
+    constexpr int H = kProlongRowMax / 2;
+    __shared__ T inc[3][2][H];
+    const int nx = gf.n[0], ny = gf.n[1];
+    const int y = blockIdx.x % ny, z = blockIdx.x / ny;
     const int ay = ofy + y, az = ofz + z;
     const int py = ay % RATIO, pz = az % RATIO;
-    const int bx = bxa - ocx, by = ay / RATIO - ocy, bz = az / RATIO - ocz;
-    const bool right = ihi > 1;                       // some node of the interval has a weight on bx + 1
-    T a0 = T(0), a1 = T(0), a2 = T(0), b0 = T(0), b1 = T(0), b2 = T(0);
+    const int by = ay / RATIO - ocy, bz = az / RATIO - ocz;
+    constexpr T inv = T(1) / T(RATIO * RATIO * RATIO);
+    for (int gxl = threadIdx.x; gxl < ngx; gxl += blockDim.x) {
+        const int bxa = gx0 + gxl;
+        const int xs = bxa * RATIO - ofx;             // fine x of the interval start
+        const int ilo = xs < 0 ? -xs : 0, ihi = nx - xs < RATIO ? nx - xs : RATIO;
+        const int bx = bxa - ocx;
+        const bool right = ihi > 1;                   // some node of the interval has a weight on bx + 1
+        T a0 = T(0), a1 = T(0), a2 = T(0), b0 = T(0), b1 = T(0), b2 = T(0);
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int wy = (j & 1) ? py : RATIO - py;
-        const int wz = (j & 2) ? pz : RATIO - pz;
-        if (wy * wz == 0) continue;
-        const T w = (T)(wy * wz);
-        const int sa = oct_p(gc, bx, by + (j & 1), bz + (j >> 1));
-        a0 += w * Uc.c[0][sa]; a1 += w * Uc.c[1][sa]; a2 += w * Uc.c[2][sa];
-        if (right) {
-            const int sb = oct_p(gc, bx + 1, by + (j & 1), bz + (j >> 1));
-            b0 += w * Uc.c[0][sb]; b1 += w * Uc.c[1][sb]; b2 += w * Uc.c[2][sb];
+        for (int j = 0; j < 4; j++) {
+            const int wy = (j & 1) ? py : RATIO - py;
+            const int wz = (j & 2) ? pz : RATIO - pz;
+            if (wy * wz == 0) continue;
+            const T w = (T)(wy * wz);
+            const int sa = oct_p(gc, bx, by + (j & 1), bz + (j >> 1));
+            a0 += w * Uc.c[0][sa]; a1 += w * Uc.c[1][sa]; a2 += w * Uc.c[2][sa];
+            if (right) {
+                const int sb = oct_p(gc, bx + 1, by + (j & 1), bz + (j >> 1));
+                b0 += w * Uc.c[0][sb]; b1 += w * Uc.c[1][sb]; b2 += w * Uc.c[2][sb];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < RATIO; i++) {
+            if (i < ilo || i >= ihi) continue;
+            const int x = xs + i;
+            const T wl = (T)(RATIO - i), wr = (T)i;
+            inc[0][x & 1][x >> 1] = (wl * a0 + wr * b0) * inv;
+            inc[1][x & 1][x >> 1] = (wl * a1 + wr * b1) * inv;
+            inc[2][x & 1][x >> 1] = (wl * a2 + wr * b2) * inv;
         }
     }
-    constexpr T inv = T(1) / T(RATIO * RATIO * RATIO);
+    __syncthreads();
+    const int nh = (nx + 1) >> 1;
+    for (int k = threadIdx.x; k < nh; k += blockDim.x) {
 #pragma unroll
-    for (int i = 0; i < RATIO; i++) {
-        if (i < ilo || i >= ihi) continue;
-        const int v = oct_p(gf, xs + i, y, z);
-        const uint8_t fl = vflag_f[v];
-        if (!(fl & NF_ACTIVE) || (skip_fixed && (fl & NF_FIXED))) continue;
-        const T wl = (T)(RATIO - i), wr = (T)i;
-        Uf.c[0][v] += (wl * a0 + wr * b0) * inv;
-        Uf.c[1][v] += (wl * a1 + wr * b1) * inv;
-        Uf.c[2][v] += (wl * a2 + wr * b2) * inv;
+        for (int par = 0; par < 2; par++) {
+            const int x = 2 * k + par;
+            if (x >= nx) continue;
+            const int v = oct_p(gf, x, y, z);
+            const uint8_t fl = vflag_f[v];
+            if (!(fl & NF_ACTIVE) || (skip_fixed && (fl & NF_FIXED))) continue;
+            Uf.c[0][v] += inc[0][par][k];
+            Uf.c[1][v] += inc[1][par][k];
+            Uf.c[2][v] += inc[2][par][k];
+        }
     }
 }
 
 template <typename T>
 void launch_prolong(const Geom& gf, const int of[3], Vec3<T> Uf, const uint8_t* vflag_f, const Geom& gc,
                     const int oc[3], Vec3<const T> Uc, int ratio, bool skip_fixed, hipStream_t s) {
-    if (ratio == 4) {
+    if (ratio == 4 && gf.n[0] <= kProlongRowMax) {
         const int gx0 = of[0] / 4, ngx = (of[0] + gf.n[0] - 1) / 4 - gx0 + 1;
-        long long tot = (long long)ngx * gf.n[1] * gf.n[2];
-        hipLaunchKernelGGL((k_prolong_iv<T, 4>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, gf, of[0],
-                           of[1], of[2], Uf, vflag_f, gc, oc[0], oc[1], oc[2], Uc, (int)skip_fixed, gx0, ngx);
+        const long long rows = (long long)gf.n[1] * gf.n[2];
+        hipLaunchKernelGGL((k_prolong_row<T, 4>), dim3((unsigned)rows), dim3(256), 0, s, gf, of[0], of[1], of[2], Uf,
+                           vflag_f, gc, oc[0], oc[1], oc[2], Uc, (int)skip_fixed, gx0, ngx);
+    } else if (ratio == 4) {
+        long long tot = (long long)gf.n[0] * gf.n[1] * gf.n[2];
+        hipLaunchKernelGGL((k_prolong<T, 4>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, gf, of[0], of[1],
+                           of[2], Uf, vflag_f, gc, oc[0], oc[1], oc[2], Uc, (int)skip_fixed);
     } else {
         long long tot = (long long)gf.n[0] * gf.n[1] * gf.n[2];
         hipLaunchKernelGGL((k_prolong<T, 2>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, gf, of[0], of[1],

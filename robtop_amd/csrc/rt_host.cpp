@@ -1478,17 +1478,20 @@ int rt_vcycle(rt_hier* h, int pre, int post, double* rel_res) {
     })
 }
 
+}  // extern "C"
+static void solve_mode(Hier& H, double tol, int maxit, int mode, int pre, int post, rt_solve_stats* stats) {
+    if (pre < 0 || post < 0 || maxit < 0) throw std::invalid_argument("negative sweep or iteration count");
+    if (mode == RT_SOLVE_FP64) pcg<double, double>(H, tol, maxit, pre, post, stats);
+    else if (mode == RT_SOLVE_FP32) pcg<float, float>(H, tol, maxit, pre, post, stats);
+    else if (mode == RT_SOLVE_MIXED) pcg<double, float>(H, tol, maxit, pre, post, stats);
+    else if (mode == RT_SOLVE_FP32_IR) pcg_ir(H, tol, maxit, pre, post, 1e-4, stats);
+    else if (mode == RT_SOLVE_RELIABLE) pcg_reliable(H, tol, maxit, pre, post, stats);
+    else throw std::invalid_argument("unknown solver mode");
+}
+extern "C" {
+
 int rt_solve(rt_hier* h, double tol, int maxit, int mode, int pre, int post, rt_solve_stats* stats) {
-    RT_TRY({
-        Hier& H = H_(h);
-        if (pre < 0 || post < 0 || maxit < 0) throw std::invalid_argument("negative sweep or iteration count");
-        if (mode == RT_SOLVE_FP64) pcg<double, double>(H, tol, maxit, pre, post, stats);
-        else if (mode == RT_SOLVE_FP32) pcg<float, float>(H, tol, maxit, pre, post, stats);
-        else if (mode == RT_SOLVE_MIXED) pcg<double, float>(H, tol, maxit, pre, post, stats);
-        else if (mode == RT_SOLVE_FP32_IR) pcg_ir(H, tol, maxit, pre, post, 1e-4, stats);
-        else if (mode == RT_SOLVE_RELIABLE) pcg_reliable(H, tol, maxit, pre, post, stats);
-        else throw std::invalid_argument("unknown solver mode");
-    })
+    RT_TRY({ solve_mode(H_(h), tol, maxit, mode, pre, post, stats); })
 }
 
 int rt_set_displacement_box(rt_hier* h, const double* u_box) {
@@ -1650,6 +1653,257 @@ static double modified_pm(Hier& H, int max_itn, int* steps) {
     return dot<double, double>(H, H.f64.data(), H.x64.data(), (size_t)3 * H.lv[0].g.N);
 }
 
+// ---------------------------------------------------------------------------
+// worst-case load search by block Lanczos (no reference counterpart; the
+// reference's modifiedPM above finds the same top eigenpair of C = P N^T K^-1 N P
+// on the admissible load space, P the per-node projector of forceProject)
+// ---------------------------------------------------------------------------
+// Dense symmetric eigenproblem by cyclic Jacobi rotations (small block
+// tridiagonal matrices only): A (n x n, row-major) is destroyed, w gets the
+// eigenvalues and V (row-major, columns = eigenvectors).
+static void jacobi_eigen(int n, std::vector<double>& A, std::vector<double>& w, std::vector<double>& V) {
+    V.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) V[(size_t)i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0, tot = 0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                const double a = A[(size_t)i * n + j] * A[(size_t)i * n + j];
+                tot += a;
+                if (i != j) off += a;
+            }
+        if (off <= 1e-30 * tot || off == 0) break;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++) {
+                const double apq = A[(size_t)p * n + q];
+                if (apq == 0) continue;
+                const double app = A[(size_t)p * n + p], aqq = A[(size_t)q * n + q];
+                const double th = (aqq - app) / (2 * apq);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1));
+                const double c = 1 / std::sqrt(t * t + 1), sn = t * c;
+                for (int k = 0; k < n; k++) {   // A <- J^T A J
+                    const double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+                    A[(size_t)k * n + p] = c * akp - sn * akq;
+                    A[(size_t)k * n + q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    const double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+                    A[(size_t)p * n + k] = c * apk - sn * aqk;
+                    A[(size_t)q * n + k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    const double vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
+                    V[(size_t)k * n + p] = c * vkp - sn * vkq;
+                    V[(size_t)k * n + q] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    w.resize(n);
+    for (int i = 0; i < n; i++) w[i] = A[(size_t)i * n + i];
+}
+
+// Orthonormalise the k columns of W (column j = W[j], each of length m)
+// against the columns of Q (all previous blocks) and among themselves, by
+// modified Gram-Schmidt applied twice. R receives the k x k coefficients of W
+// in the new basis (upper triangular); columns that vanish are replaced by
+// zero (a rank-deficient block) and reported through the return value.
+static int block_orth(std::vector<std::vector<double>>& W, const std::vector<std::vector<double>>& Q,
+                      std::vector<double>& R) {
+    const int k = (int)W.size();
+    const size_t m = W.empty() ? 0 : W[0].size();
+    auto dotv = [m](const std::vector<double>& a, const std::vector<double>& b) {
+        double s = 0;
+        for (size_t i = 0; i < m; i++) s += a[i] * b[i];
+        return s;
+    };
+    R.assign((size_t)k * k, 0.0);
+    int rank = 0;
+    for (int j = 0; j < k; j++) {
+        const double n0 = std::sqrt(dotv(W[j], W[j]));
+        for (int pass = 0; pass < 2; pass++) {
+            for (const auto& q : Q) {
+                const double c = dotv(q, W[j]);
+                for (size_t i = 0; i < m; i++) W[j][i] -= c * q[i];
+            }
+            for (int i = 0; i < j; i++) {
+                const double c = dotv(W[i], W[j]);
+                R[(size_t)i * k + j] += c;
+                for (size_t t = 0; t < m; t++) W[j][t] -= c * W[i][t];
+            }
+        }
+        const double nj = std::sqrt(dotv(W[j], W[j]));
+        if (nj <= 1e-12 * std::max(n0, 1e-300)) {
+            std::fill(W[j].begin(), W[j].end(), 0.0);
+            R[(size_t)j * k + j] = 0;
+            continue;
+        }
+        R[(size_t)j * k + j] = nj;
+        for (size_t i = 0; i < m; i++) W[j][i] /= nj;
+        rank++;
+    }
+    return rank;
+}
+
+// y = C q = P N^T K^-1 N q for a compact load vector q ([3][nl] SoA, in the
+// range of P): force = q at the load nodes, a PCG solve from zero to
+// solve_tol, then the projected gather of the displacement.
+static void worst_apply(Hier& H, const std::vector<double>& q, std::vector<double>& y, double solve_tol, int mode,
+                        int pre, int post, int* iters) {
+    const size_t nl3 = (size_t)3 * H.nload, n = (size_t)3 * H.lv[0].g.N;
+    RT_HIP(hipMemcpyAsync(H.pm_fs.p, q.data(), sizeof(double) * nl3, hipMemcpyHostToDevice, H.s));
+    launch_fill<double>(H.f64.data(), 0.0, n, H.s);
+    pm_commit(H, 1.0);
+    launch_fill<double>(H.x64.data(), 0.0, n, H.s);
+    rt_solve_stats st{};
+    solve_mode(H, solve_tol, 2000, mode, pre, post, &st);
+    if (!(st.rel_res <= 10 * solve_tol)) throw std::runtime_error("worst-case solve did not converge");
+    *iters += st.iterations;
+    pm_gather(H, H.x64);
+    y.resize(nl3);
+    RT_HIP(hipMemcpyAsync(y.data(), H.pm_fs.p, sizeof(double) * nl3, hipMemcpyDeviceToHost, H.s));
+    RT_HIP(hipStreamSynchronize(H.s));
+}
+
+// Block Lanczos with full reorthogonalisation on C restricted to the
+// admissible load space, block size k, from a seeded random block. Stops when
+// the residual of the top Ritz pair, |B_j y_last|, falls below tol * theta, or
+// after max_blocks blocks. Leaves the unit Ritz load in the force and its
+// displacement (solved to solve_tol) in the solution; returns theta.
+static double block_lanczos(Hier& H, int k, int max_blocks, double tol, double solve_tol, int mode, int pre,
+                            int post, uint64_t seed, int* blocks_out, int* solves_out, int* iters_out) {
+    if (!H.stencil_ready) throw std::logic_error("update_stencil has not been called");
+    if (!H.nload) throw std::logic_error("no load nodes set");
+    if (k < 1 || k > 64 || max_blocks < 1) throw std::invalid_argument("block size must be 1..64, max_blocks >= 1");
+    const size_t nl = (size_t)H.nload, m = 3 * nl;
+    // projector P per load node (identity, or n n^T)
+    std::vector<double> nrm;
+    if (H.load_normals) {
+        nrm.resize(m);
+        RT_HIP(hipMemcpy(nrm.data(), H.load_nrm.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+    }
+    auto project = [&](std::vector<double>& v) {
+        if (nrm.empty()) return;
+        for (size_t i = 0; i < nl; i++) {
+            const double d = v[i] * nrm[i] + v[nl + i] * nrm[nl + i] + v[2 * nl + i] * nrm[2 * nl + i];
+            v[i] = d * nrm[i]; v[nl + i] = d * nrm[nl + i]; v[2 * nl + i] = d * nrm[2 * nl + i];
+        }
+    };
+    const size_t dim = nrm.empty() ? m : nl;   // dimension of the admissible space
+    k = (int)std::min<size_t>((size_t)k, dim);
+    // seeded start block (splitmix64 uniforms in [-1, 1))
+    uint64_t st = seed;
+    auto next = [&st]() {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+    };
+    std::vector<std::vector<double>> Q;                      // all basis vectors so far
+    std::vector<std::vector<double>> blk(k, std::vector<double>(m));
+    for (auto& v : blk) {
+        for (auto& x : v) x = next();
+        project(v);
+    }
+    std::vector<double> R;
+    int r = block_orth(blk, Q, R);
+    if (r < 1) throw std::domain_error("block Lanczos start block is zero");
+    std::vector<std::vector<double>> prev;                   // previous block
+    std::vector<double> Bprev;                               // k x k, W_{j-1} = Q_j B_{j-1}
+    std::vector<std::vector<double>> Ablk, Bblk;             // diagonal and sub-diagonal blocks of T
+    int solves = 0, iters = 0, j = 0;
+    double theta = 0;
+    std::vector<double> ritz;                                // top Ritz vector coefficients
+    for (j = 0; j < max_blocks; j++) {
+        std::vector<std::vector<double>> W(k);
+        for (int c = 0; c < k; c++) {
+            if (std::all_of(blk[c].begin(), blk[c].end(), [](double x) { return x == 0; })) {
+                W[c].assign(m, 0.0);
+                continue;
+            }
+            worst_apply(H, blk[c], W[c], solve_tol, mode, pre, post, &iters);
+            project(W[c]);
+            solves++;
+        }
+        // A_j = Q_j^T W (symmetrised)
+        std::vector<double> A((size_t)k * k);
+        for (int a = 0; a < k; a++)
+            for (int b = 0; b < k; b++) {
+                double s2 = 0;
+                for (size_t i = 0; i < m; i++) s2 += blk[a][i] * W[b][i];
+                A[(size_t)a * k + b] = s2;
+            }
+        for (int a = 0; a < k; a++)
+            for (int b = a + 1; b < k; b++) A[(size_t)a * k + b] = A[(size_t)b * k + a] =
+                0.5 * (A[(size_t)a * k + b] + A[(size_t)b * k + a]);
+        Ablk.push_back(A);
+        for (const auto& v : blk) Q.push_back(v);
+        // W -= Q_j A_j + Q_{j-1} B_{j-1}^T, then full reorthogonalisation against Q
+        for (int b = 0; b < k; b++) {
+            for (int a = 0; a < k; a++) {
+                const double c = A[(size_t)a * k + b];
+                for (size_t i = 0; i < m; i++) W[b][i] -= c * blk[a][i];
+            }
+            if (!prev.empty())
+                for (int a = 0; a < k; a++) {
+                    const double c = Bprev[(size_t)b * k + a];    // (B_{j-1}^T)[a][b]
+                    for (size_t i = 0; i < m; i++) W[b][i] -= c * prev[a][i];
+                }
+        }
+        std::vector<double> B;
+        r = block_orth(W, Q, B);                                  // W = Q_{j+1} B_j
+        Bblk.push_back(B);
+        // T: block tridiagonal, (j+1) k square
+        const int nt = (j + 1) * k;
+        std::vector<double> T((size_t)nt * nt, 0.0);
+        for (int i = 0; i <= j; i++) {
+            for (int a = 0; a < k; a++)
+                for (int b = 0; b < k; b++) T[(size_t)(i * k + a) * nt + i * k + b] = Ablk[i][(size_t)a * k + b];
+            if (i < j)
+                for (int a = 0; a < k; a++)
+                    for (int b = 0; b < k; b++) {
+                        const double v = Bblk[i][(size_t)a * k + b];   // block (i+1, i) = B_i
+                        T[(size_t)((i + 1) * k + a) * nt + i * k + b] = v;
+                        T[(size_t)(i * k + b) * nt + (i + 1) * k + a] = v;
+                    }
+        }
+        std::vector<double> w, V;
+        jacobi_eigen(nt, T, w, V);
+        int top = 0;
+        for (int i = 1; i < nt; i++) if (w[i] > w[top]) top = i;
+        theta = w[top];
+        ritz.resize(nt);
+        for (int i = 0; i < nt; i++) ritz[i] = V[(size_t)i * nt + top];
+        // residual of the top Ritz pair: |B_j y_last|
+        double res = 0;
+        for (int a = 0; a < k; a++) {
+            double s2 = 0;
+            for (int b = 0; b < k; b++) s2 += B[(size_t)a * k + b] * ritz[(size_t)j * k + b];
+            res += s2 * s2;
+        }
+        res = std::sqrt(res);
+        prev = blk;
+        Bprev = B;
+        blk = W;
+        if (res <= tol * std::fabs(theta) || r == 0 || Q.size() + k > dim) { j++; break; }
+    }
+    // Ritz load, unit length, then its displacement
+    std::vector<double> f(m, 0.0);
+    for (size_t i = 0; i < ritz.size(); i++)
+        for (size_t t = 0; t < m; t++) f[t] += ritz[i] * Q[i][t];
+    double fn = 0;
+    for (double x : f) fn += x * x;
+    fn = std::sqrt(fn);
+    for (double& x : f) x /= fn;
+    std::vector<double> u;
+    worst_apply(H, f, u, solve_tol, mode, pre, post, &iters);
+    solves++;
+    if (blocks_out) *blocks_out = j;
+    if (solves_out) *solves_out = solves;
+    if (iters_out) *iters_out = iters;
+    return dot<double, double>(H, H.f64.data(), H.x64.data(), (size_t)3 * H.lv[0].g.N);
+}
+
 extern "C" {
 
 int rt_set_load_nodes(rt_hier* h, const uint8_t* load_box, const double* normals_box) {
@@ -1684,6 +1938,22 @@ int rt_modified_pm(rt_hier* h, int max_itn, double* worst_compliance, int* vcycl
         int nv = 0;
         *worst_compliance = H.vprec == 64 ? modified_pm<double>(H, max_itn, &nv) : modified_pm<float>(H, max_itn, &nv);
         if (vcycles) *vcycles = nv;
+    })
+}
+
+int rt_block_lanczos(rt_hier* h, int k, int max_blocks, double tol, double solve_tol, int mode, int pre, int post,
+                     uint64_t seed, double* worst_compliance, rt_lanczos_stats* stats) {
+    RT_TRY({
+        need(worst_compliance);
+        Hier& H = H_(h);
+        int blocks = 0, solves = 0, iters = 0;
+        *worst_compliance = block_lanczos(H, k, max_blocks, tol, solve_tol, mode, pre, post, seed, &blocks, &solves,
+                                          &iters);
+        if (stats) {
+            stats->blocks = blocks;
+            stats->solves = solves;
+            stats->pcg_iterations = iters;
+        }
     })
 }
 

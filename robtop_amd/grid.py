@@ -327,6 +327,19 @@ class HierarchyGrid:
         check(lib().rt_modified_pm(self.handle, max_itn, ctypes.byref(c), ctypes.byref(nv)))
         return c.value, nv.value
 
+    def block_lanczos(self, k=8, max_blocks=30, tol=1e-6, solve_tol=1e-8, mode=None, pre=2, post=2, seed=0):
+        """Top eigenpair of the compliance operator on the admissible loads by
+        block Lanczos (no reference counterpart; same answer as modified_pm).
+        Leaves the unit worst load in the force and its displacement in the
+        solution; returns (worst compliance, {blocks, solves, pcg_iterations})."""
+        if mode is None:
+            mode = _lib.RT_SOLVE_RELIABLE
+        c = ctypes.c_double()
+        st = _lib.LanczosStats()
+        check(lib().rt_block_lanczos(self.handle, k, max_blocks, tol, solve_tol, mode, pre, post, seed,
+                                     ctypes.byref(c), ctypes.byref(st)))
+        return c.value, {"blocks": st.blocks, "solves": st.solves, "pcg_iterations": st.pcg_iterations}
+
     def synchronize(self):
         check(lib().rt_synchronize(self.handle))
 

@@ -180,12 +180,19 @@ class MMA:
         return dict(lambda_=lam, z=z.value, sub_iters=it.value, overflowed=bool(ov.value))
 
 
-def worst_case_topopt(hg, params, n_iter=30, max_pm=500, seed=0, init_rho=None, log=None, stop=True):
+def worst_case_topopt(hg, params, n_iter=30, max_pm=500, seed=0, init_rho=None, log=None, stop=True,
+                      method="pm", lanczos=None):
     """The worst-case SIMP loop of optimization() (optimization.cpp:583-673):
     Vgoal schedule, update_stencil, modified power method for the worst load
     and its compliance, stop test (converge_criteria(1, 2, 5e-3)), sensitivity
     of the worst compliance, filter and OC update. Load nodes and normals must
-    be set (HierarchyGrid.set_load_nodes). Returns per-iteration records."""
+    be set (HierarchyGrid.set_load_nodes). Returns per-iteration records.
+
+    method="lanczos" finds the worst load by block Lanczos instead
+    (HierarchyGrid.block_lanczos, keyword arguments in `lanczos`), which
+    converges to the same eigenpair with exact solves."""
+    if method not in ("pm", "lanczos"):
+        raise ValueError("method must be 'pm' or 'lanczos'")
     hg.init_rho(params.volume_ratio if init_rho is None else init_rho)
     vgoal = params.volume_ratio
     stop_check = converge_criteria(1, 2, 5e-3)
@@ -197,12 +204,16 @@ def worst_case_topopt(hg, params, n_iter=30, max_pm=500, seed=0, init_rho=None, 
         if vgoal < params.volume_ratio:
             vgoal = params.volume_ratio
         hg.update_stencil()
-        hg.rand_force(seed + itn)
-        c, nv = hg.modified_pm(max_pm)
+        if method == "pm":
+            hg.rand_force(seed + itn)
+            c, nv = hg.modified_pm(max_pm)
+            extra = dict(vcycles=nv)
+        else:
+            c, extra = hg.block_lanczos(seed=seed + itn, **(lanczos or {}))
         if math.isnan(c):
             raise FloatingPointError(f"worst compliance NaN at iteration {itn}")
         done = stop_check.update(c, [vc]) and vgoal <= params.volume_ratio + 1e-3
-        rec = dict(iter=itn, worst_compliance=c, vgoal=vgoal, vcycles=nv)
+        rec = dict(iter=itn, worst_compliance=c, vgoal=vgoal, **extra)
         if not (stop and done):
             computeSensitivity(hg, params.filter_radius)
             rec["multiplier"] = updateDensities(hg, vgoal, params)

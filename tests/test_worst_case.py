@@ -71,17 +71,15 @@ def test_force_project_matches_tangent_basis(prob):
         assert rel_err(got, ref) < 1e-13
 
 
-@pytest.mark.parametrize("mode", ["wsff", "wscf"])
-def test_modified_pm_finds_top_eigenpair(prob, mode):
+def _dense_top(prob, normals, rhop_used=None):
+    """Largest eigenpair of the aggregated compliance C = N^T K^-1 N, with K
+    assembled by brute force and factored by scipy (test_utils.cpp:1249-1296
+    is the reference's own form of this check). Returns (lambda_max, the top
+    load as an interleaved-dof vector)."""
     a, H, fixed, rho, rhop, load, nrm, f0 = prob
+    if rhop_used is not None:
+        rhop = rhop_used
     nn = H.levels[0]["nn"]
-    normals = nrm if mode == "wscf" else None
-    H.update_stencil()
-    H.set_load_nodes(load.astype(np.uint8), normals)
-    H.set_vec(0, 1, f0)
-    c, steps = H.modified_pm(500)
-    assert 1 < steps < 500
-
     P = bf.Problem(a, bf.from_flat(rho, H.levels[0]["e"]), bf.from_flat(fixed.astype(bool), H.levels[0]["n"]),
                    rhop=bf.from_flat(rhop, H.levels[0]["e"]))
     act = np.nonzero(P.active[0] & ~np.repeat(fixed.astype(bool), 3))[0]
@@ -97,9 +95,21 @@ def test_modified_pm_finds_top_eigenpair(prob, mode):
     KN = np.zeros_like(N)
     KN[act] = lu.solve(N[act])
     w, V = np.linalg.eigh(N.T @ KN)
-    assert abs(c - w[-1]) / w[-1] < 5e-4
+    return w[-1], N @ V[:, -1]
+
+
+@pytest.mark.parametrize("mode", ["wsff", "wscf"])
+def test_modified_pm_finds_top_eigenpair(prob, mode):
+    a, H, fixed, rho, rhop, load, nrm, f0 = prob
+    normals = nrm if mode == "wscf" else None
+    H.update_stencil()
+    H.set_load_nodes(load.astype(np.uint8), normals)
+    H.set_vec(0, 1, f0)
+    c, steps = H.modified_pm(500)
+    assert 1 < steps < 500
+    lam, ftop = _dense_top(prob, normals)
+    assert abs(c - lam) / lam < 5e-4
     f = H.get_vec(0, 1).T.reshape(-1)            # interleaved dofs
-    ftop = N @ V[:, -1]
     assert abs(abs(f @ ftop) - 1) < 1e-3
     assert abs(np.linalg.norm(f) - 1) < 1e-12
 
@@ -195,4 +205,65 @@ def test_gpu_torus_worst_case_loop():
         st = g.solve(tol=1e-8, maxit=500, mode=rt.RT_SOLVE_MIXED, pre=2, post=2)
         assert st.rel_res < 1e-7
         assert g.compliance() <= cw * (1 + 2e-3)
+    g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["wsff", "wscf"])
+def test_gpu_block_lanczos_top_eigenpair(prob, mode):
+    """Block Lanczos (k = 8, no reference counterpart) reaches the largest
+    eigenvalue of the dense C to 1e-7 with FP64 solves and 1e-5 with the
+    reliable-update FP32 solver, returns its eigenvector as a unit load, and
+    agrees with the modified power method within the latter's stopping rule."""
+    import robtop_amd as rt
+    a, H, fixed, rho, rhop, load, nrm, f0 = prob
+    normals = nrm if mode == "wscf" else None
+    print(f"[lanczos] holes3 {mode}", flush=True)
+    g = rt.HierarchyGrid.genFromMesh(bits_of(a), 32)
+    g.set_material(1e6, 0.4, 3.0)
+    g.set_support(fixed)
+    g.set_density(rho)
+    g.update_stencil()
+    g.set_load_nodes(load.astype(np.uint8), normals)
+    lam, ftop = _dense_top(prob, normals, g.get_rhop())     # the device's rho^p (FP32 powf)
+    c, st = g.block_lanczos(k=8, max_blocks=40, tol=1e-9, solve_tol=1e-11, mode=rt.RT_SOLVE_FP64, seed=3)
+    assert abs(c - lam) / lam < 1e-7, (c, lam, st)
+    assert 1 < st["solves"] <= 8 * st["blocks"] + 1
+    f = g.get_force().T.reshape(-1)
+    assert abs(np.linalg.norm(f) - 1) < 1e-12
+    assert abs(abs(f @ ftop) - 1) < 1e-6
+    c32, _ = g.block_lanczos(k=8, max_blocks=40, tol=1e-7, solve_tol=1e-8, seed=4)
+    assert abs(c32 - lam) / lam < 1e-5, (c32, lam)
+    g.rand_force(7)
+    cpm, _ = g.modified_pm(500)
+    assert abs(cpm - lam) / lam < 5e-4
+    g.close()
+
+
+@pytest.mark.gpu
+def test_gpu_torus_lanczos_matches_pm():
+    """On the voxelized torus with normal-direction loads, the worst-case loop
+    runs with block Lanczos, and at the final densities block Lanczos and the
+    modified power method find the same worst compliance (the power method to
+    its stopping rule)."""
+    import robtop_amd as rt
+    from robtop_amd.workloads import torus_bits, torus_worst_case_bc
+    n = 48
+    print(f"[lanczos] torus n={n}", flush=True)
+    bits, R, solid = torus_bits(n)
+    g = rt.HierarchyGrid.genFromMesh(bits, R)
+    g.set_material(1e6, 0.4, 3.0)
+    fixed, load, normals = torus_worst_case_bc(g, n)
+    g.set_support(fixed)
+    g.set_load_nodes(load.astype(np.uint8), normals)
+    g.set_precision(32)
+    params = rt.Parameter(volume_ratio=0.3, volume_decrease=0.05)
+    lz = dict(k=8, max_blocks=12, tol=1e-6, solve_tol=1e-8)
+    recs = rt.worst_case_topopt(g, params, n_iter=2, stop=False, method="lanczos", lanczos=lz)
+    assert len(recs) == 2 and all(r["worst_compliance"] > 0 and r["solves"] > 8 for r in recs)
+    g.update_stencil()
+    cl, st = g.block_lanczos(seed=11, **lz)
+    g.rand_force(5)
+    cp, _ = g.modified_pm(500)
+    assert abs(cl - cp) / cl < 2e-3, (cl, cp, st)
     g.close()
