@@ -27,11 +27,17 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", default="511,256,256")
+    ap.add_argument("--workload", default="cantilever", choices=["cantilever", "torus"],
+                    help="cantilever: the headline s/SIMP-iteration run; torus: the worst-case loop on a "
+                         "voxelized torus with block Lanczos (its own JSON line, not the headline)")
+    ap.add_argument("--torus-n", type=int, default=384, help="torus voxels per side")
     ap.add_argument("--pre", type=int, default=2)
     ap.add_argument("--post", type=int, default=2)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--coarse-mult", type=int, default=1, help="sweep multiplier on coarse levels (preconditioner)")
     ap.add_argument("--gamma", type=int, default=1, help="preconditioner cycle index: 1 = V, 2 = W")
+    ap.add_argument("--coarse-bf16", action="store_true",
+                    help="store the coarse-level stencils of the FP32 preconditioner in bfloat16")
     ap.add_argument("--mode", default="reliable", choices=["mixed", "reliable", "fp64"],
                     help="MG-PCG variant: FP64 Krylov vectors (mixed), FP32 vectors with FP64 residual "
                          "replacement (reliable), or all FP64")
@@ -51,10 +57,10 @@ def parse():
     return ap.parse_args()
 
 
-def algorithmic_vcycle_bytes(levels, pre, post, coarsest_dofs, P=4, mult=1, gamma=1):
+def algorithmic_vcycle_bytes(levels, pre, post, coarsest_dofs, P=4, mult=1, gamma=1, Ps=4):
     """Compulsory bytes of one preconditioner cycle: each array element an
-    operation must read or write, counted once per operation; u/f/r in FP32,
-    stencils in FP32, 1 flag byte per node, rho^p 4 B per element. Coarse
+    operation must read or write, counted once per operation; u/f/r in FP32 (P),
+    stencils Ps bytes each (FP32 or bfloat16), 1 flag byte per node, rho^p 4 B per element. Coarse
     levels smooth `mult` times pre/post sweeps and level i > 0 is visited
     gamma^(i-1) times (the coarsest once per visit of its parent)."""
     n0, e0 = levels[0]["nodes"], levels[0]["elements"]
@@ -69,7 +75,7 @@ def algorithmic_vcycle_bytes(levels, pre, post, coarsest_dofs, P=4, mult=1, gamm
         per_visit += 3 * P * nc + nf * (6 * P + 1)     # prolong (read u_c, read+write u_f, flag)
         if i < nlev - 1:
             nl = levels[i]["nodes"]
-            per = nl * (243 * P + 9 * P + 1)
+            per = nl * (243 * Ps + 9 * P + 1)
             per_visit += mult * (pre + post) * per + per   # sweeps + residual
         else:
             per_visit += coarsest_dofs ** 2 * 8
@@ -197,6 +203,56 @@ def pmc_summary(fetch_csv, write_csv, size):
             "algorithmic_bytes_per_launch": alg, "traffic_over_algorithmic": (fetch + write) / alg}
 
 
+def torus_run(a):
+    """Worst-case SIMP iterations on an analytic torus (R = 0.35, r = 0.12 of the
+    box edge) voxelized at n^3: supports on the bottom 5 % of its height, loads
+    along the outward normals of the other surface nodes, the worst load found
+    by block Lanczos (k = 8, FP32-preconditioned reliable-update solves). After
+    the timed iterations the worst compliance is recomputed with FP64 solves at
+    the same densities and the relative difference reported."""
+    import robtop_amd as rt
+    from robtop_amd.workloads import torus_bits, torus_worst_case_bc
+    n = a.torus_n
+    log(f"torus {n}^3")
+    bits, R, solid = torus_bits(n)
+    hg = rt.HierarchyGrid.genFromMesh(bits, R)
+    hg.set_material(1e6, 0.4, 3.0)
+    fixed, load, normals = torus_worst_case_bc(hg, n)
+    hg.set_support(fixed)
+    nl = hg.set_load_nodes(load.astype(np.uint8), normals)
+    hg.set_precision(32)
+    hg.set_preconditioner(a.coarse_mult, a.gamma)
+    log(f"hierarchy: {hg.n_levels} levels, {nl} load nodes")
+    params = rt.Parameter(volume_ratio=0.3, volume_decrease=0.05)
+    lz = dict(k=8, max_blocks=20, tol=1e-6, solve_tol=1e-8, pre=a.pre, post=a.post)
+    recs = rt.worst_case_topopt(hg, params, n_iter=a.warmup + a.steps, stop=False, method="lanczos", lanczos=lz,
+                                log=lambda r: log(f"iter {r['iter']}: c {r['worst_compliance']:.6e}, "
+                                                  f"{r['blocks']} blocks, {r['solves']} solves, "
+                                                  f"{r['pcg_iterations']} PCG iterations, {r['seconds']:.2f} s"))
+    timed = recs[a.warmup:]
+    s_iter = float(np.mean([r["seconds"] for r in timed]))
+    hg.update_stencil()
+    c32, st32 = hg.block_lanczos(seed=99, **lz)
+    lz64 = dict(lz, tol=1e-8, solve_tol=1e-11, mode=rt.RT_SOLVE_FP64)
+    c64, st64 = hg.block_lanczos(seed=99, **lz64)
+    ne, nn = hg.num_active(0)
+    out = {
+        "metric": f"s/worst-case SIMP iter, torus {n}^3, block Lanczos k=8",
+        "value": s_iter, "unit": "s/iter", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": s_iter * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"torus R=0.35 r=0.12 voxelized at {n}^3, supports on the bottom 5 %, normal loads on "
+                               f"{nl} surface nodes, block Lanczos k=8 (tol 1e-6, solves to 1e-8), filter r=2, OC",
+                   "active_elements": ne, "active_nodes": nn, "levels": hg.n_levels,
+                   "lanczos_per_iter": [{k: r[k] for k in ("blocks", "solves", "pcg_iterations")} for r in timed]},
+        "eigen_check": {"worst_compliance_fp32_solves": c32, "worst_compliance_fp64_solves": c64,
+                        "rel_diff": abs(c32 - c64) / abs(c64), "fp64_blocks": st64["blocks"],
+                        "fp64_solves": st64["solves"]},
+    }
+    hg.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.pmc_summary:
@@ -208,12 +264,17 @@ def main():
     if a.profile:
         profile_run(a)
         return
+    if a.workload == "torus":
+        torus_run(a)
+        return
     ex, ey, ez = (int(v) for v in a.size.split(","))
     import robtop_amd as rt
     log(f"building {ex}x{ey}x{ez}")
     hg, bits, R, fixed, f, rho = build_problem(ex, ey, ez, a.seed)
     log(f"hierarchy: {hg.n_levels} levels")
     hg.set_preconditioner(a.coarse_mult, a.gamma)
+    if a.coarse_bf16:
+        hg.set_coarse_stencil_bits(16)
     hg.set_graphs(not a.no_graphs)
     g0 = hg[0]
     ne0, nn0 = hg.num_active(0)
@@ -267,7 +328,8 @@ def main():
     hg.update_stencil()
     t_v = hg.time_vcycle(32, a.pre, a.post, a.vcycles)
     t_sweep = hg.time_fine_sweep(32, 10)
-    B_V = algorithmic_vcycle_bytes(levels, a.pre, a.post, coarsest_dofs, mult=a.coarse_mult, gamma=a.gamma)
+    B_V = algorithmic_vcycle_bytes(levels, a.pre, a.post, coarsest_dofs, mult=a.coarse_mult, gamma=a.gamma,
+                                   Ps=2 if a.coarse_bf16 else 4)
     # a sweep is four launches, each relaxing two colours (half the rows of the box
     # in y-z parity class, both x parities); each launch moves a quarter of the
     # sweep's compulsory bytes: u (read + write), f, flag for its nodes and, over the
@@ -307,7 +369,8 @@ def main():
         },
         "vcycle": {"ms": t_v, "algorithmic_bytes": B_V, "GBps": B_V / (t_v * 1e-3) / 1e9,
                    "frac_of_peak": B_V / (t_v * 1e-3) / 1e9 / HBM_PEAK_GBS, "pre": a.pre, "post": a.post,
-                   "coarse_mult": a.coarse_mult, "gamma": a.gamma},
+                   "coarse_mult": a.coarse_mult, "gamma": a.gamma,
+                   "coarse_stencil_bits": 16 if a.coarse_bf16 else 32},
         "roofline": {"kernel": "k_fine_gs_pair (one colour-pair launch of the finest-level smoother)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": launch_bytes, "launch_ms": launch_ms},

@@ -167,6 +167,13 @@ int rt_vcycle(rt_hier* h, int pre, int post, double* rel_res);
 
 /* --- solve and SIMP step ------------------------------------------------- */
 
+/* Storage of the coarse-level stencils read by FP32 cycles: 32 (FP32, the
+ * default) or 16 (bfloat16, rounded from the FP64 Galerkin stencils; halves
+ * the coarse levels' stencil traffic). Used by the preconditioner and every
+ * FP32 V-cycle; FP64 cycles and the coarsest factorisation keep FP64. No
+ * counterpart in the reference (its stencils are FP64). */
+int rt_set_coarse_stencil_bits(rt_hier* h, int bits);
+
 /* Shape of the preconditioner cycle of rt_solve (and rt_time_vcycle): levels
  * below the finest smooth coarse_sweep_mult times pre/post sweeps, and each
  * coarse level is visited gamma times (1 = V-cycle, 2 = W-cycle). Default

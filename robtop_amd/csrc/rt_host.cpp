@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -83,6 +84,7 @@ struct Level {
     LevelVecs<double> v64;
     LevelVecs<float> v32;
     DBuf st64, st32, KEc;
+    DBuf st16;                            // bfloat16 copy for FP32 preconditioner cycles (optional)
     size_t nn_box() const { return (size_t)g.n[0] * g.n[1] * g.n[2]; }
     size_t ne_box() const { return (size_t)e[0] * e[1] * e[2]; }
 };
@@ -123,6 +125,7 @@ struct Hier {
     bool load_normals = false;
     int vprec = 64;
     int pc_mult = 1, pc_gamma = 1;   // preconditioner cycle: coarse-level sweep multiplier, cycle index
+    int coarse_bits = 32;            // stencil storage of FP32 coarse-level cycles: 32 or 16 (bfloat16)
     // the preconditioner cycle replayed as a HIP graph (pc_key identifies its shape)
     bool use_graphs = true;
     hipGraphExec_t pc_graph = nullptr;
@@ -414,9 +417,17 @@ static void gs(Hier& H, int li, Vec3<T> U, Vec3<const T> F, int n, bool reverse,
                 launch_fine_gs<T>(L.g, U, F, H.rhop.as<float>(), L.vflag.as<uint8_t>(), reverse ? 3 - pp : pp,
                                   reverse, zero_support, H.s);
         } else {
-            for (int cc = 0; cc < 8; cc++)
-                launch_coarse_gs<T, T>(L.g, U, F, stencil<T>(L), L.vflag.as<uint8_t>(), reverse ? 7 - cc : cc,
-                                       reverse, H.s);
+            for (int cc = 0; cc < 8; cc++) {
+                const int c = reverse ? 7 - cc : cc;
+                if constexpr (std::is_same<T, float>::value) {
+                    if (H.coarse_bits == 16) {
+                        launch_coarse_gs<float, bf16>(L.g, U, F, L.st16.as<bf16>(), L.vflag.as<uint8_t>(), c, reverse,
+                                                      H.s);
+                        continue;
+                    }
+                }
+                launch_coarse_gs<T, T>(L.g, U, F, stencil<T>(L), L.vflag.as<uint8_t>(), c, reverse, H.s);
+            }
         }
     }
 }
@@ -426,8 +437,15 @@ static void residual(Hier& H, int li, Vec3<const T> U, Vec3<const T> F, Vec3<T> 
     Level& L = H.lv[li];
     if (li == 0)
         launch_fine_residual_modal<T>(L.g, U, F, R, H.rhop.as<float>(), L.vflag.as<uint8_t>(), H.s);
-    else
+    else {
+        if constexpr (std::is_same<T, float>::value) {
+            if (H.coarse_bits == 16) {
+                launch_coarse_residual<float, bf16>(L.g, U, F, R, L.st16.as<bf16>(), L.vflag.as<uint8_t>(), H.s);
+                return;
+            }
+        }
         launch_coarse_residual<T, T>(L.g, U, F, R, stencil<T>(L), L.vflag.as<uint8_t>(), H.s);
+    }
 }
 
 template <typename T>
@@ -637,6 +655,10 @@ static void update_stencil(Hier& H) {
         Level& L = H.lv[li];
         launch_assemble<double>(L.g, L.KEc.as<double>(), L.e, L.vflag.as<uint8_t>(), L.st64.as<double>(), H.s);
         launch_convert<double, float>(L.st64.as<double>(), L.st32.as<float>(), (size_t)243 * L.g.N, H.s);
+        if (H.coarse_bits == 16) {
+            L.st16.ensure(sizeof(uint16_t) * 243 * (size_t)L.g.N);
+            launch_to_bf16(L.st64.as<double>(), L.st16.as<bf16>(), (size_t)243 * L.g.N, H.s);
+        }
     }
     factor_coarsest(H);
     H.stencil_ready = true;
@@ -1326,6 +1348,23 @@ int rt_set_graphs(rt_hier* h, int on) {
         Hier& H = H_(h);
         H.drop_graph();
         H.use_graphs = on != 0;
+    })
+}
+
+int rt_set_coarse_stencil_bits(rt_hier* h, int bits) {
+    RT_TRY({
+        Hier& H = H_(h);
+        if (bits != 16 && bits != 32) throw std::invalid_argument("coarse stencil bits must be 16 or 32");
+        if (bits == H.coarse_bits) return RT_OK;
+        H.drop_graph();
+        H.coarse_bits = bits;
+        if (bits == 16 && H.stencil_ready)
+            for (size_t li = 1; li < H.lv.size(); li++) {
+                Level& L = H.lv[li];
+                L.st16.ensure(sizeof(uint16_t) * 243 * (size_t)L.g.N);
+                launch_to_bf16(L.st64.as<double>(), L.st16.as<bf16>(), (size_t)243 * L.g.N, H.s);
+            }
+        RT_HIP(hipStreamSynchronize(H.s));
     })
 }
 

@@ -83,6 +83,15 @@ template <>
 __device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
+// bfloat16 storage (upper half of an IEEE binary32); widened on load
+struct bf16 {
+    uint16_t v;
+    __host__ __device__ operator float() const { return __builtin_bit_cast(float, (uint32_t)v << 16); }
+};
+template <>
+__device__ __forceinline__ bf16 bload<bf16>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return bf16{(uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0)};
+}
 __device__ __forceinline__ uint8_t bload_u8(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
     return __builtin_amdgcn_raw_buffer_load_b8(r, voff, soff, 0);
 }
@@ -129,6 +138,8 @@ template <typename T>
 void launch_coarsest(const double* Ainv, int n3, const int* rowslot, int unused, Vec3<const T> F, Vec3<T> U,
                      double* fbuf, hipStream_t s);
 void launch_mirror_upper(double* A, int n, hipStream_t s);
+// b = a rounded to bfloat16 (round to nearest even, through binary32)
+void launch_to_bf16(const double* a, bf16* b, size_t n, hipStream_t s);
 template <typename TA, typename TB>
 void launch_convert(const TA* a, TB* b, size_t n, hipStream_t s);
 template <typename T>

@@ -243,6 +243,10 @@ class HierarchyGrid:
         check(lib().rt_vcycle(self.handle, pre_relax, post_relax, ctypes.byref(r)))
         return r.value
 
+    def set_coarse_stencil_bits(self, bits):
+        """Coarse-level stencil storage of FP32 cycles: 32 (FP32) or 16 (bfloat16)."""
+        check(lib().rt_set_coarse_stencil_bits(self.handle, bits))
+
     def set_preconditioner(self, coarse_sweep_mult=1, gamma=1):
         """Cycle shape of the MG preconditioner used by solve()."""
         check(lib().rt_set_preconditioner(self.handle, coarse_sweep_mult, gamma))
