@@ -286,6 +286,220 @@ __global__ void __launch_bounds__(256) RT_GS_ATTR k_fine_gs_pair(Geom g, Vec3<T>
     }
 }
 
+// ---------------------------------------------------------------------------
+// Tiled sweep: two launches per sweep instead of four, reading the old values
+// from Uin and writing the new ones to Uout (a second buffer).
+//
+// In the reference's colour order all colours of the z-parity class relaxed
+// first (z even forward, z odd in reverse) come before any colour of the other
+// class, and planes of one class never neighbour each other. So a launch
+// relaxes the four colours of one class, plane by plane, and a workgroup owns a
+// tile of TR whole rows (all x) of one plane. Inside the tile the four colours
+// follow the sweep order with a barrier between them; a neighbour in the same
+// plane is new (read from Uout, written earlier by this workgroup) if its
+// colour comes earlier, old (read from Uin) otherwise; a neighbour in the other
+// z class is old in the first launch and new (Uout, written by the first
+// launch) in the second. The second pair of colours of the plane (the "lag"
+// rows, y parity relaxed last) needs the new values of the first pair on the
+// rows on either side, so the first pair is also relaxed on the one row of the
+// next tile it needs (below the tile in reverse order, above it forward). That
+// row is computed from the same values and with the same arithmetic as its
+// owner computes it, so both write identical bits. Uin is never written, so
+// the result does not depend on the order in which workgroups run, and each
+// node gets exactly the value of the colour-pair sweep above.
+// ---------------------------------------------------------------------------
+struct GsBufs2 {
+    __amdgpu_buffer_rsrc_t in0, in1, in2, out0, out1, out2, rho, flag;
+};
+
+// whether the neighbour at (DX, DY, DZ) of a colour-C node already holds its
+// new value when the node is relaxed (FIRST: this launch relaxes the first z
+// class of the sweep; REV: reverse colour order)
+template <int C, int DX, int DY, int DZ, bool FIRST, bool REV>
+constexpr bool nb_is_new() {
+    if constexpr (DZ != 0) {
+        return !FIRST;
+    } else {
+        constexpr int cn = C ^ ((DX & 1) | ((DY & 1) << 1));
+        constexpr int o = ((C >> 1) & 1) * 2 + (C & 1), on = ((cn >> 1) & 1) * 2 + (cn & 1);
+        return REV ? (on > o) : (on < o);
+    }
+}
+
+template <typename T, int C, bool CHECK_FIX, int NB, bool FIRST, bool REV>
+__device__ __forceinline__ void nb_step2(const Geom& g, int vb, const KeC<T>& K, const GsBufs2& B, T (&acc)[8][3]) {
+    if constexpr (NB != 13) {
+        constexpr int DX = NB % 3 - 1, DY = NB / 3 % 3 - 1, DZ = NB / 9 - 1;
+        constexpr bool NW = nb_is_new<C, DX, DY, DZ, FIRST, REV>();
+        const int so = rel_off<C, DX, DY, DZ>(g);
+        const int vo = vb * (int)sizeof(T), sob = so * (int)sizeof(T);
+        T u0 = bload<T>(NW ? B.out0 : B.in0, vo, sob);
+        T u1 = bload<T>(NW ? B.out1 : B.in1, vo, sob);
+        T u2 = bload<T>(NW ? B.out2 : B.in2, vo, sob);
+        if constexpr (CHECK_FIX) {
+            if (bload_u8(B.flag, vb, so) & NF_FIXED) { u0 = T(0); u1 = T(0); u2 = T(0); }
+        }
+        nb_elems<T, NB>(acc, K, u0, u1, u2, std::make_integer_sequence<int, 8>{});
+    }
+}
+
+template <typename T, int C, bool CHECK_FIX, int ROW, bool FIRST, bool REV>
+__device__ __forceinline__ void nb_row2(const Geom& g, int vb, const KeC<T>& K, const GsBufs2& B, T (&acc)[8][3]) {
+    constexpr int DY = ROW % 3 - 1, DZ = ROW / 3 - 1;
+    constexpr int NBM = 9 * (DZ + 1) + 3 * (DY + 1);
+    constexpr bool NW = nb_is_new<C, -1, DY, DZ, FIRST, REV>();   // dx = -1 and +1 share a colour
+    const int vo = vb * (int)sizeof(T);
+    {
+        const int so = rel_off<C, -1, DY, DZ>(g), sob = so * (int)sizeof(T);
+        const Two<T> a = bload2<T>(NW ? B.out0 : B.in0, vo, sob), b = bload2<T>(NW ? B.out1 : B.in1, vo, sob),
+                     c = bload2<T>(NW ? B.out2 : B.in2, vo, sob);
+        T l0 = a.lo, l1 = b.lo, l2 = c.lo, r0 = a.hi, r1 = b.hi, r2 = c.hi;
+        if constexpr (CHECK_FIX) {
+            if (bload_u8(B.flag, vb, so) & NF_FIXED) { l0 = T(0); l1 = T(0); l2 = T(0); }
+            if (bload_u8(B.flag, vb, so + 1) & NF_FIXED) { r0 = T(0); r1 = T(0); r2 = T(0); }
+        }
+        nb_elems<T, NBM>(acc, K, l0, l1, l2, std::make_integer_sequence<int, 8>{});
+        nb_elems<T, NBM + 2>(acc, K, r0, r1, r2, std::make_integer_sequence<int, 8>{});
+    }
+    nb_step2<T, C, CHECK_FIX, NBM + 1, FIRST, REV>(g, vb, K, B, acc);
+}
+
+template <typename T, int C, bool CHECK_FIX, bool FIRST, bool REV, int... ROWs>
+__device__ __forceinline__ void nb_all2(const Geom& g, int vb, const KeC<T>& K, const GsBufs2& B, T (&acc)[8][3],
+                                        std::integer_sequence<int, ROWs...>) {
+    (nb_row2<T, C, CHECK_FIX, ROWs, FIRST, REV>(g, vb, K, B, acc), ...);
+}
+
+template <typename T, int C, bool CHECK_FIX, bool FIRST, bool REV>
+__device__ __forceinline__ void gs_node2(const Geom& g, int base, const KeC<T>& K, const GsBufs2& B, T KU[3],
+                                         T S[9]) {
+    const int vb = base - g.h[0] * g.h[1] - g.h[0] - 1;
+    T acc[8][3];
+#pragma unroll
+    for (int w = 0; w < 8; w++) acc[w][0] = acc[w][1] = acc[w][2] = T(0);
+    nb_all2<T, C, CHECK_FIX, FIRST, REV>(g, vb, K, B, acc, std::make_integer_sequence<int, 9>{});
+    KU[0] = KU[1] = KU[2] = T(0);
+#pragma unroll
+    for (int i = 0; i < 9; i++) S[i] = T(0);
+    const GsBufs Be = {B.in0, B.in1, B.in2, B.rho, B.flag};
+    elem_all<T, C>(g, vb, K, Be, acc, KU, S, std::make_integer_sequence<int, 8>{});
+}
+
+// the GS update of gs_update, old values from Uin, the result to Uout
+template <typename T, int C, bool ZS, bool FIRST, bool REV>
+__device__ __forceinline__ void gs_update2(const Geom& g, int base, Vec3<const T> Uin, Vec3<T> Uout, Vec3<const T> F,
+                                           const uint8_t* __restrict__ vflag, const KeC<T>& K, const GsBufs2& B) {
+    const int v = C * g.cs + base;
+    const uint8_t fl = vflag[v];
+    if (!(fl & NF_ACTIVE)) return;
+    const T f[3] = {F.c[0][v], F.c[1][v], F.c[2][v]};
+    if (fl & NF_FIXED) {
+        const T ne = (T)(fl >> 4);
+        Uout.c[0][v] = f[0] / ne; Uout.c[1][v] = f[1] / ne; Uout.c[2][v] = f[2] / ne;
+        return;
+    }
+    T KU[3], S[9];
+    if (!ZS && (fl & NF_NEARFIX)) gs_node2<T, C, true, FIRST, REV>(g, base, K, B, KU, S);
+    else gs_node2<T, C, false, FIRST, REV>(g, base, K, B, KU, S);
+    T u[3] = {Uin.c[0][v], Uin.c[1][v], Uin.c[2][v]};
+    if (!REV) {
+        u[0] = (f[0] - S[1] * u[1] - S[2] * u[2] - KU[0]) / S[0];
+        u[1] = (f[1] - S[3] * u[0] - S[5] * u[2] - KU[1]) / S[4];
+        u[2] = (f[2] - S[6] * u[0] - S[7] * u[1] - KU[2]) / S[8];
+    } else {
+        u[2] = (f[2] - S[6] * u[0] - S[7] * u[1] - KU[2]) / S[8];
+        u[1] = (f[1] - S[3] * u[0] - S[5] * u[2] - KU[1]) / S[4];
+        u[0] = (f[0] - S[1] * u[1] - S[2] * u[2] - KU[0]) / S[0];
+    }
+    Uout.c[0][v] = u[0]; Uout.c[1][v] = u[1]; Uout.c[2][v] = u[2];
+}
+
+// colour-C nodes of the rows y in [ya, yb) (clipped to the box) of plane
+// half-index k, spread over the workgroup
+template <typename T, int C, bool ZS, bool FIRST, bool REV>
+__device__ __forceinline__ void tile_phase(const Geom& g, int k, int ya, int yb, Vec3<const T> Uin, Vec3<T> Uout,
+                                           Vec3<const T> F, const uint8_t* __restrict__ vflag, const KeC<T>& K,
+                                           const GsBufs2& B) {
+    constexpr int cx = C & 1, cy = (C >> 1) & 1;
+    const int nhx = (g.n[0] - cx + 1) >> 1, nhy = (g.n[1] - cy + 1) >> 1;
+    const int jlo = max(0, (ya - cy + 1) >> 1), jhi = min(nhy, (yb - cy + 1) >> 1);
+    const int cnt = (jhi - jlo) * nhx;
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+        const int j = jlo + t / nhx, i = t % nhx;
+        gs_update2<T, C, ZS, FIRST, REV>(g, ((k + 1) * g.h[1] + (j + 1)) * g.h[0] + (i + 1), Uin, Uout, F, vflag, K,
+                                         B);
+    }
+}
+
+#ifndef RT_GS_TILE_ROWS
+#define RT_GS_TILE_ROWS 16
+#endif
+
+template <typename T, int CZ, bool REV, bool ZS>
+__global__ void __launch_bounds__(256) k_fine_gs_tile(Geom g, Vec3<const T> Uin, Vec3<T> Uout, Vec3<const T> F,
+                                                      const float* __restrict__ rhop,
+                                                      const uint8_t* __restrict__ vflag, KeC<T> K, int nplanes) {
+    constexpr bool FIRST = CZ == (REV ? 1 : 0);
+    constexpr int TR = RT_GS_TILE_ROWS;
+    static_assert(TR % 2 == 0, "tiles start on even rows");
+    // planes fastest, so that the tiles sharing the other class's planes in
+    // between run close together on one XCD
+    const int b = xcd_row(blockIdx.x, gridDim.x);
+    const int k = b % nplanes, ty = b / nplanes;
+    const int ny = g.n[1];
+    const int y0 = ty * TR, y1 = min(y0 + TR, ny);
+    const int nbytes = g.N * (int)sizeof(T);
+    const GsBufs2 B = {rsrc(Uin.c[0], nbytes),  rsrc(Uin.c[1], nbytes),  rsrc(Uin.c[2], nbytes),
+                       rsrc(Uout.c[0], nbytes), rsrc(Uout.c[1], nbytes), rsrc(Uout.c[2], nbytes),
+                       rsrc(rhop, g.N * 4),     rsrc(vflag, g.N)};
+    // lead rows (the y parity relaxed first) with the one extra row the lag rows need
+    const int la = REV ? y0 - 1 : y0, lb = REV ? y1 : y1 + 1;
+    constexpr int Z = CZ << 2;
+    if constexpr (!REV) {
+        tile_phase<T, Z | 0, ZS, FIRST, REV>(g, k, la, lb, Uin, Uout, F, vflag, K, B);
+        __syncthreads();
+        tile_phase<T, Z | 1, ZS, FIRST, REV>(g, k, la, lb, Uin, Uout, F, vflag, K, B);
+        __syncthreads();
+        tile_phase<T, Z | 2, ZS, FIRST, REV>(g, k, y0, y1, Uin, Uout, F, vflag, K, B);
+        __syncthreads();
+        tile_phase<T, Z | 3, ZS, FIRST, REV>(g, k, y0, y1, Uin, Uout, F, vflag, K, B);
+    } else {
+        tile_phase<T, Z | 3, ZS, FIRST, REV>(g, k, la, lb, Uin, Uout, F, vflag, K, B);
+        __syncthreads();
+        tile_phase<T, Z | 2, ZS, FIRST, REV>(g, k, la, lb, Uin, Uout, F, vflag, K, B);
+        __syncthreads();
+        tile_phase<T, Z | 1, ZS, FIRST, REV>(g, k, y0, y1, Uin, Uout, F, vflag, K, B);
+        __syncthreads();
+        tile_phase<T, Z | 0, ZS, FIRST, REV>(g, k, y0, y1, Uin, Uout, F, vflag, K, B);
+    }
+}
+
+// one full sweep (colours 0..7, or 7..0 when reverse) from Uin to Uout
+template <typename T>
+void launch_fine_gs_sweep2(const Geom& g, Vec3<const T> Uin, Vec3<T> Uout, Vec3<const T> F, const float* rhop,
+                           const uint8_t* vflag, bool reverse, bool zero_support, hipStream_t s) {
+    const KeC<T> K = kec<T>();
+    const int ntile = (g.n[1] + RT_GS_TILE_ROWS - 1) / RT_GS_TILE_ROWS;
+    for (int pass = 0; pass < 2; pass++) {
+        const int cz = reverse ? 1 - pass : pass;
+        const int nplanes = (g.n[2] - cz + 1) >> 1;
+        if (nplanes <= 0) continue;
+        dim3 grid((unsigned)(nplanes * ntile));
+        switch (cz * 4 + (reverse ? 2 : 0) + (zero_support ? 1 : 0)) {
+#define RT_CASE(c, r, z) case c * 4 + r * 2 + z: hipLaunchKernelGGL((k_fine_gs_tile<T, c, (r) != 0, (z) != 0>), grid, dim3(256), 0, s, g, Uin, Uout, F, rhop, vflag, K, nplanes); break;
+            RT_CASE(0, 0, 0) RT_CASE(0, 0, 1) RT_CASE(0, 1, 0) RT_CASE(0, 1, 1)
+            RT_CASE(1, 0, 0) RT_CASE(1, 0, 1) RT_CASE(1, 1, 0) RT_CASE(1, 1, 1)
+#undef RT_CASE
+        }
+        RT_HIP(hipGetLastError());
+    }
+}
+
+template void launch_fine_gs_sweep2<float>(const Geom&, Vec3<const float>, Vec3<float>, Vec3<const float>,
+                                           const float*, const uint8_t*, bool, bool, hipStream_t);
+template void launch_fine_gs_sweep2<double>(const Geom&, Vec3<const double>, Vec3<double>, Vec3<const double>,
+                                            const float*, const uint8_t*, bool, bool, hipStream_t);
+
 // pair p = 0..3 relaxes colours (2p, 2p+1), or (2p+1, 2p) when reverse
 template <typename T>
 void launch_fine_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* rhop, const uint8_t* vflag, int pair,

@@ -126,6 +126,9 @@ struct Hier {
     int vprec = 64;
     int pc_mult = 1, pc_gamma = 1;   // preconditioner cycle: coarse-level sweep multiplier, cycle index
     int coarse_bits = 32;            // stencil storage of FP32 coarse-level cycles: 32 or 16 (bfloat16)
+    int fine_smoother = 1;           // 0: four colour-pair launches in place; 1: two tiled launches, two buffers
+    V3<float> gs_tmp32;              // second buffer of the tiled fine smoother
+    V3<double> gs_tmp64;
     // the preconditioner cycle replayed as a HIP graph (pc_key identifies its shape)
     bool use_graphs = true;
     hipGraphExec_t pc_graph = nullptr;
@@ -402,6 +405,10 @@ template <typename T> LevelVecs<T>& pcvecs(Hier& H);
 template <> LevelVecs<double>& pcvecs<double>(Hier& H) { if (!H.pc64.ok()) H.pc64.alloc(H.lv[0].g.N); return H.pc64; }
 template <> LevelVecs<float>& pcvecs<float>(Hier& H) { if (!H.pc32.ok()) H.pc32.alloc(H.lv[0].g.N); return H.pc32; }
 
+template <typename T> V3<T>& gs_tmp(Hier& H);
+template <> V3<double>& gs_tmp<double>(Hier& H) { if (!H.gs_tmp64.ok()) H.gs_tmp64.alloc(H.lv[0].g.N); return H.gs_tmp64; }
+template <> V3<float>& gs_tmp<float>(Hier& H) { if (!H.gs_tmp32.ok()) H.gs_tmp32.alloc(H.lv[0].g.N); return H.gs_tmp32; }
+
 // ---------------------------------------------------------------------------
 // multigrid operations (level index li into H.lv; the finest level is 0)
 // ---------------------------------------------------------------------------
@@ -411,7 +418,16 @@ template <typename T>
 static void gs(Hier& H, int li, Vec3<T> U, Vec3<const T> F, int n, bool reverse, bool zero_support = false) {
     Level& L = H.lv[li];
     for (int it = 0; it < n; it++) {
-        if (li == 0) {
+        if (li == 0 && H.fine_smoother == 1) {
+            // two launches per sweep, ping-ponging between U and the second buffer
+            V3<T>& tmp = gs_tmp<T>(H);
+            const bool in_u = it % 2 == 0;
+            const Vec3<T> Tv = tmp.v();
+            launch_fine_gs_sweep2<T>(L.g, in_u ? Vec3<const T>{{U.c[0], U.c[1], U.c[2]}} : tmp.cv(), in_u ? Tv : U, F,
+                                     H.rhop.as<float>(), L.vflag.as<uint8_t>(), reverse, zero_support, H.s);
+            if (in_u && it == n - 1)
+                for (int c = 0; c < 3; c++) launch_convert<T, T>(Tv.c[c], U.c[c], (size_t)L.g.N, H.s);
+        } else if (li == 0) {
             // four launches, each relaxing the colour pair (2p, 2p+1) row by row
             for (int pp = 0; pp < 4; pp++)
                 launch_fine_gs<T>(L.g, U, F, H.rhop.as<float>(), L.vflag.as<uint8_t>(), reverse ? 3 - pp : pp,
@@ -1348,6 +1364,15 @@ int rt_set_graphs(rt_hier* h, int on) {
         Hier& H = H_(h);
         H.drop_graph();
         H.use_graphs = on != 0;
+    })
+}
+
+int rt_set_fine_smoother(rt_hier* h, int kind) {
+    RT_TRY({
+        Hier& H = H_(h);
+        if (kind != 0 && kind != 1) throw std::invalid_argument("fine smoother kind must be 0 or 1");
+        H.drop_graph();
+        H.fine_smoother = kind;
     })
 }
 

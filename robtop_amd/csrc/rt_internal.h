@@ -108,6 +108,11 @@ void set_ke_constants(const double* ke576, hipStream_t s);
 template <typename T>
 void launch_fine_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* rhop, const uint8_t* vflag, int pair,
                     bool reverse, bool zero_support, hipStream_t s);
+// one full fine sweep in two launches, old values from Uin, new ones to Uout
+// (Uin is not written; the two must not overlap)
+template <typename T>
+void launch_fine_gs_sweep2(const Geom& g, Vec3<const T> Uin, Vec3<T> Uout, Vec3<const T> F, const float* rhop,
+                           const uint8_t* vflag, bool reverse, bool zero_support, hipStream_t s);
 
 // k_fine_modal.hip (element-centric residual / apply in the sum-difference basis)
 void set_modal_constants(const double* ke576);

@@ -243,6 +243,10 @@ class HierarchyGrid:
         check(lib().rt_vcycle(self.handle, pre_relax, post_relax, ctypes.byref(r)))
         return r.value
 
+    def set_fine_smoother(self, kind):
+        """Finest-level sweep: 1 = two tiled launches (default), 0 = four colour-pair launches."""
+        check(lib().rt_set_fine_smoother(self.handle, kind))
+
     def set_coarse_stencil_bits(self, bits):
         """Coarse-level stencil storage of FP32 cycles: 32 (FP32) or 16 (bfloat16)."""
         check(lib().rt_set_coarse_stencil_bits(self.handle, bits))
