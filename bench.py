@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--coarse-mult", type=int, default=1, help="sweep multiplier on coarse levels (preconditioner)")
     ap.add_argument("--gamma", type=int, default=1, help="preconditioner cycle index: 1 = V, 2 = W")
+    ap.add_argument("--fine-smoother", default="pairs", choices=["tiled", "pairs"],
+                    help="finest-level sweep: four colour-pair launches (default) or two tiled launches")
     ap.add_argument("--coarse-bf16", action="store_true",
                     help="store the coarse-level stencils of the FP32 preconditioner in bfloat16")
     ap.add_argument("--mode", default="reliable", choices=["mixed", "reliable", "fp64"],
@@ -146,13 +148,14 @@ CALIB_BYTES = 256 * 1024 * 1024
 
 
 def profile_run(a):
-    """Kernel-only run for counter collection: the fine-level smoother (two
-    sweeps = 8 colour-pair launches), one V-cycle, then a plain copy of
-    CALIB_BYTES between two device buffers as the calibration dispatch."""
+    """Kernel-only run for counter collection: two fine-level sweeps (with the
+    chosen smoother), one V-cycle, then a plain copy of CALIB_BYTES between two
+    device buffers as the calibration dispatch."""
     import torch
     ex, ey, ez = (int(v) for v in a.size.split(","))
     hg, *_ = build_problem(ex, ey, ez, a.seed)
     hg.set_graphs(False)
+    hg.set_fine_smoother(1 if a.fine_smoother == "tiled" else 0)
     hg.update_stencil()
     log(f"profile {ex}x{ey}x{ez}: fine sweeps")
     hg.time_fine_sweep(32, 2)
@@ -191,13 +194,15 @@ def pmc_summary(fetch_csv, write_csv, size):
     fa, fc = per_kernel(fetch_csv, "FETCH_SIZE")
     wa, wc = per_kernel(write_csv, "WRITE_SIZE")
     kf, kw = fc / CALIB_BYTES, wc / CALIB_BYTES
-    gs = [k for k in fa if "k_fine_gs_pair" in k]
+    kname = "k_fine_gs_tile" if any("k_fine_gs_tile" in k for k in fa) else "k_fine_gs_pair"
+    launches = 2 if kname == "k_fine_gs_tile" else 4
+    gs = [k for k in fa if kname in k]
     fetch = sum(sum(fa[k]) for k in gs) / sum(len(fa[k]) for k in gs) / kf
     write = sum(sum(wa[k]) for k in gs) / sum(len(wa[k]) for k in gs) / kw
     ex, ey, ez = size
     nodes, elems = (ex + 1) * (ey + 1) * (ez + 1), ex * ey * ez
-    alg = (nodes * (9 * 4 + 1) + 4 * elems) / 4.0
-    return {"kernel": "k_fine_gs_pair", "size": list(size), "bytes_per_launch": fetch + write,
+    alg = (nodes * (9 * 4 + 1) + 4 * elems) / launches
+    return {"kernel": kname, "size": list(size), "bytes_per_launch": fetch + write,
             "fetch_bytes_per_launch": fetch, "write_bytes_per_launch": write,
             "calibration": {"copy_bytes": CALIB_BYTES, "fetch_factor": kf, "write_factor": kw},
             "algorithmic_bytes_per_launch": alg, "traffic_over_algorithmic": (fetch + write) / alg}
@@ -275,6 +280,7 @@ def main():
     hg.set_preconditioner(a.coarse_mult, a.gamma)
     if a.coarse_bf16:
         hg.set_coarse_stencil_bits(16)
+    hg.set_fine_smoother(1 if a.fine_smoother == "tiled" else 0)
     hg.set_graphs(not a.no_graphs)
     g0 = hg[0]
     ne0, nn0 = hg.num_active(0)
@@ -330,19 +336,22 @@ def main():
     t_sweep = hg.time_fine_sweep(32, 10)
     B_V = algorithmic_vcycle_bytes(levels, a.pre, a.post, coarsest_dofs, mult=a.coarse_mult, gamma=a.gamma,
                                    Ps=2 if a.coarse_bf16 else 4)
-    # a sweep is four launches, each relaxing two colours (half the rows of the box
-    # in y-z parity class, both x parities); each launch moves a quarter of the
-    # sweep's compulsory bytes: u (read + write), f, flag for its nodes and, over the
-    # sweep, rho^p once per element and u once per node as a neighbour
+    # Compulsory bytes of a sweep: u (read + write), f and the flag once per node,
+    # rho^p once per element. The tiled smoother runs a sweep as two launches
+    # (one z-parity class each, half the nodes), the colour-pair smoother as four
+    # (one (y, z) parity class each); each launch is charged its share.
+    n_launch = 2 if a.fine_smoother == "tiled" else 4
+    kname = "k_fine_gs_tile" if a.fine_smoother == "tiled" else "k_fine_gs_pair"
     sweep_bytes = levels[0]["nodes"] * (9 * 4 + 1) + 4 * levels[0]["elements"]
-    launch_bytes = sweep_bytes / 4.0
-    launch_ms = t_sweep / 4.0
+    launch_bytes = sweep_bytes / n_launch
+    launch_ms = t_sweep / n_launch
     achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic_fine_gs.json")
     if os.path.exists(tp):
         try:
-            traffic = json.load(open(tp)).get("bytes_per_launch")
+            tj = json.load(open(tp))
+            traffic = tj.get("bytes_per_launch") if tj.get("kernel") == kname else None
         except Exception:
             traffic = None
 
@@ -371,7 +380,8 @@ def main():
                    "frac_of_peak": B_V / (t_v * 1e-3) / 1e9 / HBM_PEAK_GBS, "pre": a.pre, "post": a.post,
                    "coarse_mult": a.coarse_mult, "gamma": a.gamma,
                    "coarse_stencil_bits": 16 if a.coarse_bf16 else 32},
-        "roofline": {"kernel": "k_fine_gs_pair (one colour-pair launch of the finest-level smoother)", "bound": "hbm",
+        "roofline": {"kernel": f"{kname} (one of the {n_launch} launches of a finest-level Gauss-Seidel sweep)",
+                     "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": launch_bytes, "launch_ms": launch_ms},
     }

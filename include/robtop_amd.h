@@ -168,10 +168,10 @@ int rt_vcycle(rt_hier* h, int pre, int post, double* rel_res);
 /* --- solve and SIMP step ------------------------------------------------- */
 
 /* Implementation of the finest-level Gauss-Seidel sweep (same result bit for
- * bit): 1 (default) = two launches per sweep, each relaxing the four colours
- * of one z-parity class tile by tile from an old-value buffer into a second
- * buffer; 0 = four colour-pair launches updating in place. No counterpart in
- * the reference. */
+ * bit): 0 (default) = four colour-pair launches updating in place; 1 = two
+ * launches per sweep, each relaxing the four colours of one z-parity class
+ * tile by tile from an old-value buffer into a second buffer. No counterpart
+ * in the reference. */
 int rt_set_fine_smoother(rt_hier* h, int kind);
 
 /* Storage of the coarse-level stencils read by FP32 cycles: 32 (FP32, the

@@ -126,7 +126,7 @@ struct Hier {
     int vprec = 64;
     int pc_mult = 1, pc_gamma = 1;   // preconditioner cycle: coarse-level sweep multiplier, cycle index
     int coarse_bits = 32;            // stencil storage of FP32 coarse-level cycles: 32 or 16 (bfloat16)
-    int fine_smoother = 1;           // 0: four colour-pair launches in place; 1: two tiled launches, two buffers
+    int fine_smoother = 0;           // 0: four colour-pair launches in place; 1: two tiled launches, two buffers
     V3<float> gs_tmp32;              // second buffer of the tiled fine smoother
     V3<double> gs_tmp64;
     // the preconditioner cycle replayed as a HIP graph (pc_key identifies its shape)

@@ -244,7 +244,7 @@ class HierarchyGrid:
         return r.value
 
     def set_fine_smoother(self, kind):
-        """Finest-level sweep: 1 = two tiled launches (default), 0 = four colour-pair launches."""
+        """Finest-level sweep: 0 = four colour-pair launches (default), 1 = two tiled launches."""
         check(lib().rt_set_fine_smoother(self.handle, kind))
 
     def set_coarse_stencil_bits(self, bits):
