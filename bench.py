@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--gamma", type=int, default=1, help="preconditioner cycle index: 1 = V, 2 = W")
     ap.add_argument("--fine-smoother", default="pairs", choices=["tiled", "pairs"],
                     help="finest-level sweep: four colour-pair launches (default) or two tiled launches")
+    ap.add_argument("--warm-start", type=int, default=0,
+                    help="start each solve from the energy-optimal combination of the last M solutions (0: the "
+                         "previous solution, energy-scaled)")
     ap.add_argument("--coarse-bf16", action="store_true",
                     help="store the coarse-level stencils of the FP32 preconditioner in bfloat16")
     ap.add_argument("--mode", default="reliable", choices=["mixed", "reliable", "fp64"],
@@ -281,6 +284,7 @@ def main():
     if a.coarse_bf16:
         hg.set_coarse_stencil_bits(16)
     hg.set_fine_smoother(1 if a.fine_smoother == "tiled" else 0)
+    hg.set_warm_start(a.warm_start)
     hg.set_graphs(not a.no_graphs)
     g0 = hg[0]
     ne0, nn0 = hg.num_active(0)

@@ -167,6 +167,15 @@ int rt_vcycle(rt_hier* h, int pre, int post, double* rel_res);
 
 /* --- solve and SIMP step ------------------------------------------------- */
 
+/* Warm start of rt_solve from a subspace of previous solutions: with m >= 1
+ * every solve starts from the combination of the last m returned solutions
+ * that minimises the energy norm of the error under the current stiffness
+ * (Galerkin projection, an m x m system), and each solve's result is stored.
+ * m = 0 (default) starts from the current displacement scaled by the
+ * energy-optimal factor. m <= 8; setting m clears the stored solutions. No
+ * counterpart in the reference. */
+int rt_set_warm_start(rt_hier* h, int m);
+
 /* Implementation of the finest-level Gauss-Seidel sweep (same result bit for
  * bit): 0 (default) = four colour-pair launches updating in place; 1 = two
  * launches per sweep, each relaxing the four colours of one z-parity class

@@ -105,6 +105,7 @@ SIGNATURES = {
     "rt_modified_pm": ([P, I, P, P], I),
     "rt_set_coarse_stencil_bits": ([P, I], I),
     "rt_set_fine_smoother": ([P, I], I),
+    "rt_set_warm_start": ([P, I], I),
     "rt_block_lanczos": ([P, I, I, D, D, I, I, I, ctypes.c_uint64, P, P], I),
     "rt_synchronize": ([P], I),
     "rt_time_vcycle": ([P, I, I, I, I, P], I),

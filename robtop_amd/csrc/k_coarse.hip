@@ -913,6 +913,29 @@ void launch_scale(T* x, double a, size_t n, hipStream_t s) {
 template void launch_scale<float>(float*, double, size_t, hipStream_t);
 template void launch_scale<double>(double*, double, size_t, hipStream_t);
 
+// x = sum_i c[i] h[i] over k <= 8 vectors (FP64)
+struct LinComb {
+    const double* h[8];
+    double c[8];
+    int k;
+};
+__global__ void k_lincomb(double* __restrict__ x, LinComb lc, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double s = 0;
+        for (int j = 0; j < lc.k; j++) s += lc.c[j] * lc.h[j][i];
+        x[i] = s;
+    }
+}
+void launch_lincomb(double* x, const double* const* h, const double* c, int k, size_t n, hipStream_t s) {
+    if (k < 1 || k > 8) throw std::invalid_argument("launch_lincomb: 1..8 vectors");
+    LinComb lc{};
+    for (int j = 0; j < k; j++) { lc.h[j] = h[j]; lc.c[j] = c[j]; }
+    lc.k = k;
+    unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_lincomb, dim3(nb), dim3(256), 0, s, x, lc, n);
+    RT_HIP(hipGetLastError());
+}
+
 // x += e (TE -> T)
 template <typename T, typename TE>
 __global__ void k_add_conv(T* __restrict__ x, const TE* __restrict__ e, size_t n) {

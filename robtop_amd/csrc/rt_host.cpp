@@ -16,6 +16,9 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+// dense symmetric eigensolver (defined with the block Lanczos search below)
+static void jacobi_eigen(int n, std::vector<double>& A, std::vector<double>& w, std::vector<double>& V);
+
 namespace rt {
 
 thread_local std::string g_err;
@@ -127,6 +130,9 @@ struct Hier {
     int pc_mult = 1, pc_gamma = 1;   // preconditioner cycle: coarse-level sweep multiplier, cycle index
     int coarse_bits = 32;            // stencil storage of FP32 coarse-level cycles: 32 or 16 (bfloat16)
     int fine_smoother = 0;           // 0: four colour-pair launches in place; 1: two tiled launches, two buffers
+    int ws_m = 0;                    // solves start from the best combination of the last ws_m solutions (0: off)
+    std::vector<V3<double>> ws_hist; // ring of previous solutions
+    int ws_count = 0, ws_head = 0;
     V3<float> gs_tmp32;              // second buffer of the tiled fine smoother
     V3<double> gs_tmp64;
     // the preconditioner cycle replayed as a HIP graph (pc_key identifies its shape)
@@ -725,6 +731,58 @@ static double true_rel_residual(Hier& H) {
 // PCG on caller-given vectors x (initial guess, updated) and b; stops when the
 // recurrence residual reaches tol * |b|. Returns the iteration count; `rel`
 // receives the final recurrence residual ratio.
+// Subspace warm start: x64 = sum_i c_i h_i over the stored previous solutions
+// h_i, with c the Galerkin (energy-norm optimal) coefficients, G c = b,
+// G_ij = h_i . K h_j, b_i = h_i . f, solved through the eigen-decomposition of
+// G with near-dependent directions dropped. With one stored solution this is
+// the energy-optimal scaling applied in any case.
+static void subspace_warm_start(Hier& H) {
+    if (H.ws_m < 1 || H.ws_count < 1) return;
+    const int k = H.ws_count;
+    Level& L0 = H.lv[0];
+    const size_t n = (size_t)3 * L0.g.N;
+    if (!H.q64.ok()) H.q64.alloc(L0.g.N);
+    std::vector<double> G((size_t)k * k), b(k);
+    for (int j = 0; j < k; j++) {
+        launch_fine_apply_modal<double>(L0.g, H.ws_hist[j].cv(), H.q64.v(), H.rhop.as<float>(),
+                                        L0.vflag.as<uint8_t>(), nullptr, nullptr, H.s);
+        for (int i = 0; i <= j; i++)
+            G[(size_t)i * k + j] = G[(size_t)j * k + i] = dot<double, double>(H, H.ws_hist[i].data(), H.q64.data(), n);
+        b[j] = dot<double, double>(H, H.ws_hist[j].data(), H.f64.data(), n);
+    }
+    std::vector<double> A = G, w, V;
+    jacobi_eigen(k, A, w, V);
+    double wmax = 0;
+    for (double x : w) wmax = std::max(wmax, x);
+    if (!(wmax > 0)) return;
+    std::vector<double> c(k, 0.0);
+    for (int e = 0; e < k; e++) {
+        if (!(w[e] > 1e-10 * wmax)) continue;
+        double vb = 0;
+        for (int i = 0; i < k; i++) vb += V[(size_t)i * k + e] * b[i];
+        for (int i = 0; i < k; i++) c[i] += V[(size_t)i * k + e] * vb / w[e];
+    }
+    std::vector<const double*> h(k);
+    for (int i = 0; i < k; i++) h[i] = H.ws_hist[i].data();
+    launch_lincomb(H.x64.data(), h.data(), c.data(), k, n, H.s);
+}
+
+// store the current solution in the ring of previous solutions
+static void subspace_push(Hier& H) {
+    if (H.ws_m < 1) return;
+    Level& L0 = H.lv[0];
+    if ((int)H.ws_hist.size() != H.ws_m) {
+        H.ws_hist.clear();
+        H.ws_hist.resize(H.ws_m);
+        H.ws_count = H.ws_head = 0;
+    }
+    V3<double>& d = H.ws_hist[H.ws_head];
+    if (!d.ok()) d.alloc(L0.g.N);
+    RT_HIP(hipMemcpyAsync(d.data(), H.x64.data(), sizeof(double) * 3 * (size_t)L0.g.N, hipMemcpyDeviceToDevice, H.s));
+    H.ws_head = (H.ws_head + 1) % H.ws_m;
+    H.ws_count = std::min(H.ws_count + 1, H.ws_m);
+}
+
 template <typename TV, typename TC>
 static int pcg_core(Hier& H, V3<TV>& x, V3<TV>& b, double tol, int maxit, int pre, int post, double* rel_out) {
     Level& L0 = H.lv[0];
@@ -789,6 +847,7 @@ static void pcg(Hier& H, double tol, int maxit, int pre, int post, rt_solve_stat
     V3<TV>& b = KV<TV>::b(H);
     if (!x.ok()) x.alloc(N);
     if (!b.ok()) b.alloc(N);
+    subspace_warm_start(H);
     if ((void*)x.data() != (void*)H.x64.data()) {
         launch_convert<double, TV>(H.x64.data(), x.data(), n, H.s);
         launch_convert<double, TV>(H.f64.data(), b.data(), n, H.s);
@@ -796,6 +855,7 @@ static void pcg(Hier& H, double tol, int maxit, int pre, int post, rt_solve_stat
     double rel = 0;
     const int it = pcg_core<TV, TC>(H, x, b, tol, maxit, pre, post, &rel);
     if ((void*)x.data() != (void*)H.x64.data()) launch_convert<TV, double>(x.data(), H.x64.data(), n, H.s);
+    subspace_push(H);
     RT_HIP(hipStreamSynchronize(H.s));
     const double tr = true_rel_residual(H);
     auto t1 = std::chrono::steady_clock::now();
@@ -855,6 +915,7 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
     if (!H.dx32.ok()) H.dx32.alloc(N);
     LevelVecs<float>& pc = pcvecs<float>(H);   // pc.F holds the FP32 residual, pc.U the preconditioned one
     const int np = 1024;
+    subspace_warm_start(H);
     const double xAx = apply_dot<double>(H, H.x64, H.q64);
     if (xAx > 0) {
         const double xb = dot<double, double>(H, H.x64.data(), H.f64.data(), n);
@@ -906,6 +967,7 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
         if (!flushed) launch_pcg_flush_x(H.x64.data(), H.dx32.data(), n, H.s);
         if (it > maxit) it = maxit;
     }
+    subspace_push(H);
     RT_HIP(hipStreamSynchronize(H.s));
     const double tr = true_rel_residual(H);
     auto t1 = std::chrono::steady_clock::now();
@@ -1364,6 +1426,16 @@ int rt_set_graphs(rt_hier* h, int on) {
         Hier& H = H_(h);
         H.drop_graph();
         H.use_graphs = on != 0;
+    })
+}
+
+int rt_set_warm_start(rt_hier* h, int m) {
+    RT_TRY({
+        Hier& H = H_(h);
+        if (m < 0 || m > 8) throw std::invalid_argument("warm-start subspace size must be 0..8");
+        H.ws_m = m;
+        H.ws_hist.clear();
+        H.ws_count = H.ws_head = 0;
     })
 }
 

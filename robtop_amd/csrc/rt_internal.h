@@ -143,6 +143,8 @@ template <typename T>
 void launch_coarsest(const double* Ainv, int n3, const int* rowslot, int unused, Vec3<const T> F, Vec3<T> U,
                      double* fbuf, hipStream_t s);
 void launch_mirror_upper(double* A, int n, hipStream_t s);
+// x = sum_i c[i] h[i], k <= 8 FP64 vectors of length n
+void launch_lincomb(double* x, const double* const* h, const double* c, int k, size_t n, hipStream_t s);
 // b = a rounded to bfloat16 (round to nearest even, through binary32)
 void launch_to_bf16(const double* a, bf16* b, size_t n, hipStream_t s);
 template <typename TA, typename TB>

@@ -243,6 +243,10 @@ class HierarchyGrid:
         check(lib().rt_vcycle(self.handle, pre_relax, post_relax, ctypes.byref(r)))
         return r.value
 
+    def set_warm_start(self, m):
+        """Start solves from the energy-optimal combination of the last m solutions (0: off)."""
+        check(lib().rt_set_warm_start(self.handle, m))
+
     def set_fine_smoother(self, kind):
         """Finest-level sweep: 0 = four colour-pair launches (default), 1 = two tiled launches."""
         check(lib().rt_set_fine_smoother(self.handle, kind))

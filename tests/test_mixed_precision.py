@@ -65,3 +65,29 @@ def test_bf16_coarse_stencils_keep_iteration_count(cantilever):
     hg.set_coarse_stencil_bits(32)
     print(f"[mixed] PCG iterations to 1e-8: {its}", flush=True)
     assert its[16] <= 1.25 * its[32] + 2, its
+
+
+def test_subspace_warm_start(cantilever):
+    """A sequence of solves under slowly changing densities: starting each from
+    the energy-optimal combination of the last three solutions reaches the same
+    tolerance and the same solution as starting from the previous one."""
+    hg, f, u_ref, c_ref = cantilever
+    rho0 = hg.get_density()
+    its, sols = {}, {}
+    for m in (0, 3):
+        hg.set_warm_start(m)
+        hg.set_displacement(np.zeros_like(f))
+        its[m] = []
+        for step in range(4):
+            hg.set_density(np.clip(rho0 * (1 + 0.03 * step), 1e-3, 1.0).astype(np.float32))
+            hg.update_stencil()
+            st = hg.solve(tol=1e-8, maxit=2000, mode=rt.RT_SOLVE_RELIABLE, pre=2, post=2)
+            assert st.rel_res < 1.5e-8, (m, step, st.rel_res)
+            its[m].append(st.iterations)
+        sols[m] = hg.get_displacement()
+    hg.set_warm_start(0)
+    hg.set_density(rho0)
+    hg.update_stencil()
+    print(f"[mixed] PCG iterations per step, warm start 0 / 3: {its}", flush=True)
+    assert rel_err(sols[3], sols[0]) < 1e-5
+
