@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--warm-start", type=int, default=0,
                     help="start each solve from the energy-optimal combination of the last M solutions (0: the "
                          "previous solution, energy-scaled)")
+    ap.add_argument("--islands", type=float, default=0.0, metavar="THETA",
+                    help="deflate the rigid-body modes of floating clusters of elements with rho >= THETA in the "
+                         "preconditioner (0: off)")
     ap.add_argument("--coarse-bf16", action="store_true",
                     help="store the coarse-level stencils of the FP32 preconditioner in bfloat16")
     ap.add_argument("--mode", default="reliable", choices=["mixed", "reliable", "fp64"],
@@ -285,6 +288,8 @@ def main():
         hg.set_coarse_stencil_bits(16)
     hg.set_fine_smoother(1 if a.fine_smoother == "tiled" else 0)
     hg.set_warm_start(a.warm_start)
+    if a.islands > 0:
+        hg.set_island_deflation(a.islands, 8)
     hg.set_graphs(not a.no_graphs)
     g0 = hg[0]
     ne0, nn0 = hg.num_active(0)

@@ -167,6 +167,15 @@ int rt_vcycle(rt_hier* h, int pre, int post, double* rel_res);
 
 /* --- solve and SIMP step ------------------------------------------------- */
 
+/* Floating-cluster deflation of the preconditioner (no counterpart in the
+ * reference): at every rt_update_stencil, the face-connected clusters of at
+ * least min_size elements with rho >= theta that touch no support are found,
+ * and the preconditioner adds the exact correction on their six rigid-body
+ * modes, z += sum_i Z_i (Z_i^T K Z_i)^+ Z_i^T r, to the V-cycle. At most 4096
+ * clusters. theta <= 0 turns it off (default). */
+int rt_set_island_deflation(rt_hier* h, float theta, int min_size);
+int rt_num_islands(const rt_hier* h, int* n);
+
 /* Warm start of rt_solve from a subspace of previous solutions: with m >= 1
  * every solve starts from the combination of the last m returned solutions
  * that minimises the energy norm of the error under the current stiffness

@@ -106,6 +106,8 @@ SIGNATURES = {
     "rt_set_coarse_stencil_bits": ([P, I], I),
     "rt_set_fine_smoother": ([P, I], I),
     "rt_set_warm_start": ([P, I], I),
+    "rt_set_island_deflation": ([P, F, I], I),
+    "rt_num_islands": ([P, P], I),
     "rt_block_lanczos": ([P, I, I, D, D, I, I, I, ctypes.c_uint64, P, P], I),
     "rt_synchronize": ([P], I),
     "rt_time_vcycle": ([P, I, I, I, I, P], I),

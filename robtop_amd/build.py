@@ -31,6 +31,7 @@ UNITS = {
     "k_galerkin.hip": [],
     "k_simp.hip": [],
     "k_worst.hip": [],
+    "k_island.hip": [],
     "k_mma.hip": ["-ffp-contract=off"],
     "k_voxel.hip": ["-ffp-contract=off"],
     "rt_host.cpp": ["-x", "hip"],

@@ -24,12 +24,21 @@ namespace rt {
 #define RT_MODAL_XCD 1
 #endif
 
+#ifndef RT_MODAL_TX
+#define RT_MODAL_TX 31
+#endif
+#ifndef RT_MODAL_ZC
+#define RT_MODAL_ZC 32
+#endif
+
 namespace {
-constexpr int TX = 31, TY = 7;             // node tile
+constexpr int TX = RT_MODAL_TX;            // node tile: TX x TY with (TX + 1)(TY + 1) = 256
+constexpr int TY = 256 / (TX + 1) - 1;
 constexpr int EX = TX + 1, EY = TY + 1;    // element tile (= 256 threads)
 constexpr int PX = TX + 2, PY = TY + 2;    // node plane with halo
 constexpr int PLANE = PX * PY;
-constexpr int ZC = 32;                     // z planes per workgroup
+constexpr int ZC = RT_MODAL_ZC;            // z planes per workgroup
+static_assert(EX * EY == 256 && PLANE <= 512, "tile must fill 256 threads; a plane is staged by two points per thread");
 }
 
 template <typename T> struct ModalK { T m[RT_MODAL_NNZ]; };

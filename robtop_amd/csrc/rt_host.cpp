@@ -131,6 +131,11 @@ struct Hier {
     int coarse_bits = 32;            // stencil storage of FP32 coarse-level cycles: 32 or 16 (bfloat16)
     int fine_smoother = 0;           // 0: four colour-pair launches in place; 1: two tiled launches, two buffers
     int ws_m = 0;                    // solves start from the best combination of the last ws_m solutions (0: off)
+    // floating-cluster deflation of the preconditioner (theta <= 0: off)
+    float isl_theta = 0.f;
+    int isl_min = 8, isl_max = 4096, nisl = 0;
+    DBuf isl_lab, isl_anch, isl_cnt, isl_root, isl_node, isl_cent, isl_E, isl_Einv, isl_zr, isl_coef, isl_flag,
+        isl_KE;
     std::vector<V3<double>> ws_hist; // ring of previous solutions
     int ws_count = 0, ws_head = 0;
     V3<float> gs_tmp32;              // second buffer of the tiled fine smoother
@@ -546,6 +551,8 @@ static void precondition(Hier& H, LevelVecs<TC>& pc, int pre, int post) {
     }
     launch_fill<TC>(pc.U.data(), TC(0), n, H.s);
     vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma, true);
+    island_correct<TC>(H.lv[0].g, H.nisl, H.isl_node.as<int>(), H.isl_cent.as<double>(), H.isl_Einv.as<double>(),
+                       H.isl_zr.as<double>(), H.isl_coef.as<double>(), pc.F.cv(), pc.U.v(), H.s);
     if (!H.use_graphs) return;
     H.drop_graph();
     RT_HIP(hipStreamBeginCapture(H.s, hipStreamCaptureModeThreadLocal));
@@ -553,6 +560,9 @@ static void precondition(Hier& H, LevelVecs<TC>& pc, int pre, int post) {
     try {
         launch_fill<TC>(pc.U.data(), TC(0), n, H.s);
         vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma, true);
+        island_correct<TC>(H.lv[0].g, H.nisl, H.isl_node.as<int>(), H.isl_cent.as<double>(),
+                           H.isl_Einv.as<double>(), H.isl_zr.as<double>(), H.isl_coef.as<double>(), pc.F.cv(),
+                           pc.U.v(), H.s);
     } catch (...) {
         (void)hipStreamEndCapture(H.s, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -662,6 +672,31 @@ static void factor_coarsest(Hier& H) {
     RT_HIP(hipStreamSynchronize(H.s));
 }
 
+static void build_islands(Hier& H) {
+    H.nisl = 0;
+    if (!(H.isl_theta > 0)) return;
+    Level& L0 = H.lv[0];
+    const size_t N = (size_t)L0.g.N, mx = (size_t)H.isl_max;
+    H.isl_lab.ensure(sizeof(int) * N);
+    H.isl_anch.ensure(N);
+    H.isl_cnt.ensure(sizeof(int) * N);
+    H.isl_root.ensure(sizeof(int) * N);
+    H.isl_node.ensure(sizeof(int) * N);
+    H.isl_cent.ensure(sizeof(double) * 4 * mx);
+    H.isl_E.ensure(sizeof(double) * 36 * mx);
+    H.isl_Einv.ensure(sizeof(double) * 36 * mx);
+    if (!H.isl_zr.p) H.isl_zr.alloc(sizeof(double) * 6 * mx);     // zeroed once; the correction leaves it zero
+    H.isl_coef.ensure(sizeof(double) * 6 * mx);
+    H.isl_flag.ensure(sizeof(int));
+    H.isl_KE.ensure(sizeof(double) * 576);
+    RT_HIP(hipMemcpyAsync(H.isl_KE.p, H.KE, sizeof(double) * 576, hipMemcpyHostToDevice, H.s));
+    H.nisl = island_build(L0.g, H.rho.as<float>(), H.rhop.as<float>(), H.eflag.as<uint8_t>(), L0.vflag.as<uint8_t>(),
+                          H.isl_KE.as<double>(), H.isl_theta, H.isl_min, H.isl_max, H.isl_lab.as<int>(),
+                          H.isl_anch.as<uint8_t>(), H.isl_cnt.as<int>(), H.isl_root.as<int>(), H.isl_node.as<int>(),
+                          H.isl_cent.as<double>(), H.isl_E.as<double>(), H.isl_Einv.as<double>(),
+                          H.isl_flag.as<int>(), H.s);
+}
+
 static void update_stencil(Hier& H) {
     H.drop_graph();
     if (!H.material_set) throw std::logic_error("rt_set_material has not been called");
@@ -683,6 +718,7 @@ static void update_stencil(Hier& H) {
         }
     }
     factor_coarsest(H);
+    build_islands(H);
     H.stencil_ready = true;
 }
 
@@ -929,11 +965,13 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
     const double nb = std::sqrt(dot<double, double>(H, H.f64.data(), H.f64.data(), n));
     double rel = nb > 0 ? std::sqrt(rr) / nb : 0.0, rel_true = rel, rmax = std::sqrt(rr);
     int it = 0;
+    bool diverged = false;
     if (nb > 0 && rel > tol) {
         // The solution increment since the last residual replacement is kept
         // in FP32 (dx, updated with r) and flushed into x64 at each
         // replacement and at exit.
         const int ns = kStreamBlocks;
+        double best_true = rel_true;
         launch_fill<float>(H.dx32.data(), 0.f, n, H.s);
         precondition<float>(H, pc, pre, post);
         double rz = dot<float, float>(H, pc.F.data(), pc.U.data(), n);
@@ -952,10 +990,12 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
                 launch_pcg_flush_x(H.x64.data(), H.dx32.data(), n, H.s);
                 flushed = true;
                 rel_true = true_rel_residual(H);      // r64 = f - K x64
+                if (rel_true <= tol) { rel = rel_true; break; }
+                if (!(rel_true <= 10 * best_true)) { diverged = true; break; }
+                best_true = std::min(best_true, rel_true);
                 launch_convert<double, float>(H.r64.data(), pc.F.data(), n, H.s);
                 rmax = rel_true * nb;
                 rel = rel_true;
-                if (rel_true <= tol) break;
             }
             if (it == maxit) break;
             precondition<float>(H, pc, pre, post);
@@ -966,6 +1006,21 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
         }
         if (!flushed) launch_pcg_flush_x(H.x64.data(), H.dx32.data(), n, H.s);
         if (it > maxit) it = maxit;
+    }
+    if (diverged) {
+        // The FP32 recurrence has lost the true residual (a replacement found it
+        // 10x above the best one so far; seen with 0/1 densities at a 1e-9
+        // stiffness contrast). Finish with FP64 Krylov vectors from the current
+        // x, whose energy-optimal scaling is never worse than a zero start.
+        rt_solve_stats s2{};
+        pcg<double, float>(H, tol, std::max(0, maxit - it), pre, post, &s2);
+        if (st) {
+            st->iterations = it + s2.iterations;
+            st->rel_res = s2.rel_res;
+            st->rel_res_pcg = s2.rel_res_pcg;
+            st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return;
     }
     subspace_push(H);
     RT_HIP(hipStreamSynchronize(H.s));
@@ -1427,6 +1482,21 @@ int rt_set_graphs(rt_hier* h, int on) {
         H.drop_graph();
         H.use_graphs = on != 0;
     })
+}
+
+int rt_set_island_deflation(rt_hier* h, float theta, int min_size) {
+    RT_TRY({
+        Hier& H = H_(h);
+        if (min_size < 1) throw std::invalid_argument("island min_size must be positive");
+        H.drop_graph();
+        H.isl_theta = theta;
+        H.isl_min = min_size;
+        if (H.stencil_ready) build_islands(H);
+    })
+}
+
+int rt_num_islands(const rt_hier* h, int* n) {
+    RT_TRY({ need(n); *n = H_(h).nisl; })
 }
 
 int rt_set_warm_start(rt_hier* h, int m) {

@@ -145,6 +145,17 @@ void launch_coarsest(const double* Ainv, int n3, const int* rowslot, int unused,
 void launch_mirror_upper(double* A, int n, hipStream_t s);
 // x = sum_i c[i] h[i], k <= 8 FP64 vectors of length n
 void launch_lincomb(double* x, const double* const* h, const double* c, int k, size_t n, hipStream_t s);
+// floating-cluster deflation (k_island.hip): builds the islands of stiff
+// elements (rho >= theta, at least min_size elements, at most max_isl) that
+// touch no support, their centroids and pseudo-inverted rigid-body energies;
+// returns the island count
+int island_build(const Geom& g, const float* rho, const float* rhop, const uint8_t* eflag, const uint8_t* vflag,
+                 const double* KE_dev, float theta, int min_size, int max_isl, int* lab, uint8_t* anchored, int* count,
+                 int* root_isl, int* node_isl, double* cent, double* E, double* Einv, int* flag, hipStream_t s);
+// Z += sum_i Z_i E_i^+ Z_i^T R (zr must be zero on entry and is left zero)
+template <typename T>
+void island_correct(const Geom& g, int nisl, const int* node_isl, const double* cent, const double* Einv, double* zr,
+                    double* coef, Vec3<const T> R, Vec3<T> Zv, hipStream_t s);
 // b = a rounded to bfloat16 (round to nearest even, through binary32)
 void launch_to_bf16(const double* a, bf16* b, size_t n, hipStream_t s);
 template <typename TA, typename TB>
@@ -164,7 +175,10 @@ void launch_pcg_ru_dxr(float* r, const float* q, float* dx, const float* p, doub
 // x64 += dx, dx = 0
 void launch_pcg_flush_x(double* x, float* dx, size_t n, hipStream_t s);
 // streaming kernels' grid size (partials written by the reducing ones)
-constexpr int kStreamBlocks = 2048;
+#ifndef RT_STREAM_BLOCKS
+#define RT_STREAM_BLOCKS 2048
+#endif
+constexpr int kStreamBlocks = RT_STREAM_BLOCKS;
 template <typename T, typename TZ>
 void launch_pcg_update_p(T* p, const TZ* z, double beta, size_t n, hipStream_t s);
 template <typename T, typename TZ>

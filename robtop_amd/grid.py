@@ -243,6 +243,17 @@ class HierarchyGrid:
         check(lib().rt_vcycle(self.handle, pre_relax, post_relax, ctypes.byref(r)))
         return r.value
 
+    def set_island_deflation(self, theta=0.5, min_size=8):
+        """Deflate the rigid-body modes of floating stiff clusters (rho >= theta,
+        >= min_size elements, touching no support) in the preconditioner;
+        theta <= 0 turns it off. Clusters are rebuilt at every update_stencil."""
+        check(lib().rt_set_island_deflation(self.handle, float(theta), int(min_size)))
+
+    def num_islands(self):
+        n = ctypes.c_int()
+        check(lib().rt_num_islands(self.handle, ctypes.byref(n)))
+        return n.value
+
     def set_warm_start(self, m):
         """Start solves from the energy-optimal combination of the last m solutions (0: off)."""
         check(lib().rt_set_warm_start(self.handle, m))
