@@ -198,6 +198,16 @@ __device__ __forceinline__ void gs_node(const Geom& g, int base, const KeC<T>& K
     elem_all<T, C>(g, vb, K, B, acc, KU, S, std::make_integer_sequence<int, 8>{});
 }
 
+// the three divisions of a node update; FP32 optionally by the hardware
+// reciprocal (1 ulp) times the numerator instead of the IEEE division sequence
+#ifndef RT_GS_FAST_DIV
+#define RT_GS_FAST_DIV 0
+#endif
+template <typename T> __device__ __forceinline__ T gs_div(T a, T b) { return a / b; }
+template <> __device__ __forceinline__ float gs_div<float>(float a, float b) {
+    return RT_GS_FAST_DIV ? a * __builtin_amdgcn_rcpf(b) : a / b;
+}
+
 // GS update of the colour-C node with half-index base (Grid.cu:1007-1062)
 // ZS ("zero supports"): the caller guarantees u = 0 at every support node for
 // the whole sweep (f = 0 there, so the relaxation keeps it 0, and the
@@ -222,13 +232,13 @@ __device__ __forceinline__ void gs_update(const Geom& g, int base, Vec3<T> U, Ve
     T u[3] = {U.c[0][v], U.c[1][v], U.c[2][v]};
     // component-wise Gauss-Seidel with the 3x3 self block (Grid.cu:1056-1062)
     if (!reverse) {
-        u[0] = (f[0] - S[1] * u[1] - S[2] * u[2] - KU[0]) / S[0];
-        u[1] = (f[1] - S[3] * u[0] - S[5] * u[2] - KU[1]) / S[4];
-        u[2] = (f[2] - S[6] * u[0] - S[7] * u[1] - KU[2]) / S[8];
+        u[0] = gs_div<T>(f[0] - S[1] * u[1] - S[2] * u[2] - KU[0], S[0]);
+        u[1] = gs_div<T>(f[1] - S[3] * u[0] - S[5] * u[2] - KU[1], S[4]);
+        u[2] = gs_div<T>(f[2] - S[6] * u[0] - S[7] * u[1] - KU[2], S[8]);
     } else {
-        u[2] = (f[2] - S[6] * u[0] - S[7] * u[1] - KU[2]) / S[8];
-        u[1] = (f[1] - S[3] * u[0] - S[5] * u[2] - KU[1]) / S[4];
-        u[0] = (f[0] - S[1] * u[1] - S[2] * u[2] - KU[0]) / S[0];
+        u[2] = gs_div<T>(f[2] - S[6] * u[0] - S[7] * u[1] - KU[2], S[8]);
+        u[1] = gs_div<T>(f[1] - S[3] * u[0] - S[5] * u[2] - KU[1], S[4]);
+        u[0] = gs_div<T>(f[0] - S[1] * u[1] - S[2] * u[2] - KU[0], S[0]);
     }
     U.c[0][v] = u[0]; U.c[1][v] = u[1]; U.c[2][v] = u[2];
 }
@@ -403,13 +413,13 @@ __device__ __forceinline__ void gs_update2(const Geom& g, int base, Vec3<const T
     else gs_node2<T, C, false, FIRST, REV>(g, base, K, B, KU, S);
     T u[3] = {Uin.c[0][v], Uin.c[1][v], Uin.c[2][v]};
     if (!REV) {
-        u[0] = (f[0] - S[1] * u[1] - S[2] * u[2] - KU[0]) / S[0];
-        u[1] = (f[1] - S[3] * u[0] - S[5] * u[2] - KU[1]) / S[4];
-        u[2] = (f[2] - S[6] * u[0] - S[7] * u[1] - KU[2]) / S[8];
+        u[0] = gs_div<T>(f[0] - S[1] * u[1] - S[2] * u[2] - KU[0], S[0]);
+        u[1] = gs_div<T>(f[1] - S[3] * u[0] - S[5] * u[2] - KU[1], S[4]);
+        u[2] = gs_div<T>(f[2] - S[6] * u[0] - S[7] * u[1] - KU[2], S[8]);
     } else {
-        u[2] = (f[2] - S[6] * u[0] - S[7] * u[1] - KU[2]) / S[8];
-        u[1] = (f[1] - S[3] * u[0] - S[5] * u[2] - KU[1]) / S[4];
-        u[0] = (f[0] - S[1] * u[1] - S[2] * u[2] - KU[0]) / S[0];
+        u[2] = gs_div<T>(f[2] - S[6] * u[0] - S[7] * u[1] - KU[2], S[8]);
+        u[1] = gs_div<T>(f[1] - S[3] * u[0] - S[5] * u[2] - KU[1], S[4]);
+        u[0] = gs_div<T>(f[0] - S[1] * u[1] - S[2] * u[2] - KU[0], S[0]);
     }
     Uout.c[0][v] = u[0]; Uout.c[1][v] = u[1]; Uout.c[2][v] = u[2];
 }
