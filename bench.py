@@ -52,6 +52,10 @@ def parse():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the preconditioner kernels one by one instead of replaying a HIP graph "
                          "(use under rocprofv3: its kernel tracing has aborted the queue on graph replays)")
+    ap.add_argument("--learned", type=int, default=0, metavar="TRIPLES",
+                    help="learned warm start: train the network on TRIPLES (density, load, displacement) triples "
+                         "of 256x128x128 SIMP runs, then start every solve from its prediction (0: off)")
+    ap.add_argument("--learned-steps", type=int, default=2000, help="Adam steps (batch 4) of that training")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--vcycles", type=int, default=20, help="graphless V-cycles timed for the roofline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
@@ -291,6 +295,21 @@ def main():
     if a.islands > 0:
         hg.set_island_deflation(a.islands, 8)
     hg.set_graphs(not a.no_graphs)
+    learned = None
+    if a.learned > 0:
+        from robtop_amd.learned import WarmStartNet, collect_simp_samples, heldout_loss
+        net = WarmStartNet(a.seed)
+        t_a = time.perf_counter()
+        seeds = collect_simp_samples(net, triples=a.learned, log=log)
+        t_b = time.perf_counter()
+        train_loss = net.train(a.learned_steps, batch=4, lr=1e-3, seed=a.seed)
+        t_c = time.perf_counter()
+        learned = {"triples": net.num_samples(), "runs": len(seeds), "train_loss": train_loss,
+                   "heldout_loss": heldout_loss(net), "sample_seconds": t_b - t_a, "train_seconds": t_c - t_b,
+                   "adam_steps": a.learned_steps, "taken_steps": 0}
+        log(f"learned: {learned}")
+        net.clear_samples()
+        hg.set_learned_warm_start(net)
     g0 = hg[0]
     ne0, nn0 = hg.num_active(0)
     levels = []
@@ -312,6 +331,8 @@ def main():
         st = hg.solve(tol=a.tol, maxit=500, mode=mode, pre=a.pre, post=a.post)
         c = hg.compliance()
         t2 = time.perf_counter()
+        if learned is not None and record:
+            learned["taken_steps"] += int(hg.learned_warm_start_info()["taken"])
         hg.compute_sensitivity()
         hg.filter_sensitivity(rfilt)
         hg.update_densities(vgoal, step, damp, rmin)
@@ -394,6 +415,8 @@ def main():
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": launch_bytes, "launch_ms": launch_ms},
     }
+    if learned is not None:
+        out["config"]["learned_warm_start"] = learned
     if not a.no_cpu:
         n_v = float(np.mean(iters)) + 1.0
         n_app = float(np.mean(iters)) + 2.0

@@ -304,6 +304,54 @@ int rt_mma_get_x(const rt_mma* h, float* x);
  * subproblem iterations and whether they hit the reference's limit of 80 */
 int rt_mma_stats(const rt_mma* h, float* lambda, float* z, int* sub_iters, int* overflowed);
 
+/* --- learned warm start (no counterpart in the reference) ----------------
+ *
+ * A 3D convolutional encoder-decoder on the first coarse level's (x4) node
+ * box: 3 resolutions with 16 / 32 / 64 channels, two 3x3x3 convolutions per
+ * resolution with ReLU, 2x2x2 average pooling down, nearest upsampling with
+ * additive skips up, and a 3x3x3 output convolution. Input per node: density
+ * (mean over the 8^3 fine elements around it) and the restricted load (3
+ * components, scaled by 1 / max |load|); output: 3 displacement components
+ * (shape only: the solver fits the scale). Weights and activations are
+ * bfloat16, accumulation FP32 on the matrix cores; Adam keeps FP32 master
+ * weights. Parameters are one FP32 vector: for each of the 11 convolutions
+ * (E0A E0B E1A E1B E2A E2B D1A D1B D0A D0B HEAD) the weights [27][cout][cin]
+ * then the biases [cout], with cin / cout padded to multiples of 16 (the
+ * first layer's inputs 4..15 and the last layer's outputs 3..15 are zero). A
+ * tap t holds the offset (t%3-1, t/3%3-1, t/9-1). Not thread-safe per handle. */
+typedef struct rt_net rt_net;
+int rt_net_create(uint64_t seed, rt_net** out);
+int rt_net_destroy(rt_net* n);
+int rt_net_num_params(const rt_net* n, size_t* count);
+int rt_net_get_params(const rt_net* n, float* w);
+/* sets the parameters and resets the optimiser state */
+int rt_net_set_params(rt_net* n, const float* w);
+/* y[P][3] = net(x[P][4]) on a dense box dims = {nx, ny, nz}, P = nx ny nz,
+ * position x + nx (y + ny z); inputs are rounded to bfloat16 */
+int rt_net_forward(rt_net* n, const int* dims, const float* x, float* y);
+/* loss = mean over P and 3 channels of (net(x) - target)^2 and its gradient
+ * with respect to every parameter (layout of rt_net_get_params) */
+int rt_net_gradient(rt_net* n, const int* dims, const float* x, const float* target, double* loss, float* grad);
+/* stores the hierarchy's current (density, load, displacement) as a training
+ * sample, the displacement normalised to unit RMS; call after a solve */
+int rt_net_add_sample(rt_net* n, rt_hier* h);
+/* loss of the prediction for the hierarchy's current triple (not stored) */
+int rt_net_evaluate(rt_net* n, rt_hier* h, double* loss);
+int rt_net_num_samples(const rt_net* n, int* count);
+int rt_net_clear_samples(rt_net* n);
+/* steps Adam updates (beta 0.9 / 0.999), each on the mean loss of `batch`
+ * samples drawn with replacement; mean_loss = the last step's loss */
+int rt_net_train(rt_net* n, int steps, int batch, float lr, uint64_t seed, double* mean_loss);
+/* Starts every rt_solve from the energy-optimal combination of the plain
+ * start (the current displacement, or the rt_set_warm_start combination) and
+ * the prolonged prediction, kept only when it lowers the energy
+ * u.Ku / 2 - f.u (otherwise the plain start stays). net = NULL turns it
+ * off. The network is not owned and must outlive its use. */
+int rt_set_learned_warm_start(rt_hier* h, rt_net* n);
+/* info = {1 if the last solve took the learned start else 0, energy of the
+ * plain start, energy of the learned start, coefficient of the prediction} */
+int rt_learned_warm_start_info(const rt_hier* h, double* info);
+
 /* --- voxelizer ----------------------------------------------------------- */
 
 /* voxelize_mesh (voxelizer/voxelize_main.cpp:344-404): padded cube of side

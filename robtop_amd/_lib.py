@@ -109,6 +109,20 @@ SIGNATURES = {
     "rt_set_island_deflation": ([P, F, I], I),
     "rt_num_islands": ([P, P], I),
     "rt_block_lanczos": ([P, I, I, D, D, I, I, I, ctypes.c_uint64, P, P], I),
+    "rt_net_create": ([ctypes.c_uint64, P], I),
+    "rt_net_destroy": ([P], I),
+    "rt_net_num_params": ([P, P], I),
+    "rt_net_get_params": ([P, P], I),
+    "rt_net_set_params": ([P, P], I),
+    "rt_net_forward": ([P, P, P, P], I),
+    "rt_net_gradient": ([P, P, P, P, P, P], I),
+    "rt_net_add_sample": ([P, P], I),
+    "rt_net_evaluate": ([P, P, P], I),
+    "rt_net_num_samples": ([P, P], I),
+    "rt_net_clear_samples": ([P], I),
+    "rt_net_train": ([P, I, I, F, ctypes.c_uint64, P], I),
+    "rt_set_learned_warm_start": ([P, P], I),
+    "rt_learned_warm_start_info": ([P, P], I),
     "rt_synchronize": ([P], I),
     "rt_time_vcycle": ([P, I, I, I, I, P], I),
     "rt_time_fine_sweep": ([P, I, I, P], I),

@@ -32,9 +32,11 @@ UNITS = {
     "k_simp.hip": [],
     "k_worst.hip": [],
     "k_island.hip": [],
+    "k_net.hip": [],
     "k_mma.hip": ["-ffp-contract=off"],
     "k_voxel.hip": ["-ffp-contract=off"],
     "rt_host.cpp": ["-x", "hip"],
+    "rt_net.cpp": ["-x", "hip"],
     "rt_mma.cpp": ["-x", "hip", "-ffp-contract=off"],
 }
 

@@ -12,6 +12,7 @@
 
 #include "../../include/robtop_amd.h"
 #include "rt_internal.h"
+#include "net_internal.h"
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -22,38 +23,6 @@ static void jacobi_eigen(int n, std::vector<double>& A, std::vector<double>& w, 
 namespace rt {
 
 thread_local std::string g_err;
-
-// ---------------------------------------------------------------------------
-// device buffer
-// ---------------------------------------------------------------------------
-struct DBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DBuf() = default;
-    DBuf(const DBuf&) = delete;
-    DBuf& operator=(const DBuf&) = delete;
-    DBuf(DBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
-    DBuf& operator=(DBuf&& o) noexcept {
-        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
-        return *this;
-    }
-    ~DBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    void alloc(size_t b) {
-        if (b == bytes && p) { RT_HIP(hipMemset(p, 0, b)); return; }
-        release();
-        if (b == 0) return;
-        RT_HIP(hipMalloc(&p, b));
-        RT_HIP(hipMemset(p, 0, b));
-        bytes = b;
-    }
-    void ensure(size_t b) { if (bytes < b) alloc(b); }
-    template <class T> T* as() const { return (T*)p; }
-};
 
 // a 3-component vector: one contiguous buffer of 3N
 template <typename T>
@@ -137,6 +106,11 @@ struct Hier {
     DBuf isl_lab, isl_anch, isl_cnt, isl_root, isl_node, isl_cent, isl_E, isl_Einv, isl_zr, isl_coef, isl_flag,
         isl_KE;
     std::vector<V3<double>> ws_hist; // ring of previous solutions
+    // learned warm start: network (not owned), its prolonged prediction, and
+    // {taken, E(plain start), E(learned start), prediction coefficient} of the last solve
+    Net* net = nullptr;
+    V3<double> nn_u;
+    double nn_info[4] = {0, 0, 0, 0};
     int ws_count = 0, ws_head = 0;
     V3<float> gs_tmp32;              // second buffer of the tiled fine smoother
     V3<double> gs_tmp64;
@@ -819,6 +793,62 @@ static void subspace_push(Hier& H) {
     H.ws_count = std::min(H.ws_count + 1, H.ws_m);
 }
 
+// Learned warm start. The network predicts the displacement on the x4 level
+// from the density and the restricted load; its prolongation u_nn joins the
+// current start x (the previous solution, or the subspace combination) in a
+// Galerkin combination x' = c0 x + c1 u_nn minimising the energy
+// E(y) = y.Ky / 2 - f.y, i.e. the K-norm error. x' replaces x only when
+// E(x') < E(x); otherwise the plain start stays (the plain MG-PCG fallback).
+static void learned_warm_start(Hier& H) {
+    if (!H.net) return;
+    Level& L0 = H.lv[0];
+    Level& C = H.lv[1];
+    const size_t n = (size_t)3 * L0.g.N;
+    LevelVecs<double>& cv = vecs<double>(C);
+    restrict_to<double>(H, 1, H.f64.cv(), cv.F.v());
+    RT_HIP(hipMemsetAsync(cv.U.data(), 0, sizeof(double) * 3 * (size_t)C.g.N, H.s));
+    net_predict(*H.net, L0.g, L0.e, H.rho.as<float>(), C.g, cv.F.cv(), C.vflag.as<uint8_t>(), cv.U.v(), H.s);
+    if (!H.nn_u.ok()) H.nn_u.alloc(L0.g.N);
+    RT_HIP(hipMemsetAsync(H.nn_u.data(), 0, sizeof(double) * n, H.s));
+    prolong_from<double>(H, 0, H.nn_u.v(), cv.U.cv(), true);
+    if (!H.q64.ok()) H.q64.alloc(L0.g.N);
+    const V3<double>* h[2] = {&H.x64, &H.nn_u};
+    double G[4], b[2];
+    for (int j = 0; j < 2; j++) {
+        launch_fine_apply_modal<double>(L0.g, h[j]->cv(), H.q64.v(), H.rhop.as<float>(), L0.vflag.as<uint8_t>(),
+                                        nullptr, nullptr, H.s);
+        for (int i = 0; i <= j; i++) G[i * 2 + j] = G[j * 2 + i] = dot<double, double>(H, h[i]->data(), H.q64.data(), n);
+        b[j] = dot<double, double>(H, h[j]->data(), H.f64.data(), n);
+    }
+    std::vector<double> A(G, G + 4), w, V;
+    jacobi_eigen(2, A, w, V);
+    const double wmax = std::max(w[0], w[1]);
+    double c[2] = {0, 0};
+    for (int e = 0; e < 2 && wmax > 0; e++) {
+        if (!(w[e] > 1e-12 * wmax)) continue;
+        const double vb = V[e] * b[0] + V[2 + e] * b[1];
+        c[0] += V[e] * vb / w[e];
+        c[1] += V[2 + e] * vb / w[e];
+    }
+    const double e_plain = 0.5 * G[0] - b[0];
+    const double e_learn = 0.5 * (c[0] * c[0] * G[0] + 2 * c[0] * c[1] * G[1] + c[1] * c[1] * G[3]) - c[0] * b[0] -
+                           c[1] * b[1];
+    const bool take = std::isfinite(e_learn) && e_learn < e_plain;
+    H.nn_info[0] = take ? 1.0 : 0.0;
+    H.nn_info[1] = e_plain;
+    H.nn_info[2] = e_learn;
+    H.nn_info[3] = c[1];
+    if (!take) return;
+    const double* hp[2] = {H.x64.data(), H.nn_u.data()};
+    launch_lincomb(H.q64.data(), hp, c, 2, n, H.s);
+    RT_HIP(hipMemcpyAsync(H.x64.data(), H.q64.data(), sizeof(double) * n, hipMemcpyDeviceToDevice, H.s));
+}
+
+static void warm_start(Hier& H) {
+    subspace_warm_start(H);
+    learned_warm_start(H);
+}
+
 template <typename TV, typename TC>
 static int pcg_core(Hier& H, V3<TV>& x, V3<TV>& b, double tol, int maxit, int pre, int post, double* rel_out) {
     Level& L0 = H.lv[0];
@@ -883,7 +913,7 @@ static void pcg(Hier& H, double tol, int maxit, int pre, int post, rt_solve_stat
     V3<TV>& b = KV<TV>::b(H);
     if (!x.ok()) x.alloc(N);
     if (!b.ok()) b.alloc(N);
-    subspace_warm_start(H);
+    warm_start(H);
     if ((void*)x.data() != (void*)H.x64.data()) {
         launch_convert<double, TV>(H.x64.data(), x.data(), n, H.s);
         launch_convert<double, TV>(H.f64.data(), b.data(), n, H.s);
@@ -914,6 +944,7 @@ static void pcg_ir(Hier& H, double tol, int maxit, int pre, int post, double inn
     if (!H.x32.ok()) H.x32.alloc(N);
     if (!H.b32.ok()) H.b32.alloc(N);
     int total = 0;
+    warm_start(H);
     double rel = true_rel_residual(H), rel_in = 0;   // leaves r64 = f - K x64
     for (int outer = 0; outer < 50 && rel > tol && total < maxit; outer++) {
         launch_fill<float>(H.x32.data(), 0.f, n, H.s);
@@ -951,7 +982,7 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
     if (!H.dx32.ok()) H.dx32.alloc(N);
     LevelVecs<float>& pc = pcvecs<float>(H);   // pc.F holds the FP32 residual, pc.U the preconditioned one
     const int np = 1024;
-    subspace_warm_start(H);
+    warm_start(H);
     const double xAx = apply_dot<double>(H, H.x64, H.q64);
     if (xAx > 0) {
         const double xb = dot<double, double>(H, H.x64.data(), H.f64.data(), n);
@@ -1506,6 +1537,54 @@ int rt_set_warm_start(rt_hier* h, int m) {
         H.ws_m = m;
         H.ws_hist.clear();
         H.ws_count = H.ws_head = 0;
+    })
+}
+
+static void need_x4_level(const Hier& H) {
+    if (H.lv.size() < 2 || H.lv[1].scale != 4)
+        throw std::invalid_argument("the learned warm start needs an x4 coarse level");
+}
+
+int rt_set_learned_warm_start(rt_hier* h, rt_net* net) {
+    RT_TRY({
+        Hier& H = H_(h);
+        if (net && !net->n) throw std::invalid_argument("null rt_net handle");
+        if (net) need_x4_level(H);
+        H.net = net ? net->n : nullptr;
+        if (!H.net) H.nn_u.b.release();
+        for (double& v : H.nn_info) v = 0;
+    })
+}
+
+int rt_learned_warm_start_info(const rt_hier* h, double* info) {
+    RT_TRY({
+        need(info);
+        const Hier& H = H_(h);
+        for (int k = 0; k < 4; k++) info[k] = H.nn_info[k];
+    })
+}
+
+static double net_sample(rt_net* net, rt_hier* h, bool store) {
+    Hier& H = H_(h);
+    if (!net || !net->n) throw std::invalid_argument("null rt_net handle");
+    need_x4_level(H);
+    Level& L0 = H.lv[0];
+    Level& C = H.lv[1];
+    LevelVecs<double>& cv = vecs<double>(C);
+    restrict_to<double>(H, 1, H.f64.cv(), cv.F.v());
+    if (store) {
+        net_add_sample(*net->n, L0.g, L0.e, H.rho.as<float>(), C.g, cv.F.cv(), H.x64.cv(), H.s);
+        return 0.0;
+    }
+    return net_evaluate(*net->n, L0.g, L0.e, H.rho.as<float>(), C.g, cv.F.cv(), H.x64.cv(), H.s);
+}
+
+int rt_net_add_sample(rt_net* net, rt_hier* h) { RT_TRY({ net_sample(net, h, true); }) }
+
+int rt_net_evaluate(rt_net* net, rt_hier* h, double* loss) {
+    RT_TRY({
+        need(loss);
+        *loss = net_sample(net, h, false);
     })
 }
 

@@ -238,4 +238,36 @@ int status_of_current_exception();
                                      ":" + std::to_string(__LINE__));                                  \
     } while (0)
 
+// ---------------------------------------------------------------------------
+// device buffer
+// ---------------------------------------------------------------------------
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t b) {
+        if (b == bytes && p) { RT_HIP(hipMemset(p, 0, b)); return; }
+        release();
+        if (b == 0) return;
+        RT_HIP(hipMalloc(&p, b));
+        RT_HIP(hipMemset(p, 0, b));
+        bytes = b;
+    }
+    void ensure(size_t b) { if (bytes < b) alloc(b); }
+    template <class T> T* as() const { return (T*)p; }
+};
+
 }  // namespace rt

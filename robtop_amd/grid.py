@@ -258,6 +258,20 @@ class HierarchyGrid:
         """Start solves from the energy-optimal combination of the last m solutions (0: off)."""
         check(lib().rt_set_warm_start(self.handle, m))
 
+    def set_learned_warm_start(self, net):
+        """Start solves from the energy-optimal combination of the plain start
+        and the network's prediction (a robtop_amd.learned.WarmStartNet, or
+        None to turn it off). The network must stay alive while it is set."""
+        self._net = net
+        check(lib().rt_set_learned_warm_start(self.handle, net.handle if net is not None else None))
+
+    def learned_warm_start_info(self):
+        """{taken, energy of the plain start, energy of the learned start,
+        coefficient of the prediction} of the last solve."""
+        a = np.zeros(4, np.float64)
+        check(lib().rt_learned_warm_start_info(self.handle, ptr(a)))
+        return {"taken": bool(a[0]), "e_plain": float(a[1]), "e_learned": float(a[2]), "coef": float(a[3])}
+
     def set_fine_smoother(self, kind):
         """Finest-level sweep: 0 = four colour-pair launches (default), 1 = two tiled launches."""
         check(lib().rt_set_fine_smoother(self.handle, kind))
