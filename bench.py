@@ -56,6 +56,8 @@ def parse():
                     help="learned warm start: train the network on TRIPLES (density, load, displacement) triples "
                          "of 256x128x128 SIMP runs, then start every solve from its prediction (0: off)")
     ap.add_argument("--learned-steps", type=int, default=2000, help="Adam steps (batch 4) of that training")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of timing the alternatives before the run")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--vcycles", type=int, default=20, help="graphless V-cycles timed for the roofline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
@@ -307,9 +309,23 @@ def main():
         learned = {"triples": net.num_samples(), "runs": len(seeds), "train_loss": train_loss,
                    "heldout_loss": heldout_loss(net), "sample_seconds": t_b - t_a, "train_seconds": t_c - t_b,
                    "adam_steps": a.learned_steps, "taken_steps": 0}
+        from robtop_amd.learned import forward_flops
+        dims = tuple(int(v) for v in hg[1].nodes)
+        fwd_ms = net.time_forward(dims, 20)
+        learned["forward"] = {"box": dims, "ms": fwd_ms, "bound": "mfma",
+                              "achieved": forward_flops(dims) / (fwd_ms * 1e-3) / 1e12,
+                              "achieved_real_channels": forward_flops(dims, padded=False) / (fwd_ms * 1e-3) / 1e12,
+                              "peak": 2500.0, "unit": "TFLOP/s"}
+        learned["forward"]["frac"] = learned["forward"]["achieved"] / 2500.0
         log(f"learned: {learned}")
         net.clear_samples()
         hg.set_learned_warm_start(net)
+    tuned = None
+    if not a.no_autotune:
+        hg.update_stencil()
+        tuned = hg.autotune(reps=5)
+        log(f"autotune: {tuned}")
+        a.fine_smoother = "tiled" if tuned["fine_smoother"] == 1 else "pairs"
     g0 = hg[0]
     ne0, nn0 = hg.num_active(0)
     levels = []
@@ -415,6 +431,8 @@ def main():
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": launch_bytes, "launch_ms": launch_ms},
     }
+    if tuned is not None:
+        out["config"]["autotune"] = tuned
     if learned is not None:
         out["config"]["learned_warm_start"] = learned
     if not a.no_cpu:

@@ -342,6 +342,8 @@ int rt_net_clear_samples(rt_net* n);
 /* steps Adam updates (beta 0.9 / 0.999), each on the mean loss of `batch`
  * samples drawn with replacement; mean_loss = the last step's loss */
 int rt_net_train(rt_net* n, int steps, int batch, float lr, uint64_t seed, double* mean_loss);
+/* mean milliseconds of one forward pass on a dims box (HIP events, n passes) */
+int rt_net_time_forward(rt_net* n, const int* dims, int reps, double* ms);
 /* Starts every rt_solve from the energy-optimal combination of the plain
  * start (the current displacement, or the rt_set_warm_start combination) and
  * the prolonged prediction, kept only when it lowers the energy
@@ -360,6 +362,19 @@ int rt_learned_warm_start_info(const rt_hier* h, double* info);
  * max[3]}. */
 int rt_voxelize_mesh(const float* coords, int nverts, const int* faces, int nfaces, int prefer_reso,
                      uint32_t* bits, size_t maxwords, int* out_reso, float* out_box);
+
+/* Autotuner (no counterpart in the reference): on this hierarchy's own sizes,
+ * times each launch-shape alternative `reps` times with HIP events and keeps
+ * the fastest: z planes per workgroup of the finest residual / apply (8 .. 48),
+ * the finest smoother implementation (rt_set_fine_smoother), and per coarse
+ * level the sliced (nine waves per 64 nodes) or thread-per-node smoother and
+ * residual kernels. The operator is the same; only the summation order of a
+ * coarse node's 27 neighbour blocks differs between the two coarse kernel
+ * forms (the finest-level choices are bit-identical). Needs rt_update_stencil first; uses the current force and
+ * displacement as data. info[6] = {z planes, ms of residual + apply, smoother
+ * kind, ms per sweep, mask of coarse levels with the sliced smoother, mask of
+ * coarse levels with the sliced residual} (bit l = level l). */
+int rt_autotune(rt_hier* h, int reps, double* info);
 
 /* --- timing hooks (benchmarking) ----------------------------------------- */
 

@@ -118,11 +118,13 @@ SIGNATURES = {
     "rt_net_gradient": ([P, P, P, P, P, P], I),
     "rt_net_add_sample": ([P, P], I),
     "rt_net_evaluate": ([P, P, P], I),
+    "rt_net_time_forward": ([P, P, I, P], I),
     "rt_net_num_samples": ([P, P], I),
     "rt_net_clear_samples": ([P], I),
     "rt_net_train": ([P, I, I, F, ctypes.c_uint64, P], I),
     "rt_set_learned_warm_start": ([P, P], I),
     "rt_learned_warm_start_info": ([P, P], I),
+    "rt_autotune": ([P, I, P], I),
     "rt_synchronize": ([P], I),
     "rt_time_vcycle": ([P, I, I, I, I, P], I),
     "rt_time_fine_sweep": ([P, I, I, P], I),

@@ -211,14 +211,11 @@ __global__ void __launch_bounds__(576) k_coarse_gs9(Geom g, Vec3<T> U, Vec3<cons
     U.c[0][v] = u[0]; U.c[1][v] = u[1]; U.c[2][v] = u[2];
 }
 
-#ifndef RT_COARSE_SLICED
-#define RT_COARSE_SLICED 1
-#endif
 
 template <typename T, typename TS>
 void launch_coarse_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const TS* st, const uint8_t* vflag, int colour,
                       bool reverse, hipStream_t s) {
-    if (RT_COARSE_SLICED) {
+    if (g.coarse_gs) {
         unsigned nb = colour_blocks(g, colour, 64);
         if (!nb) return;
         switch (colour) {
@@ -327,7 +324,8 @@ void launch_coarse_residual(const Geom& g, Vec3<const T> U, Vec3<const T> F, Vec
     // sliced on the small levels only: on a level with many nodes per colour
     // (L2 at 511 x 256 x 256: 68K) one thread per node streams the stencil faster
     // (115 us against 145 us there)
-    if (RT_COARSE_SLICED && colour_blocks(g, 0, 1) < 32768) {
+    const bool sliced = g.coarse_res < 0 ? colour_blocks(g, 0, 1) < 32768 : g.coarse_res != 0;
+    if (sliced) {
         dim3 grid(colour_blocks(g, 0, 64), 8);
         hipLaunchKernelGGL((k_coarse_res9<T, TS>), grid, dim3(576), 0, s, g, U, F, R, st, vflag);
     } else {

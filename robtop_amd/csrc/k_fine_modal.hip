@@ -9,7 +9,7 @@
 // 576 FMA for KE u directly.
 //
 // Tiling: a workgroup owns a 31 x 7 column of nodes and marches through a
-// chunk of RT_MODAL_ZC z planes. Each step loads one node plane (33 x 9 with halo) into
+// chunk of Geom::zc z planes. Each step loads one node plane (33 x 9 with halo) into
 // LDS, computes the 32 x 8 elements between the two resident planes (one per
 // thread), stores their 24 vertex forces in LDS, and every node thread adds the
 // four elements below and above it. A node plane is finished when both element
@@ -27,12 +27,10 @@ namespace rt {
 #ifndef RT_MODAL_TX
 #define RT_MODAL_TX 31
 #endif
-// z planes per workgroup. Measured at 511 x 256 x 256 (ms per residual): 8
-// planes 0.81, 12 0.80, 16 0.81, 24 0.79, 32 0.86, 64 1.07; a 15 x 15 node
-// tile instead of 31 x 7: 0.95.
-#ifndef RT_MODAL_ZC
-#define RT_MODAL_ZC 24
-#endif
+// z planes per workgroup (Geom::zc, default 24, chosen per hierarchy by the
+// autotuner). Measured at 511 x 256 x 256 (ms per residual): 8 planes 0.81,
+// 12 0.80, 16 0.81, 24 0.79, 32 0.86, 64 1.07; a 15 x 15 node tile instead of
+// 31 x 7: 0.95.
 
 namespace {
 constexpr int TX = RT_MODAL_TX;            // node tile: TX x TY with (TX + 1)(TY + 1) = 256
@@ -40,7 +38,6 @@ constexpr int TY = 256 / (TX + 1) - 1;
 constexpr int EX = TX + 1, EY = TY + 1;    // element tile (= 256 threads)
 constexpr int PX = TX + 2, PY = TY + 2;    // node plane with halo
 constexpr int PLANE = PX * PY;
-constexpr int ZC = RT_MODAL_ZC;            // z planes per workgroup
 static_assert(EX * EY == 256 && PLANE <= 512, "tile must fill 256 threads; a plane is staged by two points per thread");
 }
 
@@ -116,8 +113,8 @@ __global__ void __launch_bounds__(256) k_fine_res_modal(Geom g, Vec3<const T> U,
     const int gx = gridDim.x, gy = gridDim.y;
     const int bl = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
     const int bm = RT_MODAL_XCD ? xcd_row(bl, gx * gy * (int)gridDim.z) : bl;
-    const int x0 = (bm % gx) * TX, y0 = ((bm / gx) % gy) * TY, z0 = (bm / (gx * gy)) * ZC;
-    const int z1 = min(z0 + ZC, nz);
+    const int x0 = (bm % gx) * TX, y0 = ((bm / gx) % gy) * TY, z0 = (bm / (gx * gy)) * g.zc;
+    const int z1 = min(z0 + g.zc, nz);
     // element owned by this thread (tile-local i, j -> global x0 - 1 + i, y0 - 1 + j)
     const int ei = tid % EX, ej = tid / EX;
     const int gex = x0 - 1 + ei, gey = y0 - 1 + ej;
@@ -284,7 +281,8 @@ void set_modal_constants(const double* ke576) {
 }
 
 static dim3 modal_grid(const Geom& g) {
-    return dim3((g.n[0] + TX - 1) / TX, (g.n[1] + TY - 1) / TY, (g.n[2] + ZC - 1) / ZC);
+    if (g.zc < 1) throw std::invalid_argument("z planes per workgroup must be positive");
+    return dim3((g.n[0] + TX - 1) / TX, (g.n[1] + TY - 1) / TY, (g.n[2] + g.zc - 1) / g.zc);
 }
 
 int fine_apply_partials_modal(const Geom& g) {

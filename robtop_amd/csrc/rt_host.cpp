@@ -2448,4 +2448,107 @@ int rt_time_fine_residual(rt_hier* h, int bits, int n, double* ms) {
     })
 }
 
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// autotuner: times the alternatives of each level's launch shape on this
+// hierarchy's own sizes (FP32, the preconditioner's precision) and keeps the
+// fastest
+// ---------------------------------------------------------------------------
+template <typename F>
+static double time_ms(Hier& H, int reps, F&& body) {
+    hipEvent_t a, b;
+    RT_HIP(hipEventCreate(&a));
+    RT_HIP(hipEventCreate(&b));
+    body();   // warm-up
+    RT_HIP(hipEventRecord(a, H.s));
+    for (int i = 0; i < reps; i++) body();
+    RT_HIP(hipEventRecord(b, H.s));
+    RT_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    RT_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms / reps;
+}
+
+// info[0..4] = {z planes per fine residual workgroup, ms of residual + apply,
+// fine smoother kind, ms per sweep, bit l set: coarse level l uses the sliced
+// smoother}, info[5] bit l: sliced coarse residual
+static void autotune(Hier& H, int reps, double* info) {
+    if (!H.stencil_ready) throw std::logic_error("update_stencil has not been called");
+    Level& L0 = H.lv[0];
+    LevelVecs<float>& pc = pcvecs<float>(H);
+    const size_t nn = (size_t)3 * L0.g.N;
+    launch_convert<double, float>(H.f64.data(), pc.F.data(), nn, H.s);
+    launch_convert<double, float>(H.x64.data(), pc.U.data(), nn, H.s);
+    // finest residual / apply: z planes per workgroup
+    double best = 1e30;
+    int best_zc = L0.g.zc;
+    for (int zc : {8, 12, 16, 24, 32, 48}) {
+        L0.g.zc = zc;
+        if (fine_apply_partials_modal(L0.g) > H.nparts_max) continue;
+        const double t = time_ms(H, reps, [&] {
+            launch_fine_residual_modal<float>(L0.g, pc.U.cv(), pc.F.cv(), pc.R.v(), H.rhop.as<float>(),
+                                              L0.vflag.as<uint8_t>(), H.s);
+            int np = 0;
+            launch_fine_apply_modal<float>(L0.g, pc.U.cv(), pc.R.v(), H.rhop.as<float>(), L0.vflag.as<uint8_t>(),
+                                           H.partials.as<double>(), &np, H.s);
+        });
+        if (t < best) { best = t; best_zc = zc; }
+    }
+    L0.g.zc = best_zc;
+    info[0] = best_zc;
+    info[1] = best;
+    // finest smoother implementation
+    best = 1e30;
+    int best_kind = H.fine_smoother;
+    for (int kind : {0, 1}) {
+        H.fine_smoother = kind;
+        const double t = time_ms(H, std::max(1, reps / 2), [&] { gs<float>(H, 0, pc.U.v(), pc.F.cv(), 1, false, true); });
+        if (t < best) { best = t; best_kind = kind; }
+    }
+    H.fine_smoother = best_kind;
+    info[2] = best_kind;
+    info[3] = best;
+    // coarse levels (not the coarsest): sliced or thread-per-node smoother and residual
+    int gs_mask = 0, res_mask = 0;
+    for (size_t li = 1; li + 1 < H.lv.size(); li++) {
+        Level& L = H.lv[li];
+        LevelVecs<float>& v = vecs<float>(L);
+        double tb = 1e30;
+        int kb = 1;
+        for (int k : {1, 0}) {
+            L.g.coarse_gs = k;
+            const double t = time_ms(H, reps, [&] { gs<float>(H, (int)li, v.U.v(), v.F.cv(), 1, false); });
+            if (t < tb) { tb = t; kb = k; }
+        }
+        L.g.coarse_gs = kb;
+        gs_mask |= kb << li;
+        tb = 1e30;
+        kb = 1;
+        for (int k : {1, 0}) {
+            L.g.coarse_res = k;
+            const double t = time_ms(H, reps, [&] { residual<float>(H, (int)li, v.U.cv(), v.F.cv(), v.R.v()); });
+            if (t < tb) { tb = t; kb = k; }
+        }
+        L.g.coarse_res = kb;
+        res_mask |= kb << li;
+    }
+    info[4] = gs_mask;
+    info[5] = res_mask;
+    H.drop_graph();
+    RT_HIP(hipStreamSynchronize(H.s));
+}
+
+extern "C" {
+
+int rt_autotune(rt_hier* h, int reps, double* info) {
+    RT_TRY({
+        need(info);
+        if (reps < 1) throw std::invalid_argument("reps must be positive");
+        autotune(H_(h), reps, info);
+    })
+}
 }  // extern "C"

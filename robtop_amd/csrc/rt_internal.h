@@ -38,6 +38,10 @@ struct Geom {
     int h[3];           // padded half dims, h = (n + 5) / 2
     int cs;             // colour stride h0*h1*h2
     int N;              // 8*cs slots
+    // launch shapes (chosen per level by the autotuner, rt_autotune):
+    int zc = 24;        // finest level: z planes marched per residual / apply workgroup
+    int coarse_gs = 1;  // coarse smoother: 1 = nine waves per 64 nodes (sliced), 0 = a thread per node
+    int coarse_res = -1;// coarse residual: 1 sliced, 0 a thread per node, -1 = by level size
 };
 
 inline Geom make_geom(int nx, int ny, int nz) {

@@ -386,6 +386,28 @@ int rt_net_gradient(rt_net* n, const int* dims, const float* x, const float* tar
     })
 }
 
+int rt_net_time_forward(rt_net* n, const int* dims, int reps, double* ms) {
+    RT_NET_TRY({
+        need_ptr(dims); need_ptr(ms);
+        if (reps < 1) throw std::invalid_argument("reps must be positive");
+        Net& N = N_(n);
+        shape(N, dims);
+        forward(N, N.s);   // warm-up (inputs: whatever the activation buffers hold)
+        hipEvent_t e0, e1;
+        RT_HIP(hipEventCreate(&e0));
+        RT_HIP(hipEventCreate(&e1));
+        RT_HIP(hipEventRecord(e0, N.s));
+        for (int r = 0; r < reps; r++) forward(N, N.s);
+        RT_HIP(hipEventRecord(e1, N.s));
+        RT_HIP(hipEventSynchronize(e1));
+        float t = 0;
+        RT_HIP(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ms = t / reps;
+    })
+}
+
 int rt_net_num_samples(const rt_net* n, int* count) {
     RT_NET_TRY({
         need_ptr(count);

@@ -272,6 +272,15 @@ class HierarchyGrid:
         check(lib().rt_learned_warm_start_info(self.handle, ptr(a)))
         return {"taken": bool(a[0]), "e_plain": float(a[1]), "e_learned": float(a[2]), "coef": float(a[3])}
 
+    def autotune(self, reps=10):
+        """Time each level's launch-shape alternatives and keep the fastest
+        (same operator; coarse kernels may sum in another order); returns the choices."""
+        a = np.zeros(6, np.float64)
+        check(lib().rt_autotune(self.handle, reps, ptr(a)))
+        return {"fine_zc": int(a[0]), "fine_res_apply_ms": float(a[1]), "fine_smoother": int(a[2]),
+                "fine_sweep_ms": float(a[3]), "coarse_sliced_gs_mask": int(a[4]),
+                "coarse_sliced_res_mask": int(a[5])}
+
     def set_fine_smoother(self, kind):
         """Finest-level sweep: 0 = four colour-pair launches (default), 1 = two tiled launches."""
         check(lib().rt_set_fine_smoother(self.handle, kind))

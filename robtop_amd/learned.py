@@ -94,6 +94,13 @@ class WarmStartNet:
         check(lib().rt_net_evaluate(self.handle, hg.handle, ctypes.byref(loss)))
         return loss.value
 
+    def time_forward(self, dims_xyz, reps=20):
+        """Mean ms of one forward pass on an (nx, ny, nz) box."""
+        dims = np.array(dims_xyz, np.int32)
+        ms = ctypes.c_double()
+        check(lib().rt_net_time_forward(self.handle, ptr(dims), reps, ctypes.byref(ms)))
+        return ms.value
+
     def num_samples(self):
         n = ctypes.c_int()
         check(lib().rt_net_num_samples(self.handle, ctypes.byref(n)))
@@ -114,6 +121,20 @@ class WarmStartNet:
 
     def load(self, path):
         self.set_params(np.load(path, allow_pickle=False))
+
+
+def forward_flops(dims_xyz, padded=True):
+    """Multiply-add flops (2 per MAC) of one forward pass on an (nx, ny, nz)
+    box: every 3x3x3 convolution at its resolution, with the channel counts
+    the matrix cores process (padded) or the real ones."""
+    d0 = list(dims_xyz)
+    d1 = [(n + 1) // 2 for n in d0]
+    d2 = [(n + 1) // 2 for n in d1]
+    P = [int(np.prod(d)) for d in (d0, d1, d2)]
+    lev = (0, 0, 1, 1, 2, 2, 1, 1, 0, 0, 0)
+    cin = CIN if padded else (4,) + CIN[1:]
+    cout = COUT if padded else COUT[:-1] + (3,)
+    return float(sum(2 * 27 * ci * co * P[l] for ci, co, l in zip(cin, cout, lev)))
 
 
 def cantilever_hierarchy(ex, ey, ez, seed):
