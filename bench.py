@@ -56,6 +56,9 @@ def parse():
                     help="learned warm start: train the network on TRIPLES (density, load, displacement) triples "
                          "of 256x128x128 SIMP runs, then start every solve from its prediction (0: off)")
     ap.add_argument("--learned-steps", type=int, default=2000, help="Adam steps (batch 4) of that training")
+    ap.add_argument("--net-forward", default=None, metavar="NX,NY,NZ",
+                    help="only time the learned warm start's network forward pass on this box (e.g. 129,65,65, "
+                         "the x4 level of 511x256x256) and print its MFMA roofline")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes instead of timing the alternatives before the run")
     ap.add_argument("--seed", type=int, default=0)
@@ -270,6 +273,24 @@ def torus_run(a):
     print(json.dumps(out), flush=True)
 
 
+def net_forward_run(a):
+    """Network forward pass alone (random weights; the kernels' speed does not
+    depend on the values): mean ms over --steps passes after --warmup, HIP events."""
+    from robtop_amd.learned import WarmStartNet, forward_flops
+    dims = tuple(int(v) for v in a.net_forward.split(","))
+    net = WarmStartNet(a.seed)
+    net.time_forward(dims, max(1, a.warmup))
+    ms = net.time_forward(dims, max(1, a.steps))
+    fl = forward_flops(dims)
+    out = {"metric": "learned warm start network forward pass", "value": ms, "unit": "ms", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "higher_is_better": False, "dtype": "bf16 (fp32 accumulate)",
+           "data": "synthetic", "config": {"workload": f"3-level conv encoder-decoder on a {dims} node box"},
+           "roofline": {"bound": "mfma", "achieved": fl / (ms * 1e-3) / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
+                        "frac": fl / (ms * 1e-3) / 1e12 / 2500.0, "flops_per_pass": fl}}
+    net.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.pmc_summary:
@@ -283,6 +304,9 @@ def main():
         return
     if a.workload == "torus":
         torus_run(a)
+        return
+    if a.net_forward:
+        net_forward_run(a)
         return
     ex, ey, ez = (int(v) for v in a.size.split(","))
     import robtop_amd as rt
