@@ -243,6 +243,13 @@ def torus_run(a):
     hg.set_precision(32)
     hg.set_preconditioner(a.coarse_mult, a.gamma)
     log(f"hierarchy: {hg.n_levels} levels, {nl} load nodes")
+    tuned = None
+    if not a.no_autotune:
+        eact, _ = hg[0].masks()
+        hg.set_density(np.where(eact, 0.3, 0.0).astype(np.float32))
+        hg.update_stencil()
+        tuned = hg.autotune(reps=5)
+        log(f"autotune: {tuned}")
     params = rt.Parameter(volume_ratio=0.3, volume_decrease=0.05)
     lz = dict(k=8, max_blocks=20, tol=1e-6, solve_tol=1e-8, pre=a.pre, post=a.post)
     recs = rt.worst_case_topopt(hg, params, n_iter=a.warmup + a.steps, stop=False, method="lanczos", lanczos=lz,
@@ -264,7 +271,8 @@ def torus_run(a):
         "config": {"workload": f"torus R=0.35 r=0.12 voxelized at {n}^3, supports on the bottom 5 %, normal loads on "
                                f"{nl} surface nodes, block Lanczos k=8 (tol 1e-6, solves to 1e-8), filter r=2, OC",
                    "active_elements": ne, "active_nodes": nn, "levels": hg.n_levels,
-                   "lanczos_per_iter": [{k: r[k] for k in ("blocks", "solves", "pcg_iterations")} for r in timed]},
+                   "lanczos_per_iter": [{k: r[k] for k in ("blocks", "solves", "pcg_iterations")} for r in timed],
+                   "autotune": tuned},
         "eigen_check": {"worst_compliance_fp32_solves": c32, "worst_compliance_fp64_solves": c64,
                         "rel_diff": abs(c32 - c64) / abs(c64), "fp64_blocks": st64["blocks"],
                         "fp64_solves": st64["solves"]},

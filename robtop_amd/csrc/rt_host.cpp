@@ -1987,55 +1987,75 @@ static void jacobi_eigen(int n, std::vector<double>& A, std::vector<double>& w, 
     for (int i = 0; i < n; i++) w[i] = A[(size_t)i * n + i];
 }
 
-// Orthonormalise the k columns of W (column j = W[j], each of length m)
-// against the columns of Q (all previous blocks) and among themselves, by
-// modified Gram-Schmidt applied twice. R receives the k x k coefficients of W
-// in the new basis (upper triangular); columns that vanish are replaced by
-// zero (a rank-deficient block) and reported through the return value.
-static int block_orth(std::vector<std::vector<double>>& W, const std::vector<std::vector<double>>& Q,
-                      std::vector<double>& R) {
-    const int k = (int)W.size();
-    const size_t m = W.empty() ? 0 : W[0].size();
-    auto dotv = [m](const std::vector<double>& a, const std::vector<double>& b) {
-        double s = 0;
-        for (size_t i = 0; i < m; i++) s += a[i] * b[i];
-        return s;
-    };
+// ---------------------------------------------------------------------------
+// block Lanczos on the GPU: the Krylov basis is an m x (max_blocks + 1) k
+// column-major FP64 matrix in device memory (m = 3 x load nodes); projections
+// against it are rocBLAS GEMMs on the handle's stream; the block tridiagonal T
+// and its eigenproblem ((j + 1) k square) stay on the host.
+// ---------------------------------------------------------------------------
+static void gemm(Hier& H, rocblas_operation ta, rocblas_operation tb, int M, int N, int K, double alpha,
+                 const double* A, int lda, const double* B, int ldb, double beta, double* C, int ldc) {
+    if (rocblas_dgemm(H.blas, ta, tb, M, N, K, &alpha, A, lda, B, ldb, &beta, C, ldc) != rocblas_status_success)
+        throw std::runtime_error("rocblas_dgemm failed");
+}
+static double ddot(Hier& H, int n, const double* x, const double* y) {
+    double r = 0;
+    if (rocblas_ddot(H.blas, n, x, 1, y, 1, &r) != rocblas_status_success) throw std::runtime_error("rocblas_ddot failed");
+    return r;
+}
+static void daxpy(Hier& H, int n, double a, const double* x, double* y) {
+    if (rocblas_daxpy(H.blas, n, &a, x, 1, y, 1) != rocblas_status_success) throw std::runtime_error("rocblas_daxpy failed");
+}
+static void dscal(Hier& H, int n, double a, double* x) {
+    if (rocblas_dscal(H.blas, n, &a, x, 1) != rocblas_status_success) throw std::runtime_error("rocblas_dscal failed");
+}
+
+// Orthonormalises the k columns of W (m x k, device) against the first nq
+// columns of Q (two classical Gram-Schmidt passes as GEMMs), then among
+// themselves (modified Gram-Schmidt, two passes). W_in = Q C + W_out R with R
+// upper triangular (k x k, row-major, host); a column that vanishes (norm
+// below 1e-12 of its norm on entry) is zeroed with R_jj = 0. Returns the rank.
+static int block_orth_dev(Hier& H, int m, int k, double* W, const double* Q, int nq, double* C,
+                          std::vector<double>& R, std::vector<char>& zero) {
+    std::vector<double> n0(k);
+    for (int j = 0; j < k; j++) n0[j] = std::sqrt(std::max(0.0, ddot(H, m, W + (size_t)j * m, W + (size_t)j * m)));
+    if (nq > 0)
+        for (int pass = 0; pass < 2; pass++) {
+            gemm(H, rocblas_operation_transpose, rocblas_operation_none, nq, k, m, 1.0, Q, m, W, m, 0.0, C, nq);
+            gemm(H, rocblas_operation_none, rocblas_operation_none, m, k, nq, -1.0, Q, m, C, nq, 1.0, W, m);
+        }
     R.assign((size_t)k * k, 0.0);
+    zero.assign(k, 0);
     int rank = 0;
     for (int j = 0; j < k; j++) {
-        const double n0 = std::sqrt(dotv(W[j], W[j]));
-        for (int pass = 0; pass < 2; pass++) {
-            for (const auto& q : Q) {
-                const double c = dotv(q, W[j]);
-                for (size_t i = 0; i < m; i++) W[j][i] -= c * q[i];
-            }
+        double* wj = W + (size_t)j * m;
+        for (int pass = 0; pass < 2; pass++)
             for (int i = 0; i < j; i++) {
-                const double c = dotv(W[i], W[j]);
+                if (zero[i]) continue;
+                const double c = ddot(H, m, W + (size_t)i * m, wj);
                 R[(size_t)i * k + j] += c;
-                for (size_t t = 0; t < m; t++) W[j][t] -= c * W[i][t];
+                daxpy(H, m, -c, W + (size_t)i * m, wj);
             }
-        }
-        const double nj = std::sqrt(dotv(W[j], W[j]));
-        if (nj <= 1e-12 * std::max(n0, 1e-300)) {
-            std::fill(W[j].begin(), W[j].end(), 0.0);
-            R[(size_t)j * k + j] = 0;
+        const double nj = std::sqrt(std::max(0.0, ddot(H, m, wj, wj)));
+        if (nj <= 1e-12 * std::max(n0[j], 1e-300)) {
+            dscal(H, m, 0.0, wj);
+            zero[j] = 1;
             continue;
         }
         R[(size_t)j * k + j] = nj;
-        for (size_t i = 0; i < m; i++) W[j][i] /= nj;
+        dscal(H, m, 1.0 / nj, wj);
         rank++;
     }
     return rank;
 }
 
-// y = C q = P N^T K^-1 N q for a compact load vector q ([3][nl] SoA, in the
-// range of P): force = q at the load nodes, a PCG solve from zero to
-// solve_tol, then the projected gather of the displacement.
-static void worst_apply(Hier& H, const std::vector<double>& q, std::vector<double>& y, double solve_tol, int mode,
-                        int pre, int post, int* iters) {
+// y = C q = P N^T K^-1 N q for a compact load vector q ([3][nl] SoA, device,
+// in the range of P): force = q at the load nodes, a PCG solve from zero to
+// solve_tol, then the projected gather of the displacement into y (device).
+static void worst_apply(Hier& H, const double* q, double* y, double solve_tol, int mode, int pre, int post,
+                        int* iters) {
     const size_t nl3 = (size_t)3 * H.nload, n = (size_t)3 * H.lv[0].g.N;
-    RT_HIP(hipMemcpyAsync(H.pm_fs.p, q.data(), sizeof(double) * nl3, hipMemcpyHostToDevice, H.s));
+    RT_HIP(hipMemcpyAsync(H.pm_fs.p, q, sizeof(double) * nl3, hipMemcpyDeviceToDevice, H.s));
     launch_fill<double>(H.f64.data(), 0.0, n, H.s);
     pm_commit(H, 1.0);
     launch_fill<double>(H.x64.data(), 0.0, n, H.s);
@@ -2044,9 +2064,7 @@ static void worst_apply(Hier& H, const std::vector<double>& q, std::vector<doubl
     if (!(st.rel_res <= 10 * solve_tol)) throw std::runtime_error("worst-case solve did not converge");
     *iters += st.iterations;
     pm_gather(H, H.x64);
-    y.resize(nl3);
-    RT_HIP(hipMemcpyAsync(y.data(), H.pm_fs.p, sizeof(double) * nl3, hipMemcpyDeviceToHost, H.s));
-    RT_HIP(hipStreamSynchronize(H.s));
+    RT_HIP(hipMemcpyAsync(y, H.pm_fs.p, sizeof(double) * nl3, hipMemcpyDeviceToDevice, H.s));
 }
 
 // Block Lanczos with full reorthogonalisation on C restricted to the
@@ -2060,83 +2078,85 @@ static double block_lanczos(Hier& H, int k, int max_blocks, double tol, double s
     if (!H.nload) throw std::logic_error("no load nodes set");
     if (k < 1 || k > 64 || max_blocks < 1) throw std::invalid_argument("block size must be 1..64, max_blocks >= 1");
     const size_t nl = (size_t)H.nload, m = 3 * nl;
-    // projector P per load node (identity, or n n^T)
+    if (m > (size_t)INT32_MAX / 2) throw std::invalid_argument("too many load nodes");
     std::vector<double> nrm;
     if (H.load_normals) {
         nrm.resize(m);
         RT_HIP(hipMemcpy(nrm.data(), H.load_nrm.p, sizeof(double) * m, hipMemcpyDeviceToHost));
     }
-    auto project = [&](std::vector<double>& v) {
-        if (nrm.empty()) return;
-        for (size_t i = 0; i < nl; i++) {
-            const double d = v[i] * nrm[i] + v[nl + i] * nrm[nl + i] + v[2 * nl + i] * nrm[2 * nl + i];
-            v[i] = d * nrm[i]; v[nl + i] = d * nrm[nl + i]; v[2 * nl + i] = d * nrm[2 * nl + i];
-        }
-    };
     const size_t dim = nrm.empty() ? m : nl;   // dimension of the admissible space
     k = (int)std::min<size_t>((size_t)k, dim);
-    // seeded start block (splitmix64 uniforms in [-1, 1))
-    uint64_t st = seed;
-    auto next = [&st]() {
-        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
-        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-        z ^= z >> 31;
-        return 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
-    };
-    std::vector<std::vector<double>> Q;                      // all basis vectors so far
-    std::vector<std::vector<double>> blk(k, std::vector<double>(m));
-    for (auto& v : blk) {
-        for (auto& x : v) x = next();
-        project(v);
+    const int mi = (int)m;
+    // basis: block j in columns [j k, (j + 1) k); block j + 1 is the work block
+    DBuf Qd, Cd, Md, fd;
+    Qd.alloc(sizeof(double) * m * (size_t)(max_blocks + 1) * k);
+    Cd.alloc(sizeof(double) * (size_t)(max_blocks + 1) * k * k);
+    Md.alloc(sizeof(double) * (size_t)k * k);
+    fd.alloc(sizeof(double) * m);
+    double* Q = Qd.as<double>();
+    auto col = [&](int c) { return Q + (size_t)c * m; };
+    // seeded start block (splitmix64 uniforms in [-1, 1)), projected per load node onto n n^T
+    {
+        uint64_t st = seed;
+        auto next = [&st]() {
+            uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            return 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+        };
+        std::vector<double> v((size_t)k * m);
+        for (int c = 0; c < k; c++) {
+            double* x = v.data() + (size_t)c * m;
+            for (size_t i = 0; i < m; i++) x[i] = next();
+            if (!nrm.empty())
+                for (size_t i = 0; i < nl; i++) {
+                    const double d = x[i] * nrm[i] + x[nl + i] * nrm[nl + i] + x[2 * nl + i] * nrm[2 * nl + i];
+                    x[i] = d * nrm[i]; x[nl + i] = d * nrm[nl + i]; x[2 * nl + i] = d * nrm[2 * nl + i];
+                }
+        }
+        RT_HIP(hipMemcpyAsync(Q, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, H.s));
     }
     std::vector<double> R;
-    int r = block_orth(blk, Q, R);
+    std::vector<char> zero;
+    int r = block_orth_dev(H, mi, k, Q, nullptr, 0, Cd.as<double>(), R, zero);
     if (r < 1) throw std::domain_error("block Lanczos start block is zero");
-    std::vector<std::vector<double>> prev;                   // previous block
-    std::vector<double> Bprev;                               // k x k, W_{j-1} = Q_j B_{j-1}
+    std::vector<double> Bprev;                               // k x k row-major, W_{j-1} = Q_j B_{j-1}
     std::vector<std::vector<double>> Ablk, Bblk;             // diagonal and sub-diagonal blocks of T
     int solves = 0, iters = 0, j = 0;
     double theta = 0;
     std::vector<double> ritz;                                // top Ritz vector coefficients
     for (j = 0; j < max_blocks; j++) {
-        std::vector<std::vector<double>> W(k);
+        double* blk = col(j * k);
+        double* W = col((j + 1) * k);
         for (int c = 0; c < k; c++) {
-            if (std::all_of(blk[c].begin(), blk[c].end(), [](double x) { return x == 0; })) {
-                W[c].assign(m, 0.0);
+            if (zero[c]) {
+                RT_HIP(hipMemsetAsync(W + (size_t)c * m, 0, sizeof(double) * m, H.s));
                 continue;
             }
-            worst_apply(H, blk[c], W[c], solve_tol, mode, pre, post, &iters);
-            project(W[c]);
+            worst_apply(H, blk + (size_t)c * m, W + (size_t)c * m, solve_tol, mode, pre, post, &iters);
             solves++;
         }
-        // A_j = Q_j^T W (symmetrised)
+        // A_j = Q_j^T W (symmetrised); column-major k x k, A(a, b) = q_a . w_b
         std::vector<double> A((size_t)k * k);
-        for (int a = 0; a < k; a++)
-            for (int b = 0; b < k; b++) {
-                double s2 = 0;
-                for (size_t i = 0; i < m; i++) s2 += blk[a][i] * W[b][i];
-                A[(size_t)a * k + b] = s2;
-            }
+        gemm(H, rocblas_operation_transpose, rocblas_operation_none, k, k, mi, 1.0, blk, mi, W, mi, 0.0, Md.as<double>(), k);
+        RT_HIP(hipMemcpyAsync(A.data(), Md.p, sizeof(double) * k * k, hipMemcpyDeviceToHost, H.s));
+        RT_HIP(hipStreamSynchronize(H.s));
         for (int a = 0; a < k; a++)
             for (int b = a + 1; b < k; b++) A[(size_t)a * k + b] = A[(size_t)b * k + a] =
                 0.5 * (A[(size_t)a * k + b] + A[(size_t)b * k + a]);
         Ablk.push_back(A);
-        for (const auto& v : blk) Q.push_back(v);
-        // W -= Q_j A_j + Q_{j-1} B_{j-1}^T, then full reorthogonalisation against Q
-        for (int b = 0; b < k; b++) {
-            for (int a = 0; a < k; a++) {
-                const double c = A[(size_t)a * k + b];
-                for (size_t i = 0; i < m; i++) W[b][i] -= c * blk[a][i];
-            }
-            if (!prev.empty())
-                for (int a = 0; a < k; a++) {
-                    const double c = Bprev[(size_t)b * k + a];    // (B_{j-1}^T)[a][b]
-                    for (size_t i = 0; i < m; i++) W[b][i] -= c * prev[a][i];
-                }
+        // W -= Q_j A_j + Q_{j-1} B_{j-1}^T, then full reorthogonalisation against Q_0..Q_j
+        RT_HIP(hipMemcpyAsync(Md.p, A.data(), sizeof(double) * k * k, hipMemcpyHostToDevice, H.s));
+        gemm(H, rocblas_operation_none, rocblas_operation_none, mi, k, k, -1.0, blk, mi, Md.as<double>(), k, 1.0, W, mi);
+        if (j > 0) {
+            // (B^T)(a, b) = B[b][a]: the row-major B read as column-major is B^T
+            RT_HIP(hipMemcpyAsync(Md.p, Bprev.data(), sizeof(double) * k * k, hipMemcpyHostToDevice, H.s));
+            gemm(H, rocblas_operation_none, rocblas_operation_none, mi, k, k, -1.0, col((j - 1) * k), mi,
+                 Md.as<double>(), k, 1.0, W, mi);
         }
         std::vector<double> B;
-        r = block_orth(W, Q, B);                                  // W = Q_{j+1} B_j
+        r = block_orth_dev(H, mi, k, W, Q, (j + 1) * k, Cd.as<double>(), B, zero);   // W = Q_{j+1} B_j
         Bblk.push_back(B);
         // T: block tridiagonal, (j+1) k square
         const int nt = (j + 1) * k;
@@ -2167,22 +2187,22 @@ static double block_lanczos(Hier& H, int k, int max_blocks, double tol, double s
             res += s2 * s2;
         }
         res = std::sqrt(res);
-        prev = blk;
         Bprev = B;
-        blk = W;
-        if (res <= tol * std::fabs(theta) || r == 0 || Q.size() + k > dim) { j++; break; }
+        if (res <= tol * std::fabs(theta) || r == 0 || (size_t)(j + 2) * k > dim) { j++; break; }
     }
-    // Ritz load, unit length, then its displacement
-    std::vector<double> f(m, 0.0);
-    for (size_t i = 0; i < ritz.size(); i++)
-        for (size_t t = 0; t < m; t++) f[t] += ritz[i] * Q[i][t];
-    double fn = 0;
-    for (double x : f) fn += x * x;
-    fn = std::sqrt(fn);
-    for (double& x : f) x /= fn;
-    std::vector<double> u;
-    worst_apply(H, f, u, solve_tol, mode, pre, post, &iters);
+    // Ritz load f = Q y, unit length, then its displacement
+    const int nt = (int)ritz.size();
+    DBuf yd;
+    yd.alloc(sizeof(double) * nt);
+    RT_HIP(hipMemcpyAsync(yd.p, ritz.data(), sizeof(double) * nt, hipMemcpyHostToDevice, H.s));
+    gemm(H, rocblas_operation_none, rocblas_operation_none, mi, 1, nt, 1.0, Q, mi, yd.as<double>(), nt, 0.0,
+         fd.as<double>(), mi);
+    const double fn = std::sqrt(ddot(H, mi, fd.as<double>(), fd.as<double>()));
+    if (!(fn > 0)) throw std::domain_error("Ritz load is zero");
+    dscal(H, mi, 1.0 / fn, fd.as<double>());
+    worst_apply(H, fd.as<double>(), col(0), solve_tol, mode, pre, post, &iters);
     solves++;
+    RT_HIP(hipStreamSynchronize(H.s));
     if (blocks_out) *blocks_out = j;
     if (solves_out) *solves_out = solves;
     if (iters_out) *iters_out = iters;

@@ -306,6 +306,20 @@ class HierarchyGrid:
         a = as_c(u_box, np.float64).reshape(3, -1)
         check(lib().rt_set_displacement_box(self.handle, ptr(a)))
 
+    def write_displacement(self, path):
+        """HierarchyGrid::writeDisplacement (Grid.cpp:886-897): the finest
+        level's displacement as interleaved FP64 xyz in the reference's node order."""
+        from .formats import write_vectors
+        g0 = self[0]
+        write_vectors(path, self.get_displacement(), g0.masks()[1], g0.nodes)
+
+    def read_force(self, path):
+        """Grid::readForce (Grid.cpp:1679-1700): a force file in the reference's
+        node order and interleaved layout becomes the finest level's force."""
+        from .formats import read_vectors
+        g0 = self[0]
+        self.set_force(read_vectors(path, g0.masks()[1], g0.nodes))
+
     def get_displacement(self):
         a = np.zeros((3, self[0].n_nodes_box))
         check(lib().rt_get_displacement_box(self.handle, ptr(a)))
