@@ -1,8 +1,8 @@
 // Kernels of the learned warm start's convolutional network (see
 // net_internal.h for the tensor layout). The convolutions are implicit GEMMs on
-// the matrix cores: one wave computes 16 consecutive positions of a row (the
-// MFMA's M) times all output channels (N, in tiles of 16), accumulating over the
-// 27 taps and the input channels (K, 16 per instruction).
+// the matrix cores: one wave computes 16 consecutive positions (the MFMA's M)
+// in each of 4 consecutive rows, times all output channels (N, in tiles of 16),
+// accumulating over the 27 taps and the input channels (K, 16 per instruction).
 //
 // v_mfma_f32_16x16x16_bf16 operand layout (lane l, i = l % 16, q = l / 16):
 //   A: lane l holds A[i][4q .. 4q+3]   B: lane l holds B[4q .. 4q+3][i]
@@ -33,57 +33,80 @@ static unsigned nblocks(size_t n, int per = 256, size_t cap = 1u << 20) {
 // ---------------------------------------------------------------------------
 // convolution (forward, and backward-data with flip = 1 and transposed taps)
 // ---------------------------------------------------------------------------
+// A wave computes MT row tiles: 16 consecutive x positions in each of MT
+// consecutive rows (y) of one plane, so each weight fragment it loads feeds MT
+// products.
+constexpr int kConvMT = 4;
+
 template <int CIN, int COUT, int EPI>
 __global__ void __launch_bounds__(256) k_conv(Dims D, int flip, const uint16_t* __restrict__ X,
                                               const uint16_t* __restrict__ W, const float* __restrict__ bias,
                                               const uint16_t* __restrict__ aux, void* __restrict__ Yv) {
+    constexpr int MT = kConvMT;
     const int nx = D.n[0], ny = D.n[1], nz = D.n[2];
     const int lane = threadIdx.x & 63;
-    const int segs = (nx + 15) >> 4;
+    const int segs = (nx + 15) >> 4, ygr = (ny + MT - 1) / MT;
     const int seg = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (seg >= segs * ny * nz) return;   // whole waves leave; no barrier follows
-    const int sx = seg % segs, row = seg / segs, y = row % ny, z = row / ny;
+    if (seg >= segs * ygr * nz) return;   // whole waves leave; no barrier follows
+    const int sx = seg % segs, row = seg / segs, y0 = (row % ygr) * MT, z = row / ygr;
     const int x0 = sx << 4, m = lane & 15, kq = lane >> 4;
     const int xm = x0 + m;
-    f4 acc[COUT / 16];
+    f4 acc[MT][COUT / 16];
 #pragma unroll
-    for (int nb = 0; nb < COUT / 16; nb++) acc[nb] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int r = 0; r < MT; r++)
+#pragma unroll
+        for (int nb = 0; nb < COUT / 16; nb++) acc[r][nb] = f4{0.f, 0.f, 0.f, 0.f};
     const int sg = flip ? -1 : 1;
     for (int t = 0; t < 27; t++) {
         const int dx = sg * (t % 3 - 1), dy = sg * ((t / 3) % 3 - 1), dz = sg * (t / 9 - 1);
-        const int xx = xm + dx, yy = y + dy, zz = z + dz;
-        const bool ok = xm < nx && xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz;
-        const size_t pn = ok ? (size_t)xx + (size_t)nx * ((size_t)yy + (size_t)ny * zz) : 0;
-        const uint16_t* xp = X + pn * CIN + kq * 4;
+        const int xx = xm + dx, zz = z + dz;
+        const bool okxz = xm < nx && xx >= 0 && xx < nx && zz >= 0 && zz < nz;
+        const uint16_t* xp[MT];
+        bool ok[MT];
+#pragma unroll
+        for (int r = 0; r < MT; r++) {
+            const int yy = y0 + r + dy;
+            ok[r] = okxz && y0 + r < ny && yy >= 0 && yy < ny;
+            const size_t pn = ok[r] ? (size_t)xx + (size_t)nx * ((size_t)yy + (size_t)ny * zz) : 0;
+            xp[r] = X + pn * CIN + kq * 4;
+        }
         const uint16_t* wp = W + ((size_t)t * COUT + m) * CIN + kq * 4;
 #pragma unroll
         for (int kb = 0; kb < CIN / 16; kb++) {
-            const s4 a = ok ? *(const s4*)(xp + kb * 16) : s4{0, 0, 0, 0};
+            s4 a[MT];
+#pragma unroll
+            for (int r = 0; r < MT; r++) a[r] = ok[r] ? *(const s4*)(xp[r] + kb * 16) : s4{0, 0, 0, 0};
 #pragma unroll
             for (int nb = 0; nb < COUT / 16; nb++) {
                 const s4 b = *(const s4*)(wp + (size_t)nb * 16 * CIN + kb * 16);
-                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nb], 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < MT; r++)
+                    acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[r], b, acc[r][nb], 0, 0, 0);
             }
         }
     }
-    const size_t prow = (size_t)nx * ((size_t)y + (size_t)ny * z);
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int xo = x0 + 4 * kq + j;
-        if (xo >= nx) continue;
-        const size_t po = prow + xo;
+    for (int r = 0; r < MT; r++) {
+        if (y0 + r >= ny) break;
+        const size_t prow = (size_t)nx * ((size_t)(y0 + r) + (size_t)ny * z);
 #pragma unroll
-        for (int nb = 0; nb < COUT / 16; nb++) {
-            const int co = nb * 16 + m;
-            float v = acc[nb][j];
-            if (EPI == EPI_RELU || EPI == EPI_RELU_SKIP || EPI == EPI_F32) v += bias[co];
-            if (EPI == EPI_RELU_SKIP) v += bf2f(aux[po * COUT + co]);
-            if (EPI == EPI_RELU || EPI == EPI_RELU_SKIP) v = fmaxf(v, 0.f);
-            if (EPI == EPI_MASK) v = bf2f(aux[po * COUT + co]) > 0.f ? v : 0.f;
-            if (EPI == EPI_F32 || EPI == EPI_RAW_F32)
-                ((float*)Yv)[po * COUT + co] = v;
-            else
-                ((uint16_t*)Yv)[po * COUT + co] = f2bf(v);
+        for (int j = 0; j < 4; j++) {
+            const int xo = x0 + 4 * kq + j;
+            if (xo >= nx) continue;
+            const size_t po = prow + xo;
+#pragma unroll
+            for (int nb = 0; nb < COUT / 16; nb++) {
+                const int co = nb * 16 + m;
+                float v = acc[r][nb][j];
+                if (EPI == EPI_RELU || EPI == EPI_RELU_SKIP || EPI == EPI_F32) v += bias[co];
+                if (EPI == EPI_RELU_SKIP) v += bf2f(aux[po * COUT + co]);
+                if (EPI == EPI_RELU || EPI == EPI_RELU_SKIP) v = fmaxf(v, 0.f);
+                if (EPI == EPI_MASK) v = bf2f(aux[po * COUT + co]) > 0.f ? v : 0.f;
+                if (EPI == EPI_F32 || EPI == EPI_RAW_F32)
+                    ((float*)Yv)[po * COUT + co] = v;
+                else
+                    ((uint16_t*)Yv)[po * COUT + co] = f2bf(v);
+            }
         }
     }
 }
@@ -91,8 +114,8 @@ __global__ void __launch_bounds__(256) k_conv(Dims D, int flip, const uint16_t* 
 template <int CIN, int COUT>
 static void conv_epi(int epi, Dims D, int flip, const uint16_t* X, const uint16_t* W, const float* b,
                      const uint16_t* aux, void* Y, hipStream_t s) {
-    const size_t segs = (size_t)((D.n[0] + 15) / 16) * D.n[1] * D.n[2];
-    const dim3 grid((unsigned)((segs + 3) / 4)), blk(256);
+    const size_t waves = (size_t)((D.n[0] + 15) / 16) * ((D.n[1] + kConvMT - 1) / kConvMT) * D.n[2];
+    const dim3 grid((unsigned)((waves + 3) / 4)), blk(256);
     switch (epi) {
         case EPI_RELU: hipLaunchKernelGGL((k_conv<CIN, COUT, EPI_RELU>), grid, blk, 0, s, D, flip, X, W, b, aux, Y); break;
         case EPI_RELU_SKIP: hipLaunchKernelGGL((k_conv<CIN, COUT, EPI_RELU_SKIP>), grid, blk, 0, s, D, flip, X, W, b, aux, Y); break;
