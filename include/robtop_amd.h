@@ -376,6 +376,16 @@ int rt_voxelize_mesh(const float* coords, int nverts, const int* faces, int nfac
  * coarse levels with the sliced residual} (bit l = level l). */
 int rt_autotune(rt_hier* h, int reps, double* info);
 
+/* Diagnostic (no counterpart in the reference): the k smallest (ascending)
+ * and k largest (descending) Ritz values of the preconditioned operator from
+ * the CG coefficients of the last RT_SOLVE_RELIABLE solve (the Lanczos
+ * tridiagonal T_jj = 1/a_j + b_{j-1}/a_{j-1}, T_{j,j+1} = sqrt(b_j)/a_j);
+ * n_out = its order (the iteration count). Either array may be NULL. */
+int rt_cg_ritz_values(const rt_hier* h, int k, double* smallest, double* largest, int* n_out);
+/* the CG coefficients themselves: alpha_j, and beta_j (0 past the last) for
+ * j < min(nmax, n_out) */
+int rt_cg_coefficients(const rt_hier* h, int nmax, double* alpha, double* beta, int* n_out);
+
 /* --- timing hooks (benchmarking) ----------------------------------------- */
 
 int rt_synchronize(rt_hier* h);

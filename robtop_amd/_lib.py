@@ -125,6 +125,8 @@ SIGNATURES = {
     "rt_set_learned_warm_start": ([P, P], I),
     "rt_learned_warm_start_info": ([P, P], I),
     "rt_autotune": ([P, I, P], I),
+    "rt_cg_ritz_values": ([P, I, P, P, P], I),
+    "rt_cg_coefficients": ([P, I, P, P, P], I),
     "rt_synchronize": ([P], I),
     "rt_time_vcycle": ([P, I, I, I, I, P], I),
     "rt_time_fine_sweep": ([P, I, I, P], I),

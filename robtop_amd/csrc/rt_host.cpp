@@ -110,6 +110,9 @@ struct Hier {
     // {taken, E(plain start), E(learned start), prediction coefficient} of the last solve
     Net* net = nullptr;
     V3<double> nn_u;
+    // CG coefficients of the last reliable-update solve (Lanczos tridiagonal of
+    // the preconditioned operator): alpha_j, and beta_j forming p_{j+1}
+    std::vector<double> cg_alpha, cg_beta;
     double nn_info[4] = {0, 0, 0, 0};
     int ws_count = 0, ws_head = 0;
     V3<float> gs_tmp32;              // second buffer of the tiled fine smoother
@@ -1003,6 +1006,8 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
         // replacement and at exit.
         const int ns = kStreamBlocks;
         double best_true = rel_true;
+        H.cg_alpha.clear();
+        H.cg_beta.clear();
         launch_fill<float>(H.dx32.data(), 0.f, n, H.s);
         precondition<float>(H, pc, pre, post);
         double rz = dot<float, float>(H, pc.F.data(), pc.U.data(), n);
@@ -1012,6 +1017,7 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
             const double pq = apply_dot<float>(H, H.p32, H.q32);
             if (!(pq > 0)) throw std::domain_error("PCG breakdown: p.Ap <= 0");
             const double alpha = rz / pq;
+            H.cg_alpha.push_back(alpha);
             launch_pcg_ru_dxr(pc.F.data(), H.q32.data(), H.dx32.data(), H.p32.data(), alpha, n,
                               H.partials.as<double>(), ns, H.s);
             flushed = false;
@@ -1032,6 +1038,7 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
             precondition<float>(H, pc, pre, post);
             const double rz_new = dot<float, float>(H, pc.F.data(), pc.U.data(), n);
             const double beta = rz_new / rz;
+            H.cg_beta.push_back(beta);
             rz = rz_new;
             launch_pcg_update_p<float, float>(H.p32.data(), pc.U.data(), beta, n, H.s);
         }
@@ -1543,6 +1550,65 @@ int rt_set_warm_start(rt_hier* h, int m) {
 static void need_x4_level(const Hier& H) {
     if (H.lv.size() < 2 || H.lv[1].scale != 4)
         throw std::invalid_argument("the learned warm start needs an x4 coarse level");
+}
+
+// number of eigenvalues of the symmetric tridiagonal (d, e) below x (Sturm count)
+static int sturm_count(const std::vector<double>& d, const std::vector<double>& e, double x) {
+    int c = 0;
+    double q = 1.0;
+    for (size_t i = 0; i < d.size(); i++) {
+        q = d[i] - x - (i ? e[i - 1] * e[i - 1] / q : 0.0);
+        if (q == 0.0) q = -1e-300;
+        if (q < 0) c++;
+    }
+    return c;
+}
+
+int rt_cg_coefficients(const rt_hier* h, int nmax, double* alpha, double* beta, int* n_out) {
+    RT_TRY({
+        need(n_out);
+        const Hier& H = H_(h);
+        *n_out = (int)H.cg_alpha.size();
+        for (int i = 0; i < nmax && i < (int)H.cg_alpha.size(); i++) {
+            if (alpha) alpha[i] = H.cg_alpha[i];
+            if (beta) beta[i] = i < (int)H.cg_beta.size() ? H.cg_beta[i] : 0.0;
+        }
+    })
+}
+
+int rt_cg_ritz_values(const rt_hier* h, int k, double* smallest, double* largest, int* n_out) {
+    RT_TRY({
+        need(n_out);
+        const Hier& H = H_(h);
+        const int n = (int)std::min(H.cg_alpha.size(), H.cg_beta.size() + 1);
+        *n_out = n;
+        if (n == 0 || k < 1) return RT_OK;
+        // T_jj = 1/alpha_j + beta_{j-1}/alpha_{j-1}, T_{j,j+1} = sqrt(beta_j)/alpha_j
+        std::vector<double> d(n), e(n > 1 ? n - 1 : 0);
+        for (int j = 0; j < n; j++) {
+            d[j] = 1.0 / H.cg_alpha[j] + (j ? H.cg_beta[j - 1] / H.cg_alpha[j - 1] : 0.0);
+            if (j + 1 < n) e[j] = std::sqrt(std::max(0.0, H.cg_beta[j])) / H.cg_alpha[j];
+        }
+        double lo = 0, hi = 0;
+        for (int j = 0; j < n; j++) {
+            const double r = (j ? std::fabs(e[j - 1]) : 0.0) + (j + 1 < n ? std::fabs(e[j]) : 0.0);
+            lo = std::min(lo, d[j] - r);
+            hi = std::max(hi, d[j] + r);
+        }
+        // i-th smallest eigenvalue by bisection on the Sturm count (Gershgorin bracket)
+        auto kth = [&](int i) {
+            double a = lo, b = hi;
+            for (int it = 0; it < 200 && b - a > 1e-14 * std::max(1.0, std::fabs(b)); it++) {
+                const double m = 0.5 * (a + b);
+                if (sturm_count(d, e, m) > i) b = m; else a = m;
+            }
+            return 0.5 * (a + b);
+        };
+        for (int i = 0; i < k; i++) {
+            if (smallest) smallest[i] = i < n ? kth(i) : 0.0;
+            if (largest) largest[i] = i < n ? kth(n - 1 - i) : 0.0;
+        }
+    })
 }
 
 int rt_set_learned_warm_start(rt_hier* h, rt_net* net) {

@@ -272,6 +272,22 @@ class HierarchyGrid:
         check(lib().rt_learned_warm_start_info(self.handle, ptr(a)))
         return {"taken": bool(a[0]), "e_plain": float(a[1]), "e_learned": float(a[2]), "coef": float(a[3])}
 
+    def cg_ritz_values(self, k=8):
+        """(k smallest, k largest) Ritz values of the preconditioned operator
+        from the last reliable-update solve's CG coefficients."""
+        lo, hi = np.zeros(k), np.zeros(k)
+        n = ctypes.c_int()
+        check(lib().rt_cg_ritz_values(self.handle, k, ptr(lo), ptr(hi), ctypes.byref(n)))
+        return lo[:min(k, n.value)], hi[:min(k, n.value)]
+
+    def cg_coefficients(self, nmax=4096):
+        """(alpha, beta) of the last reliable-update solve's CG recurrence."""
+        a, b = np.zeros(nmax), np.zeros(nmax)
+        n = ctypes.c_int()
+        check(lib().rt_cg_coefficients(self.handle, nmax, ptr(a), ptr(b), ctypes.byref(n)))
+        m = min(nmax, n.value)
+        return a[:m], b[:m]
+
     def autotune(self, reps=10):
         """Time each level's launch-shape alternatives and keep the fastest
         (same operator; coarse kernels may sum in another order); returns the choices."""

@@ -91,3 +91,27 @@ def test_subspace_warm_start(cantilever):
     print(f"[mixed] PCG iterations per step, warm start 0 / 3: {its}", flush=True)
     assert rel_err(sols[3], sols[0]) < 1e-5
 
+
+
+def test_cg_ritz_values(cantilever):
+    """The Ritz values the library reports (Sturm bisection on the Lanczos
+    tridiagonal of the CG coefficients) equal numpy's eigenvalues of the same
+    tridiagonal, and lie in (0, ~1]: a symmetric V(2,2) cycle is a
+    contraction-based preconditioner whose preconditioned operator has its
+    spectrum near (0, 1]."""
+    hg, f, u_ref, c_ref = cantilever
+    hg.set_displacement(np.zeros_like(f))
+    st = hg.solve(tol=1e-8, maxit=2000, mode=rt.RT_SOLVE_RELIABLE, pre=2, post=2)
+    a, b = hg.cg_coefficients()
+    n = len(a)
+    assert n == st.iterations
+    m = n if n == len(b) and b[-1] != 0 else n   # beta_j forms p_{j+1}; the last may be absent
+    d = np.array([1 / a[j] + (b[j - 1] / a[j - 1] if j else 0.0) for j in range(m)])
+    e = np.array([np.sqrt(b[j]) / a[j] for j in range(m - 1)])
+    T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    ev = np.linalg.eigvalsh(T)
+    lo, hi = hg.cg_ritz_values(5)
+    print(f"[ritz] {n} iterations, smallest {lo}, largest {hi}", flush=True)
+    assert np.allclose(lo, ev[:5], rtol=1e-9, atol=1e-12)
+    assert np.allclose(hi, ev[::-1][:5], rtol=1e-9, atol=1e-12)
+    assert 0 < lo[0] and hi[0] < 1.5
