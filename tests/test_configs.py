@@ -1,0 +1,89 @@
+"""The benchmark's configurations 1 and 2 at their full sizes, checked through
+size-independent properties (the CPU oracle cannot run them in seconds; its
+parity with the GPU kernels is pinned at small sizes in test_gpu_parity.py and
+test_simp_parity.py).
+
+Config 1: cantilever 256x128x128, seeded smoothed-random density, FP32 V(2,2)
+MG-PCG, 50 SIMP iterations with filter r = 2 and OC (volume 0.4, step 0.03,
+damping 0.5): every solve reaches 1e-4 in true FP64 residual (the reference's
+own loop stops its V(1,1) cycles at 1e-2, test_utils.cpp:633-637; at 1e-6 the
+PCG iteration count passes 1000 by SIMP step 26 as the densities approach 0/1),
+densities stay in [rho_min, 1] and within the move limit of their previous
+values, the volume constraint holds after each OC update unless its bisection
+used all 31 trials, and the compliance falls.
+
+Config 2: 512x256x256 (101.6M DOF), FP32 V-cycle with bfloat16 coarse-level
+stencils, FP64 true residual and iterative refinement to 1e-8, against an FP64
+MG-PCG solve (FP64 V-cycle and vectors; its true residual bottoms out near
+4e-10 at this size) as the reference: relative
+displacement error below 1e-5 and compliance error below 1e-7, both printed.
+"""
+import numpy as np
+import pytest
+
+import robtop_amd as rt
+from problems import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(ex, ey, ez, seed=0):
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from bench import build_problem
+    return build_problem(ex, ey, ez, seed)
+
+
+def test_config1_simp_50_iterations():
+    print("[config1] cantilever 256x128x128, 50 SIMP iterations", flush=True)
+    hg, bits, R, fixed, f, rho = _problem(256, 128, 128, 1)
+    eact, _ = hg[0].masks()
+    comps, iters, trials = [], [], []
+    d_old = hg.get_density()[eact]
+    for it in range(50):
+        hg.update_stencil()
+        st = hg.solve(tol=1e-4, maxit=4000, mode=rt.RT_SOLVE_RELIABLE, pre=2, post=2)
+        assert st.rel_res < 1.05e-4, (it, st.rel_res)
+        comps.append(hg.compliance())
+        iters.append(st.iterations)
+        hg.compute_sensitivity()
+        hg.filter_sensitivity(2)
+        _, ntr = hg.update_densities(0.4, 0.03, 0.5, 1e-3)
+        trials.append(ntr)
+        d = hg.get_density()[eact]
+        # OC (optimization.cu:136-226): bounds, move limit, and the volume target
+        # unless the bisection on lambda in [0, max |g|] used all its trials
+        assert d.min() >= 1e-3 - 1e-7 and d.max() <= 1.0 + 1e-7
+        assert np.abs(d - d_old).max() <= 0.03 + 1e-6
+        assert abs(d.mean() - 0.4) < 2e-4 or ntr >= 31, (it, d.mean(), ntr)
+        d_old = d
+        if it % 10 == 9 or it < 2:
+            print(f"[config1] iteration {it + 1}: c {comps[-1]:.6e}, PCG iterations {st.iterations}, "
+                  f"volume {d.mean():.5f}, OC trials {ntr}", flush=True)
+    assert comps[-1] < 0.5 * comps[0]
+    print(f"[config1] PCG iterations per step {iters}", flush=True)
+    hg.close()
+
+
+def test_config2_mixed_precision_against_fp64():
+    print("[config2] cantilever 512x256x256", flush=True)
+    hg, bits, R, fixed, f, rho = _problem(512, 256, 256, 2)
+    hg.update_stencil()
+    hg.set_displacement(np.zeros_like(f))
+    ref = hg.solve(tol=1e-11, maxit=3000, mode=rt.RT_SOLVE_FP64, pre=2, post=2)
+    assert ref.rel_res < 1e-9, ref.rel_res     # FP64's attainable true residual at 101.6M DOF is ~4e-10
+    u_ref = hg.get_displacement()
+    c_ref = hg.compliance()
+    hg.set_coarse_stencil_bits(16)
+    hg.set_displacement(np.zeros_like(f))
+    st = hg.solve(tol=1e-8, maxit=3000, mode=rt.RT_SOLVE_FP32_IR, pre=2, post=2)
+    hg.set_coarse_stencil_bits(32)
+    eu = rel_err(hg.get_displacement(), u_ref)
+    ec = abs(hg.compliance() - c_ref) / abs(c_ref)
+    print(f"[config2] FP64 reference: {ref.iterations} iterations, {ref.seconds:.2f} s; bf16-coarse FP32 IR: "
+          f"{st.iterations} iterations, {st.seconds:.2f} s, residual {st.rel_res:.2e}, displacement error {eu:.2e}, "
+          f"compliance error {ec:.2e}", flush=True)
+    assert st.rel_res < 1.05e-8
+    assert eu < 1e-5 and ec < 1e-7
+    hg.close()
