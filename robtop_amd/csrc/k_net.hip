@@ -1,7 +1,7 @@
 // Kernels of the learned warm start's convolutional network (see
 // net_internal.h for the tensor layout). The convolutions are implicit GEMMs on
 // the matrix cores: one wave computes 16 consecutive positions (the MFMA's M)
-// in each of 4 consecutive rows, times all output channels (N, in tiles of 16),
+// in each of 1 or 4 consecutive rows, times all output channels (N, in tiles of 16),
 // accumulating over the 27 taps and the input channels (K, 16 per instruction).
 //
 // v_mfma_f32_16x16x16_bf16 operand layout (lane l, i = l % 16, q = l / 16):
@@ -35,14 +35,14 @@ static unsigned nblocks(size_t n, int per = 256, size_t cap = 1u << 20) {
 // ---------------------------------------------------------------------------
 // A wave computes MT row tiles: 16 consecutive x positions in each of MT
 // consecutive rows (y) of one plane, so each weight fragment it loads feeds MT
-// products.
-constexpr int kConvMT = 4;
+// products. MT = 4 on boxes that still give at least kConvMinWaves waves, else 1
+// (small boxes need the waves more than the reuse).
+constexpr int kConvMinWaves = 1024;
 
-template <int CIN, int COUT, int EPI>
+template <int CIN, int COUT, int EPI, int MT>
 __global__ void __launch_bounds__(256) k_conv(Dims D, int flip, const uint16_t* __restrict__ X,
                                               const uint16_t* __restrict__ W, const float* __restrict__ bias,
                                               const uint16_t* __restrict__ aux, void* __restrict__ Yv) {
-    constexpr int MT = kConvMT;
     const int nx = D.n[0], ny = D.n[1], nz = D.n[2];
     const int lane = threadIdx.x & 63;
     const int segs = (nx + 15) >> 4, ygr = (ny + MT - 1) / MT;
@@ -111,19 +111,26 @@ __global__ void __launch_bounds__(256) k_conv(Dims D, int flip, const uint16_t* 
     }
 }
 
+template <int CIN, int COUT, int MT>
+static void conv_epi_mt(int epi, Dims D, int flip, const uint16_t* X, const uint16_t* W, const float* b,
+                        const uint16_t* aux, void* Y, hipStream_t s) {
+    const size_t waves = (size_t)((D.n[0] + 15) / 16) * ((D.n[1] + MT - 1) / MT) * D.n[2];
+    const dim3 grid((unsigned)((waves + 3) / 4)), blk(256);
+    switch (epi) {
+        case EPI_RELU: hipLaunchKernelGGL((k_conv<CIN, COUT, EPI_RELU, MT>), grid, blk, 0, s, D, flip, X, W, b, aux, Y); break;
+        case EPI_RELU_SKIP: hipLaunchKernelGGL((k_conv<CIN, COUT, EPI_RELU_SKIP, MT>), grid, blk, 0, s, D, flip, X, W, b, aux, Y); break;
+        case EPI_F32: hipLaunchKernelGGL((k_conv<CIN, COUT, EPI_F32, MT>), grid, blk, 0, s, D, flip, X, W, b, aux, Y); break;
+        case EPI_MASK: hipLaunchKernelGGL((k_conv<CIN, COUT, EPI_MASK, MT>), grid, blk, 0, s, D, flip, X, W, b, aux, Y); break;
+        case EPI_RAW_F32: hipLaunchKernelGGL((k_conv<CIN, COUT, EPI_RAW_F32, MT>), grid, blk, 0, s, D, flip, X, W, b, aux, Y); break;
+        default: throw std::invalid_argument("conv: unknown epilogue");
+    }
+}
 template <int CIN, int COUT>
 static void conv_epi(int epi, Dims D, int flip, const uint16_t* X, const uint16_t* W, const float* b,
                      const uint16_t* aux, void* Y, hipStream_t s) {
-    const size_t waves = (size_t)((D.n[0] + 15) / 16) * ((D.n[1] + kConvMT - 1) / kConvMT) * D.n[2];
-    const dim3 grid((unsigned)((waves + 3) / 4)), blk(256);
-    switch (epi) {
-        case EPI_RELU: hipLaunchKernelGGL((k_conv<CIN, COUT, EPI_RELU>), grid, blk, 0, s, D, flip, X, W, b, aux, Y); break;
-        case EPI_RELU_SKIP: hipLaunchKernelGGL((k_conv<CIN, COUT, EPI_RELU_SKIP>), grid, blk, 0, s, D, flip, X, W, b, aux, Y); break;
-        case EPI_F32: hipLaunchKernelGGL((k_conv<CIN, COUT, EPI_F32>), grid, blk, 0, s, D, flip, X, W, b, aux, Y); break;
-        case EPI_MASK: hipLaunchKernelGGL((k_conv<CIN, COUT, EPI_MASK>), grid, blk, 0, s, D, flip, X, W, b, aux, Y); break;
-        case EPI_RAW_F32: hipLaunchKernelGGL((k_conv<CIN, COUT, EPI_RAW_F32>), grid, blk, 0, s, D, flip, X, W, b, aux, Y); break;
-        default: throw std::invalid_argument("conv: unknown epilogue");
-    }
+    const size_t waves4 = (size_t)((D.n[0] + 15) / 16) * ((D.n[1] + 3) / 4) * D.n[2];
+    if (waves4 >= (size_t)kConvMinWaves) conv_epi_mt<CIN, COUT, 4>(epi, D, flip, X, W, b, aux, Y, s);
+    else conv_epi_mt<CIN, COUT, 1>(epi, D, flip, X, W, b, aux, Y, s);
 }
 
 #define RT_CONV_SHAPES(X_) X_(16, 16) X_(16, 32) X_(32, 16) X_(32, 32) X_(32, 64) X_(64, 32) X_(64, 64)

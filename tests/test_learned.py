@@ -27,7 +27,9 @@ def rel(a, b):
     return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
 
 
-@pytest.mark.parametrize("shape", [(7, 9, 21), (2, 3, 4), (5, 8, 33)])
+# (32, 32, 64) is large enough for the four-row-tile convolution kernels at the
+# full resolution; the others exercise the one-row kernels and ragged rows
+@pytest.mark.parametrize("shape", [(7, 9, 21), (2, 3, 4), (5, 8, 33), (32, 32, 64)])
 def test_gpu_net_forward_matches_oracle(shape):
     print(f"[learned] forward {shape}", flush=True)
     rng = np.random.default_rng(sum(shape))
@@ -41,7 +43,7 @@ def test_gpu_net_forward_matches_oracle(shape):
     net.close()
 
 
-@pytest.mark.parametrize("shape", [(6, 10, 19), (3, 2, 5)])
+@pytest.mark.parametrize("shape", [(6, 10, 19), (3, 2, 5), (32, 32, 64)])
 def test_gpu_net_gradient_matches_oracle(shape):
     print(f"[learned] gradient {shape}", flush=True)
     rng = np.random.default_rng(7 + sum(shape))
