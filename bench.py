@@ -31,6 +31,8 @@ def parse():
                     help="cantilever: the headline s/SIMP-iteration run; torus: the worst-case loop on a "
                          "voxelized torus with block Lanczos (its own JSON line, not the headline)")
     ap.add_argument("--torus-n", type=int, default=384, help="torus voxels per side")
+    ap.add_argument("--lanczos-solve-tol", type=float, default=1e-6,
+                    help="relative residual of the PCG solves inside block Lanczos (torus workload)")
     ap.add_argument("--pre", type=int, default=2)
     ap.add_argument("--post", type=int, default=2)
     ap.add_argument("--tol", type=float, default=1e-6)
@@ -251,7 +253,7 @@ def torus_run(a):
         tuned = hg.autotune(reps=5)
         log(f"autotune: {tuned}")
     params = rt.Parameter(volume_ratio=0.3, volume_decrease=0.05)
-    lz = dict(k=8, max_blocks=20, tol=1e-6, solve_tol=1e-8, pre=a.pre, post=a.post)
+    lz = dict(k=8, max_blocks=20, tol=1e-6, solve_tol=a.lanczos_solve_tol, pre=a.pre, post=a.post)
     recs = rt.worst_case_topopt(hg, params, n_iter=a.warmup + a.steps, stop=False, method="lanczos", lanczos=lz,
                                 log=lambda r: log(f"iter {r['iter']}: c {r['worst_compliance']:.6e}, "
                                                   f"{r['blocks']} blocks, {r['solves']} solves, "
@@ -269,7 +271,8 @@ def torus_run(a):
         "ms_per_step": s_iter * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": f"torus R=0.35 r=0.12 voxelized at {n}^3, supports on the bottom 5 %, normal loads on "
-                               f"{nl} surface nodes, block Lanczos k=8 (tol 1e-6, solves to 1e-8), filter r=2, OC",
+                               f"{nl} surface nodes, block Lanczos k=8 (tol 1e-6, solves to {a.lanczos_solve_tol:g}), "
+                               f"filter r=2, OC",
                    "active_elements": ne, "active_nodes": nn, "levels": hg.n_levels,
                    "lanczos_per_iter": [{k: r[k] for k in ("blocks", "solves", "pcg_iterations")} for r in timed],
                    "autotune": tuned},
