@@ -31,6 +31,8 @@ def parse():
                     help="cantilever: the headline s/SIMP-iteration run; torus: the worst-case loop on a "
                          "voxelized torus with block Lanczos (its own JSON line, not the headline)")
     ap.add_argument("--torus-n", type=int, default=384, help="torus voxels per side")
+    ap.add_argument("--no-eigen-check", action="store_true",
+                    help="torus workload: skip the FP64 recomputation of the worst compliance (for profiling)")
     ap.add_argument("--lanczos-solve-tol", type=float, default=1e-6,
                     help="relative residual of the PCG solves inside block Lanczos (torus workload)")
     ap.add_argument("--pre", type=int, default=2)
@@ -245,13 +247,15 @@ def torus_run(a):
     hg.set_precision(32)
     hg.set_preconditioner(a.coarse_mult, a.gamma)
     log(f"hierarchy: {hg.n_levels} levels, {nl} load nodes")
+    hg.set_graphs(not a.no_graphs)
     tuned = None
+    eact, _ = hg[0].masks()
+    hg.set_density(np.where(eact, 0.3, 0.0).astype(np.float32))
+    hg.update_stencil()
     if not a.no_autotune:
-        eact, _ = hg[0].masks()
-        hg.set_density(np.where(eact, 0.3, 0.0).astype(np.float32))
-        hg.update_stencil()
         tuned = hg.autotune(reps=5)
         log(f"autotune: {tuned}")
+    t_v = hg.time_vcycle(32, a.pre, a.post, a.vcycles)
     params = rt.Parameter(volume_ratio=0.3, volume_decrease=0.05)
     lz = dict(k=8, max_blocks=20, tol=1e-6, solve_tol=a.lanczos_solve_tol, pre=a.pre, post=a.post)
     recs = rt.worst_case_topopt(hg, params, n_iter=a.warmup + a.steps, stop=False, method="lanczos", lanczos=lz,
@@ -261,9 +265,13 @@ def torus_run(a):
     timed = recs[a.warmup:]
     s_iter = float(np.mean([r["seconds"] for r in timed]))
     hg.update_stencil()
-    c32, st32 = hg.block_lanczos(seed=99, **lz)
-    lz64 = dict(lz, tol=1e-8, solve_tol=1e-11, mode=rt.RT_SOLVE_FP64)
-    c64, st64 = hg.block_lanczos(seed=99, **lz64)
+    eig = None
+    if not a.no_eigen_check:
+        c32, st32 = hg.block_lanczos(seed=99, **lz)
+        lz64 = dict(lz, tol=1e-8, solve_tol=1e-11, mode=rt.RT_SOLVE_FP64)
+        c64, st64 = hg.block_lanczos(seed=99, **lz64)
+        eig = {"worst_compliance_fp32_solves": c32, "worst_compliance_fp64_solves": c64,
+               "rel_diff": abs(c32 - c64) / abs(c64), "fp64_blocks": st64["blocks"], "fp64_solves": st64["solves"]}
     ne, nn = hg.num_active(0)
     out = {
         "metric": f"s/worst-case SIMP iter, torus {n}^3, block Lanczos k=8",
@@ -275,10 +283,8 @@ def torus_run(a):
                                f"filter r=2, OC",
                    "active_elements": ne, "active_nodes": nn, "levels": hg.n_levels,
                    "lanczos_per_iter": [{k: r[k] for k in ("blocks", "solves", "pcg_iterations")} for r in timed],
-                   "autotune": tuned},
-        "eigen_check": {"worst_compliance_fp32_solves": c32, "worst_compliance_fp64_solves": c64,
-                        "rel_diff": abs(c32 - c64) / abs(c64), "fp64_blocks": st64["blocks"],
-                        "fp64_solves": st64["solves"]},
+                   "autotune": tuned, "vcycle_ms": t_v, "node_box": [int(v) for v in hg[0].nodes]},
+        "eigen_check": eig,
     }
     hg.close()
     print(json.dumps(out), flush=True)
