@@ -113,6 +113,10 @@ __global__ void __launch_bounds__(256) k_fine_res_modal(Geom g, Vec3<const T> U,
     const int gx = gridDim.x, gy = gridDim.y;
     const int bl = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
     const int bm = RT_MODAL_XCD ? xcd_row(bl, gx * gy * (int)gridDim.z) : bl;
+    if (g.tile_act && !g.tile_act[bm]) {   // no active node in this tile: nothing to write
+        if (APPLY && partials && tid == 0) partials[bl] = 0.0;
+        return;
+    }
     const int x0 = (bm % gx) * TX, y0 = ((bm / gx) % gy) * TY, z0 = (bm / (gx * gy)) * g.zc;
     const int z1 = min(z0 + g.zc, nz);
     // element owned by this thread (tile-local i, j -> global x0 - 1 + i, y0 - 1 + j)
@@ -284,6 +288,8 @@ static dim3 modal_grid(const Geom& g) {
     if (g.zc < 1) throw std::invalid_argument("z planes per workgroup must be positive");
     return dim3((g.n[0] + TX - 1) / TX, (g.n[1] + TY - 1) / TY, (g.n[2] + g.zc - 1) / g.zc);
 }
+
+void modal_tile_dims(int* tx, int* ty) { *tx = TX; *ty = TY; }
 
 int fine_apply_partials_modal(const Geom& g) {
     dim3 gr = modal_grid(g);

@@ -57,6 +57,7 @@ struct Level {
     LevelVecs<float> v32;
     DBuf st64, st32, KEc;
     DBuf st16;                            // bfloat16 copy for FP32 preconditioner cycles (optional)
+    DBuf tile_act;                        // finest level: mask of residual / apply tiles holding an active node
     size_t nn_box() const { return (size_t)g.n[0] * g.n[1] * g.n[2]; }
     size_t ne_box() const { return (size_t)e[0] * e[1] * e[2]; }
 };
@@ -357,6 +358,26 @@ static void ensure_partials(Hier& H) {
     H.scal.alloc(sizeof(double) * 16);
 }
 
+// Finest-level work mask (Geom::tile_act) from the active nodes: the residual /
+// apply kernels skip tiles that hold no active node (their outputs are absent
+// slots, never written). Rebuilt whenever Geom::zc changes.
+static void build_tile_mask(Level& L) {
+    int tx, ty;
+    modal_tile_dims(&tx, &ty);
+    const int nx = L.g.n[0], ny = L.g.n[1], nz = L.g.n[2], tz = L.g.zc;
+    const int gx = (nx + tx - 1) / tx, gy = (ny + ty - 1) / ty, gz = (nz + tz - 1) / tz;
+    std::vector<uint8_t> m((size_t)gx * gy * gz, 0);
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++) {
+            const uint8_t* row = L.vact.data() + (size_t)nx * ((size_t)y + (size_t)ny * z);
+            for (int x = 0; x < nx; x++)
+                if (row[x]) m[(size_t)(x / tx) + (size_t)gx * ((size_t)(y / ty) + (size_t)gy * (z / tz))] = 1;
+        }
+    L.tile_act.alloc(m.size());
+    RT_HIP(hipMemcpy(L.tile_act.p, m.data(), m.size(), hipMemcpyHostToDevice));
+    L.g.tile_act = L.tile_act.as<uint8_t>();
+}
+
 static void setup_device(Hier& H) {
     for (auto& L : H.lv) upload_flags(H, L);
     Level& L0 = H.lv[0];
@@ -381,6 +402,7 @@ static void setup_device(Hier& H) {
             H.xfer.ensure(sizeof(double) * restrict_scratch_elems(H.lv[li - 1].g, L.g));
     }
     ensure_partials(H);
+    build_tile_mask(L0);
 }
 
 template <typename T> LevelVecs<T>& vecs(Level& L);
@@ -2575,6 +2597,7 @@ static void autotune(Hier& H, int reps, double* info) {
     for (int zc : {8, 12, 16, 24, 32, 48}) {
         L0.g.zc = zc;
         if (fine_apply_partials_modal(L0.g) > H.nparts_max) continue;
+        build_tile_mask(L0);
         const double t = time_ms(H, reps, [&] {
             launch_fine_residual_modal<float>(L0.g, pc.U.cv(), pc.F.cv(), pc.R.v(), H.rhop.as<float>(),
                                               L0.vflag.as<uint8_t>(), H.s);
@@ -2585,6 +2608,7 @@ static void autotune(Hier& H, int reps, double* info) {
         if (t < best) { best = t; best_zc = zc; }
     }
     L0.g.zc = best_zc;
+    build_tile_mask(L0);
     info[0] = best_zc;
     info[1] = best;
     // finest smoother implementation

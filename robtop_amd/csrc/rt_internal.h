@@ -42,6 +42,9 @@ struct Geom {
     int zc = 24;        // finest level: z planes marched per residual / apply workgroup
     int coarse_gs = 1;  // coarse smoother: 1 = nine waves per 64 nodes (sliced), 0 = a thread per node
     int coarse_res = -1;// coarse residual: 1 sliced, 0 a thread per node, -1 = by level size
+    // finest level: device mask of the residual / apply tiles (of the current
+    // zc) that hold an active node; null = all
+    const uint8_t* tile_act = nullptr;
 };
 
 inline Geom make_geom(int nx, int ny, int nz) {
@@ -127,6 +130,8 @@ template <typename T>
 void launch_fine_apply_modal(const Geom& g, Vec3<const T> X, Vec3<T> Y, const float* rhop, const uint8_t* vflag,
                              double* partials, int* nparts, hipStream_t s);
 int fine_apply_partials_modal(const Geom& g);
+// node tile of the residual / apply kernels (x by y; z by Geom::zc)
+void modal_tile_dims(int* tx, int* ty);
 
 // k_coarse.hip
 template <typename T, typename TS>
