@@ -1008,12 +1008,18 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
     LevelVecs<float>& pc = pcvecs<float>(H);   // pc.F holds the FP32 residual, pc.U the preconditioned one
     const int np = 1024;
     warm_start(H);
-    const double xAx = apply_dot<double>(H, H.x64, H.q64);
-    if (xAx > 0) {
-        const double xb = dot<double, double>(H, H.x64.data(), H.f64.data(), n);
-        const double alpha = xb > 0 ? xb / xAx : 0.0;
-        launch_scale<double>(H.x64.data(), alpha, n, H.s);
-        launch_scale<double>(H.q64.data(), alpha, n, H.s);
+    // energy-optimal scaling of the start; a zero start (the worst-case
+    // search's solves) needs no operator application: K x = 0
+    if (dot<double, double>(H, H.x64.data(), H.x64.data(), n) > 0) {
+        const double xAx = apply_dot<double>(H, H.x64, H.q64);
+        if (xAx > 0) {
+            const double xb = dot<double, double>(H, H.x64.data(), H.f64.data(), n);
+            const double alpha = xb > 0 ? xb / xAx : 0.0;
+            launch_scale<double>(H.x64.data(), alpha, n, H.s);
+            launch_scale<double>(H.q64.data(), alpha, n, H.s);
+        }
+    } else {
+        launch_fill<double>(H.q64.data(), 0.0, n, H.s);
     }
     launch_sub_conv<double, float>(H.f64.data(), H.q64.data(), H.r64.data(), pc.F.data(), n, H.partials.as<double>(),
                                    np, H.s);
@@ -1022,6 +1028,7 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
     double rel = nb > 0 ? std::sqrt(rr) / nb : 0.0, rel_true = rel, rmax = std::sqrt(rr);
     int it = 0;
     bool diverged = false;
+    bool have_true = false;   // rel_true is the true residual of the returned x64
     if (nb > 0 && rel > tol) {
         // The solution increment since the last residual replacement is kept
         // in FP32 (dx, updated with r) and flushed into x64 at each
@@ -1049,7 +1056,7 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
                 launch_pcg_flush_x(H.x64.data(), H.dx32.data(), n, H.s);
                 flushed = true;
                 rel_true = true_rel_residual(H);      // r64 = f - K x64
-                if (rel_true <= tol) { rel = rel_true; break; }
+                if (rel_true <= tol) { rel = rel_true; have_true = true; break; }
                 if (!(rel_true <= 10 * best_true)) { diverged = true; break; }
                 best_true = std::min(best_true, rel_true);
                 launch_convert<double, float>(H.r64.data(), pc.F.data(), n, H.s);
@@ -1084,7 +1091,7 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
     }
     subspace_push(H);
     RT_HIP(hipStreamSynchronize(H.s));
-    const double tr = true_rel_residual(H);
+    const double tr = have_true ? rel_true : true_rel_residual(H);
     auto t1 = std::chrono::steady_clock::now();
     if (st) {
         st->iterations = it;
