@@ -24,6 +24,12 @@ namespace rt {
 
 thread_local std::string g_err;
 
+// coarsening stops at the first layer with at most this many solid elements
+// (Grid.cpp:229: 400)
+#ifndef RT_COARSEST_ELEMS
+#define RT_COARSEST_ELEMS 400
+#endif
+
 // a 3-component vector: one contiguous buffer of 3N
 template <typename T>
 struct V3 {
@@ -222,7 +228,7 @@ static Hier* build(int R, const uint32_t* bits, size_t nwords) {
     counts.push_back(count(act[0]));
     if (counts[0] == 0) throw std::invalid_argument("no solid element");
     int reso = R;
-    while (counts.back() > 400 && reso > 1) {
+    while (counts.back() > RT_COARSEST_ELEMS && reso > 1) {
         const int fr = reso;
         reso >>= 1;
         std::vector<uint8_t> c((size_t)reso * reso * reso, 0);
