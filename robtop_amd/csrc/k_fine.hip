@@ -24,6 +24,18 @@
 
 namespace rt {
 
+// the smoother addresses u, rho^p and the flags through buffer resources whose
+// byte ranges are 32-bit: a level whose vectors pass 2 GiB is refused instead of
+// being read as zeros past the limit
+template <typename T>
+static void check_buffer_range(const Geom& g) {
+    if ((long long)g.N * (long long)sizeof(T) >= (1ll << 31))
+        throw std::invalid_argument(std::string("finest level of ") + std::to_string(g.N) +
+                                    " slots is too large for 32-bit buffer offsets in this precision's smoother" +
+                                    (sizeof(T) == 8 ? " (FP32 cycles reach twice the size)" : ""));
+}
+
+
 template <typename T> struct KeC { T c[RT_KE_NVAL]; };
 
 static double g_kec[RT_KE_NVAL];
@@ -494,6 +506,7 @@ __global__ void __launch_bounds__(256) k_fine_gs_tile(Geom g, Vec3<const T> Uin,
 template <typename T>
 void launch_fine_gs_sweep2(const Geom& g, Vec3<const T> Uin, Vec3<T> Uout, Vec3<const T> F, const float* rhop,
                            const uint8_t* vflag, bool reverse, bool zero_support, hipStream_t s) {
+    check_buffer_range<T>(g);
     const KeC<T> K = kec<T>();
     const int ntile = (g.n[1] + RT_GS_TILE_ROWS - 1) / RT_GS_TILE_ROWS;
     for (int pass = 0; pass < 2; pass++) {
@@ -520,6 +533,7 @@ template void launch_fine_gs_sweep2<double>(const Geom&, Vec3<const double>, Vec
 template <typename T>
 void launch_fine_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* rhop, const uint8_t* vflag, int pair,
                     bool reverse, bool zero_support, hipStream_t s) {
+    check_buffer_range<T>(g);
     const int cy = pair & 1, cz = pair >> 1;
     const long long rows = (long long)((g.n[1] - cy + 1) >> 1) * ((g.n[2] - cz + 1) >> 1);
     if (rows <= 0) return;

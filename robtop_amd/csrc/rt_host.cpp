@@ -209,6 +209,21 @@ static void galerkin4_matrices(const double KE[576], std::vector<double>& M) {
 // ---------------------------------------------------------------------------
 // hierarchy construction
 // ---------------------------------------------------------------------------
+// The smoothers and coarse kernels load through buffer resources whose byte
+// ranges are 32-bit. A level whose vectors (at `value_bytes` per entry) or whose
+// coarse stencil (nine FP64 3x3 blocks per slot, the larger precision) would pass
+// 2 GiB is refused here rather than read as zeros past the limit.
+static void check_load_range(const Level& L, int value_bytes) {
+    const long long lim = 1ll << 31;
+    const long long vec = (long long)L.g.N * value_bytes;
+    const long long st = L.layer > 0 ? 9ll * L.g.N * (long long)sizeof(double) : 0;
+    if (vec >= lim || st >= lim)
+        throw std::invalid_argument("level " + std::to_string(L.layer) + " has " + std::to_string(L.g.N) +
+                                    " slots: its " + (vec >= lim ? std::to_string(value_bytes * 8) + "-bit vectors"
+                                                                  : std::string("coarse stencil")) +
+                                    " pass the 2 GiB range of 32-bit buffer offsets");
+}
+
 static Hier* build(int R, const uint32_t* bits, size_t nwords) {
     if (R < 4 || (R & (R - 1))) throw std::invalid_argument("R must be a power of two >= 4");
     const size_t n0 = (size_t)R * R * R;
@@ -262,7 +277,6 @@ static Hier* build(int R, const uint32_t* bits, size_t nwords) {
     h->R = R;
     h->nlayers = nl;
     h->layer_active = counts;
-    RT_HIP(hipStreamCreate(&h->s));
     for (int layer = 0; layer < nl; layer++) {
         if (layer == 1) continue;
         Level L;
@@ -275,6 +289,7 @@ static Hier* build(int R, const uint32_t* bits, size_t nwords) {
             L.e[k] = (h0 - l0) >> layer;
         }
         L.g = make_geom(L.e[0] + 1, L.e[1] + 1, L.e[2] + 1);
+        check_load_range(L, 4);
         L.eact.assign(L.ne_box(), 0);
         for (int z = 0; z < L.e[2]; z++)
             for (int y = 0; y < L.e[1]; y++)
@@ -303,6 +318,8 @@ static Hier* build(int R, const uint32_t* bits, size_t nwords) {
                 }
         h->lv.push_back(std::move(L));
     }
+    // the first device call, after every check of the input
+    RT_HIP(hipStreamCreate(&h->s));
     return h.release();
 }
 
@@ -1737,6 +1754,7 @@ int rt_set_preconditioner(rt_hier* h, int coarse_sweep_mult, int gamma) {
 int rt_set_vcycle_precision(rt_hier* h, int bits) {
     RT_TRY({
         if (bits != 32 && bits != 64) throw std::invalid_argument("precision must be 32 or 64");
+        for (const Level& L : H_(h).lv) check_load_range(L, bits / 8);
         H_(h).vprec = bits;
     })
 }

@@ -36,6 +36,17 @@ def test_short_bit_array_is_rejected():
     assert code == _lib.RT_EARG and "too short" in msg
 
 
+def test_level_past_the_32_bit_load_range_is_rejected():
+    # a 1024 x 1024 x 512 slab: 8 * 515 * 515 * 259 = 549.6M slots, whose FP32
+    # vectors pass the 2 GiB a buffer load can address
+    R = 1024
+    bits = np.zeros(R ** 3 // 32, np.uint32)
+    bits[: R * R * 512 // 32] = 0xFFFFFFFF          # z < 512, lowest-bit-first x order
+    code, h, msg = _create(R, bits)
+    assert code == _lib.RT_EARG and not h.value
+    assert "level 0" in msg and "32-bit vectors" in msg and "2 GiB" in msg
+
+
 def test_network_null_handle_is_reported():
     L = _lib.lib()
     dims = np.array([4, 4, 4], np.int32)
