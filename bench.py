@@ -314,7 +314,7 @@ def main():
         print(json.dumps(pmc_summary(*a.pmc_summary, tuple(int(v) for v in a.size.split(",")))))
         return
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "robtop_amd benchmarks one GPU; --gpus must be 1"}))
         sys.exit(2)
     if a.profile:
         profile_run(a)

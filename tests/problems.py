@@ -19,7 +19,7 @@ def active_cube(kind, R):
     elif kind == "lshape4":   # 4 layers, L-shaped, offset from the origin
         a[8:56, 4:28, 4:28] = True
         a[32:56, 16:28, 4:28] = False
-    elif kind == "config0":   # BASELINE config 0 geometry
+    elif kind == "cantilever64":   # the 64x32x32 cantilever box
         a[:64, :32, :32] = True
     else:
         raise ValueError(kind)

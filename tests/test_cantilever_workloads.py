@@ -1,9 +1,9 @@
-"""The benchmark's configurations 1 and 2 at their full sizes, checked through
+"""The two large cantilever workloads at their full sizes, checked through
 size-independent properties (the CPU oracle cannot run them in seconds; its
 parity with the GPU kernels is pinned at small sizes in test_gpu_parity.py and
 test_simp_parity.py).
 
-Config 1: cantilever 256x128x128, seeded smoothed-random density, FP32 V(2,2)
+Cantilever 256x128x128, seeded smoothed-random density, FP32 V(2,2)
 MG-PCG, 50 SIMP iterations with filter r = 2 and OC (volume 0.4, step 0.03,
 damping 0.5): every solve reaches 1e-4 in true FP64 residual (the reference's
 own loop stops its V(1,1) cycles at 1e-2, test_utils.cpp:633-637; at 1e-6 the
@@ -12,7 +12,7 @@ densities stay in [rho_min, 1] and within the move limit of their previous
 values, the volume constraint holds after each OC update unless its bisection
 used all 31 trials, and the compliance falls.
 
-Config 2: 512x256x256 (101.6M DOF), FP32 V-cycle with bfloat16 coarse-level
+Cantilever 512x256x256 (101.6M DOF), FP32 V-cycle with bfloat16 coarse-level
 stencils, FP64 true residual and iterative refinement to 1e-8, against an FP64
 MG-PCG solve (FP64 V-cycle and vectors; its true residual bottoms out near
 4e-10 at this size) as the reference: relative
@@ -35,8 +35,8 @@ def _problem(ex, ey, ez, seed=0):
     return build_problem(ex, ey, ez, seed)
 
 
-def test_config1_simp_50_iterations():
-    print("[config1] cantilever 256x128x128, 50 SIMP iterations", flush=True)
+def test_cantilever256_simp_50_iterations():
+    print("[cantilever256] cantilever 256x128x128, 50 SIMP iterations", flush=True)
     hg, bits, R, fixed, f, rho = _problem(256, 128, 128, 1)
     eact, _ = hg[0].masks()
     comps, iters, trials = [], [], []
@@ -59,15 +59,15 @@ def test_config1_simp_50_iterations():
         assert abs(d.mean() - 0.4) < 2e-4 or ntr >= 31, (it, d.mean(), ntr)
         d_old = d
         if it % 10 == 9 or it < 2:
-            print(f"[config1] iteration {it + 1}: c {comps[-1]:.6e}, PCG iterations {st.iterations}, "
+            print(f"[cantilever256] iteration {it + 1}: c {comps[-1]:.6e}, PCG iterations {st.iterations}, "
                   f"volume {d.mean():.5f}, OC trials {ntr}", flush=True)
     assert comps[-1] < 0.5 * comps[0]
-    print(f"[config1] PCG iterations per step {iters}", flush=True)
+    print(f"[cantilever256] PCG iterations per step {iters}", flush=True)
     hg.close()
 
 
-def test_config2_mixed_precision_against_fp64():
-    print("[config2] cantilever 512x256x256", flush=True)
+def test_cantilever512_mixed_precision_against_fp64():
+    print("[cantilever512] cantilever 512x256x256", flush=True)
     hg, bits, R, fixed, f, rho = _problem(512, 256, 256, 2)
     hg.update_stencil()
     hg.set_displacement(np.zeros_like(f))
@@ -81,7 +81,7 @@ def test_config2_mixed_precision_against_fp64():
     hg.set_coarse_stencil_bits(32)
     eu = rel_err(hg.get_displacement(), u_ref)
     ec = abs(hg.compliance() - c_ref) / abs(c_ref)
-    print(f"[config2] FP64 reference: {ref.iterations} iterations, {ref.seconds:.2f} s; bf16-coarse FP32 IR: "
+    print(f"[cantilever512] FP64 reference: {ref.iterations} iterations, {ref.seconds:.2f} s; bf16-coarse FP32 IR: "
           f"{st.iterations} iterations, {st.seconds:.2f} s, residual {st.rel_res:.2e}, displacement error {eu:.2e}, "
           f"compliance error {ec:.2e}", flush=True)
     assert st.rel_res < 1.05e-8

@@ -17,7 +17,7 @@ TOL = {64: 1e-12, 32: 2e-5}
 
 class Pair:
     def __init__(self, kind, seed=0, E=1e6, nu=0.4, p=3.0):
-        a = active_cube(kind, 64 if kind.endswith("4") or kind == "config0" else 32)
+        a = active_cube(kind, 64 if kind.endswith("4") or kind == "cantilever64" else 32)
         R = a.shape[0]
         bits = bits_of(a)
         print(f"[pair] kind={kind} seed={seed}", flush=True)

@@ -56,8 +56,8 @@ def test_levels_match_bruteforce(kind):
 
 
 def test_config_level_counts():
-    """Node counts of the reference's coarsening rule at config 0 (64x32x32)."""
-    a = active_cube("config0", 64)
+    """Node counts of the reference's coarsening rule on the 64x32x32 cantilever box."""
+    a = active_cube("cantilever64", 64)
     H = oracle.Hierarchy(64, bits_of(a))
     assert H.nlayers == 4
     nodes = [int(l["vact"].sum()) for l in H.levels]

@@ -19,10 +19,10 @@ from problems import rel_err
 pytestmark = pytest.mark.gpu
 
 
-def test_config0_simp_iterations_match_oracle():
+def test_cantilever64_simp_iterations_match_oracle():
     from robtop_amd.workloads import box_bits, cantilever_bc
     ex, ey, ez, R = 64, 32, 32, 64
-    print("[simp] config0 cantilever 64x32x32", flush=True)
+    print("[simp] cantilever 64x32x32", flush=True)
     bits = box_bits(R, ex, ey, ez)
     g = rt.HierarchyGrid.genFromMesh(bits, R)
     o = oracle.Hierarchy(R, bits)
