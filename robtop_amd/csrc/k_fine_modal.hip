@@ -163,6 +163,19 @@ __global__ void __launch_bounds__(256) k_fine_res_modal(Geom g, Vec3<const T> U,
         }
     };
 
+    // rho^p of the thread's element and (residual) f of its output node are
+    // loaded one step ahead of their use, like the planes above
+    auto load_rp = [&](int z) -> float {
+        return (e_in && z >= 0 && z < nz - 1) ? rhop[oct_p(g, gex, gey, z)] : 0.f;
+    };
+    auto load_f = [&](int z, T (&f)[3]) {
+        f[0] = f[1] = f[2] = T(0);
+        if (!APPLY && n_in && z >= z0 && z < z1) {
+            const int s = oct_p(g, gnx, gny, z);
+            f[0] = F.c[0][s]; f[1] = F.c[1][s]; f[2] = F.c[2][s];
+        }
+    };
+
     T part[3] = {T(0), T(0), T(0)};
     double dotacc = 0.0;
     int lo = 0;
@@ -171,16 +184,21 @@ __global__ void __launch_bounds__(256) k_fine_res_modal(Geom g, Vec3<const T> U,
     commit(sa, lo);
     fetch(sa, z0);
     fetch(sb, z0 + 1);
+    float rp_next = load_rp(z0 - 1);
+    T f_next[3];
+    load_f(z0 - 1, f_next);
     for (int ez = z0 - 1; ez < z1; ez++) {
         const int hi = lo ^ 1;
         commit(sa, hi);        // plane ez + 1
         sa = sb;
         fetch(sb, ez + 3);     // in flight during this step and the next
+        const float rp = rp_next;
+        rp_next = load_rp(ez + 1);
+        const T fo[3] = {f_next[0], f_next[1], f_next[2]};   // f of node plane ez
+        load_f(ez + 1, f_next);
         __syncthreads();
         // element (gex, gey, ez) between planes lo (kz = 0) and hi (kz = 1)
         T y[24];
-        float rp = 0.f;
-        if (e_in && ez >= 0 && ez < nz - 1) rp = rhop[oct_p(g, gex, gey, ez)];
         if (rp != 0.f) {
             T a[24];
 #pragma unroll
@@ -230,9 +248,9 @@ __global__ void __launch_bounds__(256) k_fine_res_modal(Geom g, Vec3<const T> U,
                                   (double)KU[2] * (double)su[lo][2][p];
                     } else {
                         if (fl & NF_FIXED) {
-                            R.c[0][s] = F.c[0][s]; R.c[1][s] = F.c[1][s]; R.c[2][s] = F.c[2][s];
+                            R.c[0][s] = fo[0]; R.c[1][s] = fo[1]; R.c[2][s] = fo[2];
                         } else {
-                            R.c[0][s] = F.c[0][s] - KU[0]; R.c[1][s] = F.c[1][s] - KU[1]; R.c[2][s] = F.c[2][s] - KU[2];
+                            R.c[0][s] = fo[0] - KU[0]; R.c[1][s] = fo[1] - KU[1]; R.c[2][s] = fo[2] - KU[2];
                         }
                     }
                 }
