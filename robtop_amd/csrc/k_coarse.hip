@@ -403,19 +403,41 @@ __global__ void __launch_bounds__(256) k_restrict_xy(Geom gf, int ofx, int ofy, 
 #pragma unroll
     for (int m = 0; m < MAXC; m++) s[m][0] = s[m][1] = s[m][2] = T(0);
     const int nh = (nx + 1) >> 1;
-    for (int dy = -Wd; dy <= Wd; dy++) {
-        const int py = fy + dy;
-        if (py < 0 || py >= ny) continue;           // uniform across the workgroup
-        __syncthreads();                             // the previous row has been consumed
-        for (int k = threadIdx.x; k < nh; k += blockDim.x) {
+    // rows of the window are double-buffered through registers: the loads of
+    // the next row are issued once the current one is in LDS, so they overlap
+    // its accumulation
+    constexpr int MAXK = (kProlongRowMax / 2 + 255) / 256;
+    T nxt[MAXK][2][3];
+    auto load_row = [&](int py) {
+#pragma unroll
+        for (int m = 0; m < MAXK; m++) {
+            const int k = threadIdx.x + 256 * m;
 #pragma unroll
             for (int par = 0; par < 2; par++) {
                 const int x = 2 * k + par;
-                if (x >= nx) continue;
-                const int sl = oct_p(gf, x, py, z);
-                row[0][x] = Rf.c[0][sl]; row[1][x] = Rf.c[1][sl]; row[2][x] = Rf.c[2][sl];
+                if (k < nh && x < nx) {
+                    const int sl = oct_p(gf, x, py, z);
+                    nxt[m][par][0] = Rf.c[0][sl]; nxt[m][par][1] = Rf.c[1][sl]; nxt[m][par][2] = Rf.c[2][sl];
+                }
             }
         }
+    };
+    const int dy_lo = max(-Wd, -fy), dy_hi = min(Wd, ny - 1 - fy);   // rows inside the box
+    if (dy_lo <= dy_hi) load_row(fy + dy_lo);
+    for (int dy = dy_lo; dy <= dy_hi; dy++) {
+        __syncthreads();                             // the previous row has been consumed
+#pragma unroll
+        for (int m = 0; m < MAXK; m++) {
+            const int k = threadIdx.x + 256 * m;
+#pragma unroll
+            for (int par = 0; par < 2; par++) {
+                const int x = 2 * k + par;
+                if (k < nh && x < nx) {
+                    row[0][x] = nxt[m][par][0]; row[1][x] = nxt[m][par][1]; row[2][x] = nxt[m][par][2];
+                }
+            }
+        }
+        if (dy < dy_hi) load_row(fy + dy + 1);
         __syncthreads();
         const int wy = RATIO - (dy < 0 ? -dy : dy);
 #pragma unroll
@@ -550,6 +572,22 @@ __global__ void __launch_bounds__(256) k_prolong_row(Geom gf, int ofx, int ofy, 
     const int py = ay % RATIO, pz = az % RATIO;
     const int by = ay / RATIO - ocy, bz = az / RATIO - ocz;
     constexpr T inv = T(1) / T(RATIO * RATIO * RATIO);
+    // the fine row's flags and values do not depend on the interpolation: the
+    // thread's first pair of fine nodes is loaded before it, so the two load
+    // rounds overlap (each fine node is read and written by one thread only)
+    const int nh = (nx + 1) >> 1;
+    const int k0 = threadIdx.x;
+    uint8_t pf[2] = {0, 0};
+    T pu[2][3] = {{T(0), T(0), T(0)}, {T(0), T(0), T(0)}};
+#pragma unroll
+    for (int par = 0; par < 2; par++) {
+        const int x = 2 * k0 + par;
+        if (k0 < nh && x < nx) {
+            const int v = oct_p(gf, x, y, z);
+            pf[par] = vflag_f[v];
+            pu[par][0] = Uf.c[0][v]; pu[par][1] = Uf.c[1][v]; pu[par][2] = Uf.c[2][v];
+        }
+    }
     for (int gxl = threadIdx.x; gxl < ngx; gxl += blockDim.x) {
         const int bxa = gx0 + gxl;
         const int xs = bxa * RATIO - ofx;             // fine x of the interval start
@@ -581,15 +619,20 @@ __global__ void __launch_bounds__(256) k_prolong_row(Geom gf, int ofx, int ofy, 
         }
     }
     __syncthreads();
-    const int nh = (nx + 1) >> 1;
     for (int k = threadIdx.x; k < nh; k += blockDim.x) {
 #pragma unroll
         for (int par = 0; par < 2; par++) {
             const int x = 2 * k + par;
             if (x >= nx) continue;
             const int v = oct_p(gf, x, y, z);
-            const uint8_t fl = vflag_f[v];                // loaded together with the values
-            const T u0 = Uf.c[0][v], u1 = Uf.c[1][v], u2 = Uf.c[2][v];
+            uint8_t fl;
+            T u0, u1, u2;
+            if (k == k0) {
+                fl = pf[par]; u0 = pu[par][0]; u1 = pu[par][1]; u2 = pu[par][2];
+            } else {
+                fl = vflag_f[v];                          // loaded together with the values
+                u0 = Uf.c[0][v]; u1 = Uf.c[1][v]; u2 = Uf.c[2][v];
+            }
             if (!(fl & NF_ACTIVE) || (skip_fixed && (fl & NF_FIXED))) continue;
             Uf.c[0][v] = u0 + inc[0][par][k];
             Uf.c[1][v] = u1 + inc[1][par][k];
