@@ -154,6 +154,10 @@ __global__ void __launch_bounds__(576) k_coarse_gs9(Geom g, Vec3<T> U, Vec3<cons
     const int N = g.N;
     const int sh = g.h[0] * g.h[1] + g.h[0] + 1;
     const int vbu = act ? (base - sh) * (int)sizeof(T) : 0, vbs = act ? v * (int)sizeof(TS) : 0;
+    // f of the node, needed after the barrier by the wave that updates it, is
+    // loaded with the neighbour values so it does not add a second load round
+    T fo[3] = {T(0), T(0), T(0)};
+    if (act && w == 4) { fo[0] = F.c[0][v]; fo[1] = F.c[1][v]; fo[2] = F.c[2][v]; }
     T Au[3] = {T(0), T(0), T(0)};
     T S[9], uc[3] = {T(0), T(0), T(0)};
     if (act) {
@@ -198,7 +202,7 @@ __global__ void __launch_bounds__(576) k_coarse_gs9(Geom g, Vec3<T> U, Vec3<cons
         Au[r] = acc;
     }
     T u[3] = {uc[0], uc[1], uc[2]};
-    const T f[3] = {F.c[0][v], F.c[1][v], F.c[2][v]};
+    const T f[3] = {fo[0], fo[1], fo[2]};
     if (!reverse) {
         u[0] = (f[0] - Au[0] - (S[1] * u[1] + S[2] * u[2])) / S[0];
         u[1] = (f[1] - Au[1] - (S[3] * u[0] + S[5] * u[2])) / S[4];
@@ -281,6 +285,8 @@ __global__ void __launch_bounds__(576) k_coarse_res9(Geom g, Vec3<const T> U, Ve
     }
     const int N = g.N;
     const int sh = g.h[0] * g.h[1] + g.h[0] + 1;
+    T fo[3] = {T(0), T(0), T(0)};   // f for wave 0's output, loaded before the barrier
+    if (act && w == 0) { fo[0] = F.c[0][v]; fo[1] = F.c[1][v]; fo[2] = F.c[2][v]; }
     T Au[3] = {T(0), T(0), T(0)};
     if (act) {
         const int vbu = (base - sh) * (int)sizeof(T), vbs = v * (int)sizeof(TS);
@@ -314,7 +320,7 @@ __global__ void __launch_bounds__(576) k_coarse_res9(Geom g, Vec3<const T> U, Ve
         T acc = T(0);
 #pragma unroll
         for (int q = 0; q < 9; q++) acc += part[q][r][lane];
-        R.c[r][v] = F.c[r][v] - acc;
+        R.c[r][v] = fo[r] - acc;
     }
 }
 
