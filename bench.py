@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--pre", type=int, default=2)
     ap.add_argument("--post", type=int, default=2)
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--maxit", type=int, default=5000,
+                    help="PCG iteration cap per solve; a timed step that reaches it without the tolerance makes "
+                         "the run invalid (exit status 3)")
     ap.add_argument("--coarse-mult", type=int, default=1, help="sweep multiplier on coarse levels (preconditioner)")
     ap.add_argument("--gamma", type=int, default=1, help="preconditioner cycle index: 1 = V, 2 = W")
     ap.add_argument("--fine-smoother", default="pairs", choices=["tiled", "pairs"],
@@ -385,7 +388,7 @@ def main():
         hg.update_stencil()
         hg.synchronize()
         t1 = time.perf_counter()
-        st = hg.solve(tol=a.tol, maxit=500, mode=mode, pre=a.pre, post=a.post)
+        st = hg.solve(tol=a.tol, maxit=a.maxit, mode=mode, pre=a.pre, post=a.post)
         c = hg.compliance()
         t2 = time.perf_counter()
         if learned is not None and record:
@@ -405,17 +408,22 @@ def main():
         st, _ = simp_step()
         log(f"warmup {w}: {st.iterations} PCG iterations, rel_res {st.rel_res:.2e}")
     hg.synchronize()
-    iters, rels, comps = [], [], []
+    iters, rels, comps, conv, secs = [], [], [], [], []
     t0 = time.perf_counter()
-    for _ in range(a.steps):
+    for k in range(a.steps):
+        ts = time.perf_counter()
         st, c = simp_step(record=True)
-        log(f"step: {st.iterations} PCG iterations, rel_res {st.rel_res:.2e}, c {c:.6e}")
+        secs.append(time.perf_counter() - ts)
+        log(f"step {a.warmup + k}: {st.iterations} PCG iterations, rel_res {st.rel_res:.2e}, "
+            f"{'converged' if st.converged else 'NOT CONVERGED'}, c {c:.6e}, {secs[-1]:.3f} s")
         iters.append(st.iterations)
         rels.append(st.rel_res)
         comps.append(c)
+        conv.append(bool(st.converged))
     hg.synchronize()
     t1 = time.perf_counter()
     s_per_iter = (t1 - t0) / max(1, a.steps)
+    n_conv = int(sum(conv))
 
     # V-cycle and dominant-kernel timing (HIP events on the handle's stream)
     hg.update_stencil()
@@ -460,6 +468,10 @@ def main():
                         f"MG-PCG {a.mode} V({a.pre},{a.post}) to {a.tol:g}, filter r=2, OC",
             "active_elements": ne0, "active_nodes": nn0, "dofs": 3 * nn0, "levels": hg.n_levels,
             "pcg_iters_per_step": float(np.mean(iters)), "rel_res": float(np.max(rels)),
+            "converged_steps": n_conv, "maxit": a.maxit,
+            "per_step": [{"simp_step": a.warmup + k, "pcg_iterations": int(iters[k]), "rel_res": float(rels[k]),
+                          "converged": conv[k], "seconds": float(secs[k])} for k in range(a.steps)],
+            "median_seconds_per_step": float(np.median(secs)),
             "compliance_last": comps[-1],
             "seconds_per_step": {k: v / max(1, a.steps) for k, v in phase.items()},
         },
@@ -482,7 +494,13 @@ def main():
         hg.close()
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(bits, R, fixed, f, rho, a.pre, a.post, n_v, n_app)
-    print(json.dumps(out))
+    out["valid"] = n_conv == a.steps
+    if not out["valid"]:
+        out["invalid_reason"] = (f"{a.steps - n_conv} of {a.steps} timed solves stopped at maxit={a.maxit} above "
+                                 f"rel-res {a.tol:g}")
+    print(json.dumps(out), flush=True)
+    if not out["valid"]:
+        sys.exit(3)
 
 
 if __name__ == "__main__":

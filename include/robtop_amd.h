@@ -58,6 +58,7 @@ typedef struct rt_solve_stats {
     double rel_res;         /* ||f - K u|| / ||f|| of the returned u, recomputed in FP64 */
     double rel_res_pcg;     /* the PCG recurrence residual at exit */
     double seconds;         /* wall time of the solve */
+    int converged;          /* 1 if rel_res <= tol; 0 if maxit ran out first (the call still returns RT_OK) */
 } rt_solve_stats;
 
 const char* rt_last_error(void);
@@ -298,6 +299,10 @@ int rt_mma_destroy(rt_mma* h);
  * moves the asymptotes (mma_t.cu:161-226); *converged = 1 once the design
  * change has stayed below 1e-3 for more than 10 updates. */
 int rt_mma_update(rt_mma* h, const float* df, const float* dg, const float* g, int* converged);
+/* The same update with the constraint gradients as m separate rows of n values
+ * (host or device arrays), the Scalar** dev_dg form of
+ * MMA::mma_t::update(Scalar*, Scalar**, Scalar*) (optimization/mma_t.h:255). */
+int rt_mma_update_rows(rt_mma* h, const float* df, const float* const* dg_rows, const float* g, int* converged);
 /* MMA::mma_t::get_x (mma_t.cu:156-159): current design (host or device array) */
 int rt_mma_get_x(const rt_mma* h, float* x);
 /* last update's multipliers lambda[m], artificial variable z, number of

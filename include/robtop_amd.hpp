@@ -175,7 +175,7 @@ public:
         const auto o = g.origin();
         const std::vector<uint8_t> surf = g.surfaceNodes();
         const size_t n = g.n_nodes_box();
-        std::vector<uint8_t> fixed(n, 0);
+        std::vector<uint8_t> fixed(n, 0), load(n, 0);
         std::vector<double> f(3 * n, 0.0);
         const double eh = (double)(box_[1][0] - box_[0][0]) / R_;
         nload_ = 0;
@@ -188,6 +188,7 @@ public:
                                      box_[0][2] + (o[2] + z) * eh};
                     if (fixarea(pos)) { fixed[v] = 1; continue; }
                     if (loadarea(pos)) {
+                        load[v] = 1;
                         const auto fv = field(pos);
                         for (int c = 0; c < 3; c++) f[c * n + v] = fv[c];
                         nload_++;
@@ -195,8 +196,21 @@ public:
                 }
         check(rt_set_support_box(handle(), fixed.data()));
         check(rt_set_force_box(handle(), f.data()));
+        // the load nodes are also the admissible load set of the worst-case
+        // search (setLoadNodes, projection.cpp:129-269), free direction (wsff)
+        check(rt_set_load_nodes(handle(), load.data(), nullptr));
         fixed_ = std::move(fixed);
+        load_ = std::move(load);
     }
+    /* constrained-direction worst case (wscf, setWorkMode): the admissible load
+     * at each load node is along the given unit normal ([3][node box], the
+     * reference takes outward normals from CGAL, Grid.cpp:1405-1423) */
+    void setLoadNormals(const std::vector<double>& normals_box) {
+        if (normals_box.size() != 3 * load_.size()) throw error(RT_EARG, "normals must be 3 x the finest node box");
+        check(rt_set_load_nodes(handle(), load_.data(), normals_box.data()));
+    }
+    /* load-node flags of the finest node box set by setBoundaryCondition */
+    const std::vector<uint8_t>& loadMask() const { return load_; }
     size_t n_load_nodes() const { return nload_; }
     /* support flags of the finest node box set by setBoundaryCondition */
     const std::vector<uint8_t>& supportMask() const { return fixed_; }
@@ -237,7 +251,7 @@ private:
     int R_ = 0;
     int prefer_reso_ = 200;
     size_t nload_ = 0;
-    std::vector<uint8_t> fixed_;
+    std::vector<uint8_t> fixed_, load_;
     float box_[2][3] = {{0, 0, 0}, {0, 0, 0}};
     std::vector<std::unique_ptr<Grid>> levels_;
 };
@@ -400,10 +414,36 @@ inline void uploadTemplateMatrix(void) {
 inline void initDensities(double rho) { getGrids().init_rho(rho); }
 /* update_stencil (optimization.cpp:692-695) */
 inline void update_stencil(void) { getGrids().update_stencil(); }
-/* solveFEM (optimization.cpp:85-91): the reference repeats V(1,1) to 1e-4;
- * this solves with MG-PCG to the same tolerance */
-inline rt_solve_stats solveFEM(double tol = 1e-4, int mode = RT_SOLVE_MIXED) {
-    return getGrids().solve(tol, 500, mode, 1, 1);
+/* MG-PCG solve of the current hierarchy to ||f - K u|| / ||f|| <= tol (no
+ * reference counterpart); returns the solver statistics and throws if maxit
+ * runs out first */
+inline rt_solve_stats solveFEM(double tol, int mode = RT_SOLVE_MIXED, int maxit = 5000) {
+    rt_solve_stats st = getGrids().solve(tol, maxit, mode, 1, 1);
+    if (!st.converged)
+        throw error(RT_ENUM, "solveFEM: relative residual " + std::to_string(st.rel_res) + " above " +
+                                 std::to_string(tol) + " after " + std::to_string(st.iterations) + " iterations");
+    return st;
+}
+/* void solveFEM(void) (optimization.h:47, optimization.cpp:85-91): the
+ * reference repeats V(1,1) cycles until the relative residual is at most 1e-4;
+ * this reaches the same tolerance with MG-PCG (V(1,1) preconditioner) */
+inline void solveFEM(void) { (void)solveFEM(1e-4, RT_SOLVE_RELIABLE); }
+
+/* double modifiedPM(void) (optimization.h:50, optimization.cpp:123-261): the
+ * worst-case load of the load nodes set with rt_set_load_nodes, by the
+ * modified power method from a random start (randForce, Grid.cu:2242-2245;
+ * seeded from a per-process counter here instead of the clock), at most 500
+ * steps of one V(1,1) cycle each. The worst load and its displacement stay in
+ * the force and displacement; returns the worst compliance. */
+inline double modifiedPM(void) {
+    static uint64_t seed = 0;
+    rt_hier* h = getGrids().handle();
+    check(rt_rand_force(h, ++seed));
+    double c = 0;
+    int cycles = 0;
+    check(rt_modified_pm(h, 500, &c, &cycles));
+    if (std::isnan(c)) throw error(RT_ENUM, "modifiedPM: NaN compliance");
+    return c;
 }
 /* computeSensitivity (optimization.cu:109-133): element sensitivity, then the
  * filter of radius params.filter_radius */
@@ -419,6 +459,61 @@ inline float updateDensities(float Vgoal) {
     check(rt_update_densities(getGrids().handle(), Vgoal, p.design_step, p.damp_ratio, p.min_rho, &thr, &trials));
     return thr;
 }
+
+namespace MMA {
+using Scalar = float;
+/* MMA::mma_t (optimization/mma_t.h:159-266): the method of moving asymptotes
+ * on the GPU for ndim FP32 design variables and nConstrain <= 8 constraints
+ * g(x) <= 0. Usage as in the reference: construct, init(bounds), then
+ * update(df, dg, g) once per design iteration; get_x() returns the design. */
+class mma_t {
+public:
+    mma_t(int ndim, int nConstrain, Scalar /*err_tol*/ = 1e-7f) : n_(ndim), m_(nConstrain) {}
+    ~mma_t() { reset(); }
+    mma_t(const mma_t&) = delete;
+    mma_t& operator=(const mma_t&) = delete;
+
+    /* mma_t::init (mma_t.h:215-224): bounds per variable or uniform */
+    void init(const Scalar* lower_bound, const Scalar* upper_bound) {
+        reset();
+        check(rt_mma_create(n_, m_, lower_bound, upper_bound, &h_));
+    }
+    void init(Scalar lower_bound, Scalar upper_bound) {
+        const std::vector<Scalar> lo((size_t)n_, lower_bound), hi((size_t)n_, upper_bound);
+        init(lo.data(), hi.data());
+    }
+    /* bool mma_t::update(Scalar* dev_df, Scalar** dev_dg, Scalar* dev_g)
+     * (mma_t.h:255, mma_t.cpp:18-51): df[n] objective gradient, dg[i][n] the
+     * i-th constraint's gradient, g[m] the constraint values (host or device
+     * arrays); true once the design change has stayed below 1e-3 for more than
+     * 10 updates */
+    bool update(Scalar* dev_df, Scalar** dev_dg, Scalar* dev_g) {
+        int conv = 0;
+        check(rt_mma_update_rows(H(), dev_df, dev_dg, dev_g, &conv));
+        return conv != 0;
+    }
+    /* mma_t::get_x(Scalar* dst) (mma_t.cu:156-159): host or device array of n */
+    void get_x(Scalar* dst) const { check(rt_mma_get_x(H(), dst)); }
+    std::vector<Scalar> get_x(void) const {
+        std::vector<Scalar> x((size_t)n_);
+        get_x(x.data());
+        return x;
+    }
+    int n_dim(void) const { return n_; }
+    int n_constrain(void) const { return m_; }
+
+private:
+    rt_mma* H() const {
+        if (!h_) throw error(RT_ESTATE, "mma_t::init has not been called");
+        return h_;
+    }
+    void reset() {
+        if (h_) { rt_mma_destroy(h_); h_ = nullptr; }
+    }
+    int n_, m_;
+    rt_mma* h_ = nullptr;
+};
+}  // namespace MMA
 
 /* converge_criteria (utils/snippet.h:156-213) */
 class converge_criteria {

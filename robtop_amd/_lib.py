@@ -25,7 +25,8 @@ class RobtopError(RuntimeError):
 
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("rel_res", ctypes.c_double),
-                ("rel_res_pcg", ctypes.c_double), ("seconds", ctypes.c_double)]
+                ("rel_res_pcg", ctypes.c_double), ("seconds", ctypes.c_double),
+                ("converged", ctypes.c_int)]
 
 
 class LanczosStats(ctypes.Structure):
@@ -96,6 +97,7 @@ SIGNATURES = {
     "rt_mma_create": ([I, I, P, P, P], I),
     "rt_mma_destroy": ([P], I),
     "rt_mma_update": ([P, P, P, P, P], I),
+    "rt_mma_update_rows": ([P, P, P, P, P], I),
     "rt_mma_get_x": ([P, P], I),
     "rt_mma_stats": ([P, P, P, P, P], I),
     "rt_set_load_nodes": ([P, P, P], I),

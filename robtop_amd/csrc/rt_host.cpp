@@ -846,7 +846,9 @@ static void subspace_push(Hier& H) {
 // current start x (the previous solution, or the subspace combination) in a
 // Galerkin combination x' = c0 x + c1 u_nn minimising the energy
 // E(y) = y.Ky / 2 - f.y, i.e. the K-norm error. x' replaces x only when
-// E(x') < E(x); otherwise the plain start stays (the plain MG-PCG fallback).
+// E(x') < E(a x), the energy of the best multiple of x alone (the PCG scales
+// its start that way); otherwise the plain start stays (the plain MG-PCG
+// fallback).
 static void learned_warm_start(Hier& H) {
     if (!H.net) return;
     Level& L0 = H.lv[0];
@@ -878,7 +880,9 @@ static void learned_warm_start(Hier& H) {
         c[0] += V[e] * vb / w[e];
         c[1] += V[2 + e] * vb / w[e];
     }
-    const double e_plain = 0.5 * G[0] - b[0];
+    // the plain start is scaled by its own energy-optimal factor b0 / G0 before
+    // the PCG begins, so the learned start must beat that: E(a x) = -b0^2 / (2 G0)
+    const double e_plain = G[0] > 0 ? -0.5 * b[0] * b[0] / G[0] : 0.0;
     const double e_learn = 0.5 * (c[0] * c[0] * G[0] + 2 * c[0] * c[1] * G[1] + c[1] * c[1] * G[3]) - c[0] * b[0] -
                            c[1] * b[1];
     const bool take = std::isfinite(e_learn) && e_learn < e_plain;
@@ -950,8 +954,9 @@ static int pcg_core(Hier& H, V3<TV>& x, V3<TV>& b, double tol, int maxit, int pr
     return it;
 }
 
+// warm: apply the warm starts first (false when continuing from the current x64)
 template <typename TV, typename TC>
-static void pcg(Hier& H, double tol, int maxit, int pre, int post, rt_solve_stats* st) {
+static void pcg(Hier& H, double tol, int maxit, int pre, int post, rt_solve_stats* st, bool warm = true) {
     if (!H.stencil_ready) throw std::logic_error("update_stencil has not been called");
     auto t0 = std::chrono::steady_clock::now();
     Level& L0 = H.lv[0];
@@ -961,7 +966,7 @@ static void pcg(Hier& H, double tol, int maxit, int pre, int post, rt_solve_stat
     V3<TV>& b = KV<TV>::b(H);
     if (!x.ok()) x.alloc(N);
     if (!b.ok()) b.alloc(N);
-    warm_start(H);
+    if (warm) warm_start(H);
     if ((void*)x.data() != (void*)H.x64.data()) {
         launch_convert<double, TV>(H.x64.data(), x.data(), n, H.s);
         launch_convert<double, TV>(H.f64.data(), b.data(), n, H.s);
@@ -978,6 +983,7 @@ static void pcg(Hier& H, double tol, int maxit, int pre, int post, rt_solve_stat
         st->rel_res = tr;
         st->rel_res_pcg = rel;
         st->seconds = std::chrono::duration<double>(t1 - t0).count();
+        st->converged = tr <= tol;
     }
 }
 
@@ -1008,6 +1014,7 @@ static void pcg_ir(Hier& H, double tol, int maxit, int pre, int post, double inn
         st->rel_res = rel;
         st->rel_res_pcg = rel_in;
         st->seconds = std::chrono::duration<double>(t1 - t0).count();
+        st->converged = rel <= tol;
     }
 }
 
@@ -1112,12 +1119,13 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
         // stiffness contrast). Finish with FP64 Krylov vectors from the current
         // x, whose energy-optimal scaling is never worse than a zero start.
         rt_solve_stats s2{};
-        pcg<double, float>(H, tol, std::max(0, maxit - it), pre, post, &s2);
+        pcg<double, float>(H, tol, std::max(0, maxit - it), pre, post, &s2, false);
         if (st) {
             st->iterations = it + s2.iterations;
             st->rel_res = s2.rel_res;
             st->rel_res_pcg = s2.rel_res_pcg;
             st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            st->converged = s2.converged;
         }
         return;
     }
@@ -1130,6 +1138,7 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
         st->rel_res = tr;
         st->rel_res_pcg = rel;
         st->seconds = std::chrono::duration<double>(t1 - t0).count();
+        st->converged = tr <= tol;
     }
 }
 

@@ -130,11 +130,17 @@ std::pair<float, float> kkt_from_trial(rt_mma& M, const std::vector<double>& h, 
 }
 
 // mma_subproblem_t::solve (mma_t.cu:330-527) + mma_t::update (mma_t.cpp:18-51)
-bool mma_update(rt_mma& M, const float* df, const float* dg, const float* g) {
+// dg: the m constraint gradients as one row-major [m][n] array, or (dg null)
+// as m separate rows dg_rows[i] of n values, the reference's Scalar** form
+bool mma_update(rt_mma& M, const float* df, const float* dg, const float* g, const float* const* dg_rows = nullptr) {
     const int n = M.n, m = M.m, nb = M.nb;
     hipStream_t s = M.s;
     RT_HIP(hipMemcpyAsync(M.df_dev, df, sizeof(float) * n, hipMemcpyDefault, s));
-    RT_HIP(hipMemcpyAsync(M.dg_dev, dg, sizeof(float) * (size_t)n * m, hipMemcpyDefault, s));
+    if (dg)
+        RT_HIP(hipMemcpyAsync(M.dg_dev, dg, sizeof(float) * (size_t)n * m, hipMemcpyDefault, s));
+    else
+        for (int i = 0; i < m; i++)
+            RT_HIP(hipMemcpyAsync(M.dg_dev + (size_t)i * n, dg_rows[i], sizeof(float) * n, hipMemcpyDefault, s));
     std::vector<float> gv(m);
     RT_HIP(hipMemcpyAsync(gv.data(), g, sizeof(float) * m, hipMemcpyDefault, s));
     // p, q at the current design and the proxy constants b = proxy - g
@@ -321,6 +327,17 @@ int rt_mma_update(rt_mma* h, const float* df, const float* dg, const float* g, i
         rt_mma& M = MM_(h);
         if (!df || !dg || !g) throw std::invalid_argument("null pointer argument");
         const bool c = mma_update(M, df, dg, g);
+        if (converged) *converged = c ? 1 : 0;
+    })
+}
+
+int rt_mma_update_rows(rt_mma* h, const float* df, const float* const* dg_rows, const float* g, int* converged) {
+    RT_MMA_TRY({
+        rt_mma& M = MM_(h);
+        if (!df || !dg_rows || !g) throw std::invalid_argument("null pointer argument");
+        for (int i = 0; i < M.m; i++)
+            if (!dg_rows[i]) throw std::invalid_argument("null constraint-gradient row");
+        const bool c = mma_update(M, df, nullptr, g, dg_rows);
         if (converged) *converged = c ? 1 : 0;
     })
 }
