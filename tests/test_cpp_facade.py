@@ -143,5 +143,7 @@ def test_modified_pm_and_void_solve_fem_through_cpp_facade():
     hg.rand_force(1)
     c, steps = hg.modified_pm(500)
     hg.close()
-    assert abs(c - r["worst_compliance"]) <= 1e-12 * abs(c)
+    # setParameters takes floats (optimization.h:32-35), so the facade's
+    # Poisson ratio is 0.4f, 1.5e-8 relative away from the 0.4 used here
+    assert abs(c - r["worst_compliance"]) <= 1e-7 * abs(c)
     assert abs(r["compliance_after_solveFEM"] - c) <= 2e-2 * c
