@@ -51,6 +51,7 @@ def parse():
     ap.add_argument("--islands", type=float, default=0.0, metavar="THETA",
                     help="deflate the rigid-body modes of floating clusters of elements with rho >= THETA in the "
                          "preconditioner (0: off)")
+    ap.add_argument("--island-min", type=int, default=1, help="smallest floating cluster deflated (elements)")
     ap.add_argument("--coarse-bf16", action="store_true",
                     help="store the coarse-level stencils of the FP32 preconditioner in bfloat16")
     ap.add_argument("--mode", default="reliable", choices=["mixed", "reliable", "fp64"],
@@ -339,7 +340,7 @@ def main():
     hg.set_fine_smoother(1 if a.fine_smoother == "tiled" else 0)
     hg.set_warm_start(a.warm_start)
     if a.islands > 0:
-        hg.set_island_deflation(a.islands, 8)
+        hg.set_island_deflation(a.islands, a.island_min)
     hg.set_graphs(not a.no_graphs)
     learned = None
     if a.learned > 0:

@@ -347,6 +347,9 @@ int rt_net_clear_samples(rt_net* n);
 /* steps Adam updates (beta 0.9 / 0.999), each on the mean loss of `batch`
  * samples drawn with replacement; mean_loss = the last step's loss */
 int rt_net_train(rt_net* n, int steps, int batch, float lr, uint64_t seed, double* mean_loss);
+/* one Adam update (beta 0.9 / 0.999, eps 1e-8, bias-corrected; the optimiser
+ * state that rt_net_train uses) with a caller-given gradient grad[num_params] */
+int rt_net_adam_step(rt_net* n, const float* grad, float lr);
 /* mean milliseconds of one forward pass on a dims box (HIP events, n passes) */
 int rt_net_time_forward(rt_net* n, const int* dims, int reps, double* ms);
 /* Starts every rt_solve from the energy-optimal combination of the plain

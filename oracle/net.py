@@ -221,3 +221,18 @@ def loss_and_grad(theta, x4, target, emulate=False):
     wb("E0A", c["F"], g_a0)
     vec = np.concatenate([np.concatenate([grads[k][0].ravel(), grads[k][1]]) for k in LAYERS])
     return loss, vec
+
+
+def adam_step(w, m, v, g, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update (Kingma and Ba 2015, Algorithm 1) in float32, the
+    arithmetic order of the GPU kernel: m = b1 m + (1 - b1) g,
+    v = b2 v + (1 - b2) g^2, w -= lr (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps).
+    t is the 1-based step count. Returns new (w, m, v)."""
+    f = np.float32
+    w, m, v, g = (np.asarray(a, np.float32) for a in (w, m, v, g))
+    m = f(b1) * m + f(1 - f(b1)) * g
+    v = f(b2) * v + f(1 - f(b2)) * g * g
+    c1 = f(1) - np.power(f(b1), f(t))
+    c2 = f(1) - np.power(f(b2), f(t))
+    w = w - f(lr) * (m / c1) / (np.sqrt(v / c2) + f(eps))
+    return w, m, v

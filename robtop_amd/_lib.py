@@ -124,6 +124,7 @@ SIGNATURES = {
     "rt_net_num_samples": ([P, P], I),
     "rt_net_clear_samples": ([P], I),
     "rt_net_train": ([P, I, I, F, ctypes.c_uint64, P], I),
+    "rt_net_adam_step": ([P, P, F], I),
     "rt_set_learned_warm_start": ([P, P], I),
     "rt_learned_warm_start_info": ([P, P], I),
     "rt_autotune": ([P, I, P], I),

@@ -333,9 +333,11 @@ void launch_coarse_residual(const Geom& g, Vec3<const T> U, Vec3<const T> F, Vec
     const bool sliced = g.coarse_res < 0 ? colour_blocks(g, 0, 1) < 32768 : g.coarse_res != 0;
     if (sliced) {
         dim3 grid(colour_blocks(g, 0, 64), 8);
+        if (!grid.x) return;
         hipLaunchKernelGGL((k_coarse_res9<T, TS>), grid, dim3(576), 0, s, g, U, F, R, st, vflag);
     } else {
         dim3 grid(colour_blocks(g, 0, 256), 8);
+        if (!grid.x) return;
         hipLaunchKernelGGL((k_coarse_res<T, TS>), grid, dim3(256), 0, s, g, U, F, R, st, vflag);
     }
     RT_HIP(hipGetLastError());
@@ -696,6 +698,7 @@ template <typename T>
 void launch_coarsest(const double* Ainv, int n3, const int* rowslot, int, Vec3<const T> F, Vec3<T> U, double* fbuf,
                      hipStream_t s) {
     const int nrows = n3 / 3;
+    if (nrows <= 0) return;
     hipLaunchKernelGGL((k_gather_rows<T>), dim3((nrows + 255) / 256), dim3(256), 0, s, rowslot, nrows, F, fbuf);
     hipLaunchKernelGGL((k_dense_gemv<T>), dim3(n3), dim3(64), 0, s, Ainv, n3, (const double*)fbuf, rowslot, U);
     RT_HIP(hipGetLastError());

@@ -14,10 +14,11 @@
 // assembled element by element: only elements with some but not all nodes on
 // the island contribute (rigid motions are in the null space of KE).
 //
-// Labels: every stiff element starts with its own slot index; rounds of
-// min-propagation over the six face neighbours and pointer jumping run until
-// no label changes (the host checks a flag), leaving each component labelled
-// with its smallest slot.
+// Labels: union-find over the face links of stiff elements (each element
+// unites with its -x, -y and -z neighbours, linking the larger root under the
+// smaller with atomicMin until the link holds), then every element takes its
+// root. Each component ends labelled with its smallest slot, in three passes
+// and no host round trips.
 #include "rt_internal.h"
 
 namespace rt {
@@ -28,39 +29,58 @@ __global__ void k_isl_init(int N, const float* __restrict__ rho, const uint8_t* 
         lab[e] = ((eflag[e] & EF_ACTIVE) && rho[e] >= theta) ? e : -1;
 }
 
-// one round: each stiff element takes the smallest label among itself and its
-// face neighbours (writes only decrease a label), then follows its label once
-__global__ void k_isl_propagate(Geom g, int* __restrict__ lab, int* __restrict__ changed) {
+// labels are read with device-scope atomic loads: other workgroups relink
+// roots while this one walks, and a plain load could be served stale from L1
+__device__ __forceinline__ int uf_load(int* lab, int x) {
+    return __hip_atomic_load(&lab[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// root of x: labels only decrease and lab[x] <= x, so the walk ends
+__device__ __forceinline__ int uf_find(int* lab, int x) {
+    int y = uf_load(lab, x);
+    while (y != x) {
+        x = y;
+        y = uf_load(lab, x);
+    }
+    return x;
+}
+
+// unite the components of stiff elements a and b: repeatedly find both roots
+// and hang the larger under the smaller; atomicMin only lowers a root's label,
+// and a failed link (the root was relinked meanwhile) retries from the label
+// it returned
+__device__ __forceinline__ void uf_unite(int* lab, int a, int b) {
+    for (;;) {
+        a = uf_find(lab, a);
+        b = uf_find(lab, b);
+        if (a == b) return;
+        if (a > b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        const int old = atomicMin(&lab[b], a);
+        if (old == b) return;
+        b = old;
+    }
+}
+
+__global__ void k_isl_union(Geom g, int* __restrict__ lab) {
     const int ex = g.n[0] - 1, ey = g.n[1] - 1, ez = g.n[2] - 1;
     const int tot = ex * ey * ez;
-    int any = 0;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += gridDim.x * blockDim.x) {
         const int x = t % ex, y = (t / ex) % ey, z = t / (ex * ey);
         const int e = oct_p(g, x, y, z);
-        int l = lab[e];
-        if (l < 0) continue;
-        const int l0 = l;
-        int nb[6] = {x > 0 ? oct_p(g, x - 1, y, z) : -1, x + 1 < ex ? oct_p(g, x + 1, y, z) : -1,
-                     y > 0 ? oct_p(g, x, y - 1, z) : -1, y + 1 < ey ? oct_p(g, x, y + 1, z) : -1,
-                     z > 0 ? oct_p(g, x, y, z - 1) : -1, z + 1 < ez ? oct_p(g, x, y, z + 1) : -1};
-#pragma unroll
-        for (int k = 0; k < 6; k++) {
-            if (nb[k] < 0) continue;
-            const int m = lab[nb[k]];
-            if (m >= 0 && m < l) l = m;
-        }
-#pragma unroll
-        for (int jump = 0; jump < 4; jump++) {       // follow the label chain
-            const int j = lab[l];
-            if (j >= 0 && j < l) l = j;
-        }
-        if (l < l0) {
-            atomicMin(&lab[e], l);
-            atomicMin(&lab[l0], l);        // hook the old root too
-            any = 1;
-        }
+        if (lab[e] < 0) continue;
+        if (x > 0) { const int n = oct_p(g, x - 1, y, z); if (lab[n] >= 0) uf_unite(lab, e, n); }
+        if (y > 0) { const int n = oct_p(g, x, y - 1, z); if (lab[n] >= 0) uf_unite(lab, e, n); }
+        if (z > 0) { const int n = oct_p(g, x, y, z - 1); if (lab[n] >= 0) uf_unite(lab, e, n); }
     }
-    if (any) atomicOr(changed, 1);
+}
+
+__global__ void k_isl_flatten(int N, int* __restrict__ lab) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < N; e += gridDim.x * blockDim.x)
+        if (lab[e] >= 0) lab[e] = uf_find(lab, e);
 }
 
 // elements with a support node anchor their component
@@ -325,16 +345,10 @@ int island_build(const Geom& g, const float* rho, const float* rhop, const uint8
     const long long nn = (long long)g.n[0] * g.n[1] * g.n[2];
     hipLaunchKernelGGL(k_isl_init, dim3(grid_for(g.N)), dim3(256), 0, s, g.N, rho, eflag, theta, lab);
     RT_HIP(hipGetLastError());
-    for (int round = 0; round < 4096; round++) {
-        RT_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
-        hipLaunchKernelGGL(k_isl_propagate, dim3(grid_for(ne)), dim3(256), 0, s, g, lab, flag);
-        RT_HIP(hipGetLastError());
-        int h = 0;
-        RT_HIP(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, s));
-        RT_HIP(hipStreamSynchronize(s));
-        if (!h) break;
-        if (round == 4095) throw std::runtime_error("island labelling did not converge");
-    }
+    hipLaunchKernelGGL(k_isl_union, dim3(grid_for(ne)), dim3(256), 0, s, g, lab);
+    RT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_isl_flatten, dim3(grid_for(g.N)), dim3(256), 0, s, g.N, lab);
+    RT_HIP(hipGetLastError());
     RT_HIP(hipMemsetAsync(anchored, 0, (size_t)g.N, s));
     RT_HIP(hipMemsetAsync(count, 0, sizeof(int) * (size_t)g.N, s));
     hipLaunchKernelGGL(k_isl_anchor, dim3(grid_for(ne)), dim3(256), 0, s, g, lab, vflag, anchored, count);

@@ -527,9 +527,13 @@ static void coarsest(Hier& H, Vec3<const T> F, Vec3<T> U) {
 template <typename T>
 // zero_support: F0 and U0 vanish at support nodes (the PCG preconditioner), so
 // the finest smoother runs without neighbour support checks and the
-// prolongation keeps supports at 0
+// prolongation keeps supports at 0.
+// islands: the floating-cluster correction U0 += sum_i Z_i E_i^+ Z_i^T R0 joins
+// the coarse-grid correction of the finest level, from the same residual R0
+// (an additive pair of coarse spaces inside the multiplicative cycle, so the
+// cycle stays symmetric when its smoothing is)
 static void vcycle_core(Hier& H, V3<T>& U0, V3<T>& F0, V3<T>& R0, int pre, int post, bool symmetric,
-                        int coarse_mult = 1, int gamma = 1, bool zero_support = false) {
+                        int coarse_mult = 1, int gamma = 1, bool zero_support = false, bool islands = false) {
     const int n = (int)H.lv.size();
     auto Uv = [&](int i) { return i == 0 ? U0.v() : vecs<T>(H.lv[i]).U.v(); };
     auto Fv = [&](int i) { return i == 0 ? F0.cv() : vecs<T>(H.lv[i]).F.cv(); };
@@ -553,6 +557,10 @@ static void vcycle_core(Hier& H, V3<T>& U0, V3<T>& F0, V3<T>& R0, int pre, int p
         const int visits = i + 1 == n - 1 ? 1 : gamma;
         for (int v = 0; v < visits; v++) self(self, i + 1);
         prolong_from<T>(H, i, Uv(i), cvt(Uv(i + 1)), zs);
+        if (i == 0 && islands)
+            island_correct<T>(H.lv[0].g, H.nisl, H.isl_node.as<int>(), H.isl_cent.as<double>(),
+                              H.isl_Einv.as<double>(), H.isl_zr.as<double>(), H.isl_coef.as<double>(),
+                              cvt(Rv(0)), Uv(0), H.s);
         gs<T>(H, i, Uv(i), Fv(i), post * m, symmetric, zs);
     };
     cyc(cyc, 0);
@@ -572,19 +580,14 @@ static void precondition(Hier& H, LevelVecs<TC>& pc, int pre, int post) {
         return;
     }
     launch_fill<TC>(pc.U.data(), TC(0), n, H.s);
-    vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma, true);
-    island_correct<TC>(H.lv[0].g, H.nisl, H.isl_node.as<int>(), H.isl_cent.as<double>(), H.isl_Einv.as<double>(),
-                       H.isl_zr.as<double>(), H.isl_coef.as<double>(), pc.F.cv(), pc.U.v(), H.s);
+    vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma, true, true);
     if (!H.use_graphs) return;
     H.drop_graph();
     RT_HIP(hipStreamBeginCapture(H.s, hipStreamCaptureModeThreadLocal));
     hipGraph_t g = nullptr;
     try {
         launch_fill<TC>(pc.U.data(), TC(0), n, H.s);
-        vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma, true);
-        island_correct<TC>(H.lv[0].g, H.nisl, H.isl_node.as<int>(), H.isl_cent.as<double>(),
-                           H.isl_Einv.as<double>(), H.isl_zr.as<double>(), H.isl_coef.as<double>(), pc.F.cv(),
-                           pc.U.v(), H.s);
+        vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma, true, true);
     } catch (...) {
         (void)hipStreamEndCapture(H.s, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -2478,18 +2481,8 @@ int rt_voxelize_mesh(const float* coords, int nverts, const int* faces, int nfac
         float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
         for (int i = 0; i < nverts; i++)
             for (int j = 0; j < 3; j++) { lo[j] = std::min(lo[j], coords[3 * i + j]); hi[j] = std::max(hi[j], coords[3 * i + j]); }
-        unsigned padd = 1;
-        for (int i = 0; i < 32; i++) { if ((unsigned)prefer_reso < padd) break; padd <<= 1; }
-        const float len[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
-        const float maxlen = std::max(std::max(len[0], len[1]), len[2]);
         float bmin[3], bmax[3], unit[3];
-        for (int j = 0; j < 3; j++) {   // padd2SatifyResolution (voxelize_main.cpp:312-340), voxinfo (util.h:57-68)
-            const float pl = maxlen * (float)padd / (float)prefer_reso;
-            bmin[j] = lo[j] - (pl - len[j]) / 2;
-            bmax[j] = hi[j] + (pl - len[j]) / 2;
-            unit[j] = (bmax[j] - bmin[j]) / float(padd);
-        }
-        const int R = (int)padd;
+        const int R = voxel_frame(lo, hi, prefer_reso, bmin, bmax, unit);
         const size_t nw = ((size_t)R * R * R + 31) / 32;
         if (maxwords < nw) throw std::invalid_argument("bit buffer too small for R^3 bits");
         std::vector<float> tris((size_t)9 * nfaces);

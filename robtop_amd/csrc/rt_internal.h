@@ -235,6 +235,30 @@ void launch_pm_commit(int nl, const int* slots, const double* fs, double scale, 
 void launch_voxelize(const float* tris, int ntris, const float bmin[3], const float unit[3], int R, uint32_t* bits,
                      hipStream_t s);
 
+// The padded cube of a mesh with bounding box [lo, hi] (padd2SatifyResolution,
+// voxelize_main.cpp:312-340, and voxinfo, util.h:57-68): side R = the smallest
+// power of two above prefer_reso, the box grown about its centre to
+// maxlen R / prefer_reso per axis, unit = box / R. Returns R.
+inline int voxel_frame(const float lo[3], const float hi[3], int prefer_reso, float bmin[3], float bmax[3],
+                       float unit[3]) {
+    unsigned padd = 1;
+    for (int i = 0; i < 32; i++) {
+        if ((unsigned)prefer_reso < padd) break;
+        padd <<= 1;
+    }
+    const float len[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+    float maxlen = len[0];
+    if (len[1] > maxlen) maxlen = len[1];
+    if (len[2] > maxlen) maxlen = len[2];
+    for (int j = 0; j < 3; j++) {
+        const float pl = maxlen * (float)padd / (float)prefer_reso;
+        bmin[j] = lo[j] - (pl - len[j]) / 2;
+        bmax[j] = hi[j] + (pl - len[j]) / 2;
+        unit[j] = (bmax[j] - bmin[j]) / float(padd);
+    }
+    return (int)padd;
+}
+
 // maps the exception being handled to an RT_E* status and records its message
 // for rt_last_error (call only inside a catch block)
 int status_of_current_exception();

@@ -424,6 +424,21 @@ int rt_net_clear_samples(rt_net* n) {
     })
 }
 
+int rt_net_adam_step(rt_net* n, const float* grad, float lr) {
+    RT_NET_TRY({
+        need_ptr(grad);
+        Net& N = N_(n);
+        RT_HIP(hipStreamSynchronize(N.s));
+        RT_HIP(hipMemcpy(N.G.p, grad, sizeof(float) * N.nparam, hipMemcpyHostToDevice));
+        N.adam_t++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        launch_adam(N.nparam, B<float>(N.W), B<float>(N.G), B<float>(N.M), B<float>(N.V), lr, b1, b2, 1e-8f,
+                    1.f - std::pow(b1, (float)N.adam_t), 1.f - std::pow(b2, (float)N.adam_t), 1.f, N.s);
+        pack_all(N, N.s);
+        RT_HIP(hipStreamSynchronize(N.s));
+    })
+}
+
 int rt_net_train(rt_net* n, int steps, int batch, float lr, uint64_t seed, double* mean_loss) {
     RT_NET_TRY({
         const double l = train(N_(n), steps, batch, lr, seed);

@@ -116,6 +116,13 @@ class WarmStartNet:
                                  ctypes.byref(loss)))
         return loss.value
 
+    def adam_step(self, grad, lr=1e-3):
+        """One Adam update with the given gradient (layout of params())."""
+        g = np.ascontiguousarray(grad, np.float32)
+        if g.size != self.num_params:
+            raise ValueError("gradient size differs from the parameter count")
+        check(lib().rt_net_adam_step(self.handle, ptr(g), ctypes.c_float(lr)))
+
     def save(self, path):
         np.save(path, self.params())
 

@@ -11,9 +11,16 @@
 //                  density 0.5; update_stencil, then modifiedPM() (the
 //                  reference's double modifiedPM(void), optimization.cpp:123-261)
 //                  and solveFEM() (void, to 1e-4). Prints the worst compliance.
+//   vox FILE RESO  buildGrids(coords, trifaces) (optimization.cpp:14-30) on the
+//                  triangle mesh in FILE (int32 vertex count, int32 face count,
+//                  float32 xyz per vertex, int32 vertex triple per face) with
+//                  gridreso RESO. Prints the cube side, the box and the finest
+//                  level's element mask (origin, sizes, 64-bit FNV-1a hash).
 #include "robtop_amd.hpp"
 
 #include <cmath>
+#include <cstdint>
+#include <stdexcept>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -79,15 +86,54 @@ static int run_pm(int ex, int ey, int ez) {
     return 0;
 }
 
+static int run_vox(const char* path, int reso) {
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) throw std::runtime_error("cannot open mesh file");
+    int32_t nv = 0, nf = 0;
+    if (std::fread(&nv, 4, 1, fp) != 1 || std::fread(&nf, 4, 1, fp) != 1 || nv < 0 || nf < 0) {
+        std::fclose(fp);
+        throw std::runtime_error("bad mesh header");
+    }
+    std::vector<float> coords((size_t)nv * 3);
+    std::vector<int> faces((size_t)nf * 3);
+    const bool ok = std::fread(coords.data(), 4, coords.size(), fp) == coords.size() &&
+                    std::fread(faces.data(), 4, faces.size(), fp) == faces.size();
+    std::fclose(fp);
+    if (!ok) throw std::runtime_error("short mesh file");
+    ra::setParameters(0.4f, 0.02f, 0.03f, 2.f, 0.5f, 3.f, 1e-3f, reso, 1e6f, 0.4f);
+    ra::buildGrids(coords, faces);
+    ra::grid::HierarchyGrid& grids = ra::getGrids();
+    rt_hier* h = grids.handle();
+    int info[11];
+    if (rt_level_info(h, 0, info) != RT_OK) throw std::runtime_error(rt_last_error());
+    const size_t ne = (size_t)info[5] * info[6] * info[7], nn = (size_t)info[8] * info[9] * info[10];
+    std::vector<uint8_t> eact(ne), vact(nn);
+    if (rt_level_masks(h, 0, eact.data(), vact.data()) != RT_OK) throw std::runtime_error(rt_last_error());
+    uint64_t hash = 1469598103934665603ull;
+    size_t nact = 0;
+    for (uint8_t v : eact) {
+        hash = (hash ^ v) * 1099511628211ull;
+        nact += v;
+    }
+    const auto& box = grids.box();
+    std::printf("{\"R\": %d, \"origin\": [%d, %d, %d], \"elements\": [%d, %d, %d], \"active\": %zu, "
+                "\"hash\": \"%016llx\", \"box\": [%.9g, %.9g, %.9g, %.9g, %.9g, %.9g]}\n",
+                grids.resolution(), info[2], info[3], info[4], info[5], info[6], info[7], nact,
+                (unsigned long long)hash, box[0][0], box[0][1], box[0][2], box[1][0], box[1][1], box[1][2]);
+    grids.release();
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc < 2) {
-        std::printf("usage: %s mma | pm EX EY EZ\n", argv[0]);
+        std::printf("usage: %s mma | pm EX EY EZ | vox FILE RESO\n", argv[0]);
         return 0;
     }
     try {
         const std::string mode = argv[1];
         if (mode == "mma") return run_mma();
         if (mode == "pm" && argc == 5) return run_pm(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]));
+        if (mode == "vox" && argc == 4) return run_vox(argv[2], std::atoi(argv[3]));
         std::fprintf(stderr, "bad arguments\n");
         return 2;
     } catch (const std::exception& e) {

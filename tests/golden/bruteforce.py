@@ -367,16 +367,22 @@ def filter_sens(lv, g, R):
     return out
 
 
-def oc_update(eact, rho, g, Vgoal, step, damp, rhomin):
+def oc_update(eact, rho, g, Vgoal, step, damp, rhomin, shell=None):
+    """OC bisection; shell elements (mask like eact) take density 1 in every
+    trial, so they count as 1 in the volume"""
     act = eact
     gg = np.abs(np.minimum(g[act], 0)).astype(np.float32)
     ro = rho[act].astype(np.float32)
     gmax = np.float32(np.max(np.abs(g[act])))
+    sh = None if shell is None else np.asarray(shell, bool)[act]
 
     def trial(thr):
         rn = ro * np.power(gg / np.float32(thr), np.float32(damp))
         rn = np.clip(rn, ro - np.float32(step), ro + np.float32(step))
-        return np.clip(rn, np.float32(rhomin), np.float32(1)).astype(np.float32)
+        rn = np.clip(rn, np.float32(rhomin), np.float32(1)).astype(np.float32)
+        if sh is not None:
+            rn[sh] = 1
+        return rn
 
     lo, up = np.float32(0), gmax
     itn = 0

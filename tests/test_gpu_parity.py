@@ -325,3 +325,29 @@ def test_sensitivity_filter_oc(pair):
     assert ntr_g == ntr_o and thr_g == thr_o
     assert rel_err(g.get_density(), o.get_rho()) < 2e-6
     assert abs(g.volume() - 0.4) <= 1e-4 + 1e-6 or ntr_g == 31
+
+
+def test_oc_shell_elements(pair):
+    """The OC update's shell branch (optimization.cu:159: shell elements take
+    density 1 in every trial): GPU (rt_set_shell_box) against the oracle."""
+    g, o = pair.g, pair.o
+    lv0 = o.levels[0]
+    e = lv0["e"]
+    act = lv0["eact"].reshape(e[2], e[1], e[0])
+    shell = np.zeros_like(act)
+    shell[:, :, :2] = True          # two element layers at the low-x end
+    shell &= act
+    sh = shell.reshape(-1)
+    rng = np.random.default_rng(21)
+    rho0 = (rng.uniform(0.2, 0.6, lv0["ne"]) * lv0["eact"]).astype(np.float32)
+    sens = (-rng.uniform(0, 1, lv0["ne"]) * lv0["eact"]).astype(np.float32)
+    g.set_density(rho0); o.set_rho(rho0)
+    g.set_shell(sh); o.set_shell(sh)
+    g.set_sensitivity(sens)
+    thr_g, ntr_g = g.update_densities(0.45, 0.03, 0.5, 1e-3)
+    thr_o, ntr_o = o.oc_update(0.45, 0.03, 0.5, 1e-3, sens)
+    assert ntr_g == ntr_o and thr_g == thr_o
+    rg = g.get_density()
+    assert rel_err(rg, o.get_rho()) < 2e-6
+    assert np.all(rg[sh] == 1)
+    g.set_shell(np.zeros_like(sh)); o.set_shell(np.zeros_like(sh))

@@ -182,3 +182,28 @@ def test_gpu_untrained_network_falls_back_or_helps(trained):
     assert info["e_learned"] <= info["e_plain"] or not info["taken"]
     assert st.rel_res < 1.5e-8
     assert rel(hg.get_displacement(), u0) < 1e-5
+
+
+def test_gpu_adam_matches_oracle():
+    """k_adam against the oracle's float32 Adam (oracle/net.py adam_step,
+    pinned by tests/test_net_oracle.py) over four steps from a set state:
+    random gradients over seven decades, exact zeros, a repeated gradient."""
+    net = WarmStartNet(1)
+    n = net.num_params
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal(n).astype(np.float32) * 0.1
+    net.set_params(w)
+    m = np.zeros(n, np.float32)
+    v = np.zeros(n, np.float32)
+    g1 = (rng.standard_normal(n) * 10.0 ** rng.uniform(-4, 3, n)).astype(np.float32)
+    g1[::7] = 0
+    grads = [g1, (rng.standard_normal(n) * 0.01).astype(np.float32), g1, np.zeros(n, np.float32)]
+    lrs = [1e-3, 5e-4, 1e-3, 1e-3]
+    for t, (g, lr) in enumerate(zip(grads, lrs), start=1):
+        print(f"[adam] step {t}", flush=True)
+        net.adam_step(g, lr)
+        w, m, v = on.adam_step(w, m, v, g, t, lr)
+        wg = net.params()
+        # the GPU's sqrtf and division differ from numpy's by an ulp or two
+        assert np.max(np.abs(wg - w)) <= 1e-5 * lr + 2e-7 * np.max(np.abs(w)), t
+    net.close()

@@ -82,3 +82,32 @@ def test_bf16_rounding():
     r = on.bf16(a)
     assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0 + 4 * 2 ** -8 and r[3] == -2.5
     assert abs(r[4] - 1e-30) / 1e-30 < 2 ** -8
+
+
+def test_adam_oracle_closed_forms():
+    """Adam's bias correction makes the first step, and every step of a
+    constant gradient, exactly lr g / (|g| + eps): m_hat = g and v_hat = g^2.
+    A zero gradient on the first step leaves the weights; the update scales
+    with lr and, for |g| >> eps, does not depend on the gradient's scale."""
+    rng = np.random.default_rng(3)
+    n = 1000
+    w0 = rng.standard_normal(n).astype(np.float32)
+    g = (rng.standard_normal(n) * 10.0 ** rng.uniform(-4, 3, n)).astype(np.float32)
+    lr = 1e-3
+    w, m, v = on.adam_step(w0, np.zeros(n), np.zeros(n), g, 1, lr)
+    step = lr * g.astype(np.float64) / (np.abs(g) + 1e-8)
+    assert np.allclose(w0 - w, step, rtol=1e-3, atol=1e-9)
+    for t in range(2, 6):
+        wp = w
+        w, m, v = on.adam_step(w, m, v, g, t, lr)
+        assert np.allclose(wp - w, step, rtol=1e-3, atol=1e-9)
+    w1, _, _ = on.adam_step(w0, np.zeros(n), np.zeros(n), np.zeros(n), 1, lr)
+    assert np.array_equal(w1, w0)
+    w2, _, _ = on.adam_step(w0, np.zeros(n), np.zeros(n), 1e3 * g, 1, 2 * lr)
+    assert np.allclose(w0 - w2, 2 * lr * np.sign(g), rtol=1e-3, atol=1e-9)
+    # a sign change of the gradient after a step reverses the momentum only partly
+    w3, m3, v3 = on.adam_step(np.zeros(n), np.zeros(n), np.zeros(n), np.ones(n), 1, lr)
+    w4, _, _ = on.adam_step(w3, m3, v3, -np.ones(n), 2, lr)
+    mh = (0.9 * 0.1 - 0.1) / (1 - 0.81)
+    vh = (0.999 * 0.001 + 0.001) / (1 - 0.999 ** 2)
+    assert np.allclose(w3 - w4, lr * mh / (np.sqrt(vh) + 1e-8), rtol=1e-4)

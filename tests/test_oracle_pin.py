@@ -206,6 +206,36 @@ def box_mesh(lo, hi):
     return V, Fc
 
 
+def test_oc_shell_elements(setup):
+    """Shell elements are set to density 1 in every bisection trial
+    (optimization.cu:159), so they also count as 1 in the volume."""
+    H, P, fixed, f = setup
+    lv0 = H.levels[0]
+    e = lv0["e"]
+    rng = np.random.default_rng(12)
+    rho0 = (rng.uniform(0.2, 0.6, lv0["ne"]) * lv0["eact"]).astype(np.float32)
+    g = (-rng.uniform(0, 1, lv0["ne"]) * lv0["eact"]).astype(np.float32)
+    act = bf.from_flat(lv0["eact"], e)
+    shell = np.zeros_like(act)
+    shell[0] = shell[-1] = True
+    shell &= act
+    sh = bf.to_flat(shell)
+    assert 0 < sh.sum() < lv0["eact"].sum()
+    H.set_rho(rho0)
+    H.set_shell(sh)
+    thr, nt = H.oc_update(0.45, 0.03, 0.5, 1e-3, g)
+    rb, thrb = bf.oc_update(lv0["eact"], rho0, g, 0.45, 0.03, 0.5, 1e-3, shell=sh)
+    assert abs(thr - thrb) <= 1e-6 * abs(thrb)
+    r = H.get_rho()
+    assert rel_err(r, rb) < 1e-6
+    assert np.all(r[sh] == 1) and np.all(r[lv0["eact"] & ~sh] < 1)
+    # without the shell the multiplier and densities differ
+    H.set_rho(rho0)
+    H.set_shell(np.zeros_like(sh))
+    thr2, _ = H.oc_update(0.45, 0.03, 0.5, 1e-3, g)
+    assert thr2 != thr
+
+
 def test_voxelize_box_exact():
     """A box whose faces avoid voxel centres voxelizes to exactly the voxels
     whose centres it contains."""
