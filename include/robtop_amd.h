@@ -370,6 +370,19 @@ int rt_learned_warm_start_info(const rt_hier* h, double* info);
  * max[3]}. */
 int rt_voxelize_mesh(const float* coords, int nverts, const int* faces, int nfaces, int prefer_reso,
                      uint32_t* bits, size_t maxwords, int* out_reso, float* out_box);
+/* No counterpart in the reference: rt_voxelize_mesh for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). d_coords[3 nverts], d_faces[3 nfaces] and
+ * d_bits[maxwords] are DEVICE arrays of the current device; out_reso[3] and
+ * out_box[6] are host arrays. The result equals rt_voxelize_mesh's. It calls
+ * no hipMalloc/hipFree: the staged triangles and bounding-box partials live
+ * in a static per-device scratch array (about 19 MB, held by each device the
+ * library runs on until the process ends), so nfaces is limited to 2^19
+ * (524288; more returns RT_EARG). Work runs on the null stream and the call
+ * returns after it has finished. Calls that overlap on one device share the
+ * scratch and race: serialise them. */
+int rt_voxelize_mesh_noalloc(const float* d_coords, int nverts, const int* d_faces, int nfaces, int prefer_reso,
+                             uint32_t* d_bits, size_t maxwords, int* out_reso, float* out_box);
 
 /* Autotuner (no counterpart in the reference): on this hierarchy's own sizes,
  * times each launch-shape alternative `reps` times with HIP events and keeps

@@ -6,6 +6,11 @@
 // after wordReverse_g (Grid.cu:2662-2687). One thread per triangle; column
 // flips use atomicXor so overlapping triangles commute.
 #include "rt_internal.h"
+#include "robtop_amd.h"
+
+#include <cstring>
+#include <mutex>
+#include <string>
 
 namespace rt {
 
@@ -83,4 +88,140 @@ void launch_voxelize(const float* tris, int ntris, const float bmin[3], const fl
     RT_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// rt_voxelize_mesh_noalloc: rt_voxelize_mesh on device arrays for callers that
+// cannot allocate device memory. Its scratch (the staged triangles, the
+// bounding-box partials and an index-error flag) lives in a file-scope device
+// array of fixed size; each device holds its own copy, found once per device
+// with hipGetSymbolAddress and used only through that address.
+// ---------------------------------------------------------------------------
+constexpr int kVoxScratchTris = 1 << 19;                 // triangles the scratch stages
+constexpr int kVoxBoxBlocks = 256;                       // bounding-box partials
+constexpr int kVoxHeader = 2048;                         // floats before the triangles
+constexpr size_t kVoxScratchFloats = (size_t)kVoxHeader + (size_t)9 * kVoxScratchTris;
+static_assert(6 * kVoxBoxBlocks + 8 <= kVoxHeader, "scratch header too small");
+
+static __device__ __align__(256) float scratch[kVoxScratchFloats];
+
+// block b: min and max over its grid-stride share of the vertices
+__global__ void __launch_bounds__(256) k_vox_box_partials(const float* __restrict__ coords, int nverts,
+                                                          float* __restrict__ part) {
+    __shared__ float red[6][256];
+    float v[6] = {3.0e38f, 3.0e38f, 3.0e38f, -3.0e38f, -3.0e38f, -3.0e38f};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nverts; i += gridDim.x * blockDim.x)
+        for (int j = 0; j < 3; j++) {
+            const float c = coords[3 * (size_t)i + j];
+            v[j] = fminf(v[j], c);
+            v[3 + j] = fmaxf(v[3 + j], c);
+        }
+    for (int j = 0; j < 6; j++) red[j][threadIdx.x] = v[j];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int j = 0; j < 6; j++)
+                red[j][threadIdx.x] = j < 3 ? fminf(red[j][threadIdx.x], red[j][threadIdx.x + w])
+                                            : fmaxf(red[j][threadIdx.x], red[j][threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// one block: the box from the partials; clears the index-error flag
+__global__ void __launch_bounds__(256) k_vox_box_final(const float* __restrict__ part, float* __restrict__ box,
+                                                       int* __restrict__ flag) {
+    __shared__ float red[6][256];
+    for (int j = 0; j < 6; j++) red[j][threadIdx.x] = part[6 * threadIdx.x + j];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int j = 0; j < 6; j++)
+                red[j][threadIdx.x] = j < 3 ? fminf(red[j][threadIdx.x], red[j][threadIdx.x + w])
+                                            : fmaxf(red[j][threadIdx.x], red[j][threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) box[threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x == 0) *flag = 0;
+}
+
+// tris[t] = the three vertices of face t; an index outside [0, nverts) sets
+// the flag and stages zeros
+__global__ void k_vox_gather(const float* __restrict__ coords, int nverts, const int* __restrict__ faces, int nfaces,
+                             float* __restrict__ tris, int* __restrict__ flag) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nfaces; t += gridDim.x * blockDim.x)
+        for (int k = 0; k < 3; k++) {
+            const int vi = faces[3 * (size_t)t + k];
+            const bool ok = vi >= 0 && vi < nverts;
+            if (!ok) atomicOr(flag, 1);
+            for (int j = 0; j < 3; j++) tris[(size_t)9 * t + 3 * k + j] = ok ? coords[3 * (size_t)vi + j] : 0.f;
+        }
+}
+
+__global__ void k_vox_clear(uint32_t* __restrict__ bits, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        bits[i] = 0u;
+}
+
+// the scratch's device address on each device, looked up once per device
+static float* vox_scratch_ptr() {
+    static std::mutex mu;
+    static float* ptr[256] = {};
+    int dev = 0;
+    RT_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 256) throw std::runtime_error("device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        RT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptr[dev] = (float*)p;
+    }
+    return ptr[dev];
+}
+
 }  // namespace rt
+
+extern "C" int rt_voxelize_mesh_noalloc(const float* d_coords, int nverts, const int* d_faces, int nfaces,
+                                        int prefer_reso, uint32_t* d_bits, size_t maxwords, int* out_reso,
+                                        float* out_box) {
+    using namespace rt;
+    try {
+        if (!d_coords || !d_faces || !d_bits || !out_reso || !out_box) throw std::invalid_argument("null pointer");
+        if (nverts <= 0 || nfaces <= 0 || prefer_reso <= 0) throw std::invalid_argument("empty mesh or bad resolution");
+        if (nfaces > kVoxScratchTris)
+            throw std::invalid_argument("rt_voxelize_mesh_noalloc: " + std::to_string(nfaces) +
+                                        " faces exceed the scratch limit of " + std::to_string(kVoxScratchTris));
+        int R = 1;
+        while (R <= prefer_reso && R < (1 << 30)) R <<= 1;
+        const size_t nw = ((size_t)R * R * R + 31) / 32;
+        if (maxwords < nw) throw std::invalid_argument("bit buffer too small for R^3 bits");
+        float* sc = vox_scratch_ptr();
+        float* part = sc;
+        float* box = sc + 6 * kVoxBoxBlocks;
+        int* flag = (int*)(box + 6);
+        float* tris = sc + kVoxHeader;
+        hipStream_t s = nullptr;
+        hipLaunchKernelGGL(k_vox_box_partials, dim3(kVoxBoxBlocks), dim3(256), 0, s, d_coords, nverts, part);
+        RT_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_vox_box_final, dim3(1), dim3(256), 0, s, part, box, flag);
+        RT_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_vox_gather, dim3(std::min(4096, (nfaces + 255) / 256)), dim3(256), 0, s, d_coords, nverts,
+                           d_faces, nfaces, tris, flag);
+        RT_HIP(hipGetLastError());
+        float hb[7];
+        RT_HIP(hipMemcpy(hb, box, sizeof(float) * 7, hipMemcpyDeviceToHost));
+        int bad = 0;
+        std::memcpy(&bad, hb + 6, sizeof(int));
+        if (bad) throw std::invalid_argument("face index out of range");
+        float bmin[3], bmax[3], unit[3];
+        const int Rf = voxel_frame(hb, hb + 3, prefer_reso, bmin, bmax, unit);
+        if (Rf != R) throw std::logic_error("cube side mismatch");
+        hipLaunchKernelGGL(k_vox_clear, dim3((unsigned)std::min<size_t>(4096, (nw + 255) / 256)), dim3(256), 0, s, d_bits,
+                           nw);
+        RT_HIP(hipGetLastError());
+        launch_voxelize(tris, nfaces, bmin, unit, R, d_bits, s);
+        RT_HIP(hipStreamSynchronize(s));
+        for (int j = 0; j < 3; j++) { out_reso[j] = R; out_box[j] = bmin[j]; out_box[3 + j] = bmax[j]; }
+        return RT_OK;
+    } catch (...) {
+        return status_of_current_exception();
+    }
+}
