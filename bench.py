@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--warm-start", type=int, default=0,
                     help="start each solve from the energy-optimal combination of the last M solutions (0: the "
                          "previous solution, energy-scaled)")
-    ap.add_argument("--islands", type=float, default=0.0, metavar="THETA",
+    ap.add_argument("--islands", type=float, default=0.02, metavar="THETA",
                     help="deflate the rigid-body modes of floating clusters of elements with rho >= THETA in the "
                          "preconditioner (0: off)")
     ap.add_argument("--island-min", type=int, default=1, help="smallest floating cluster deflated (elements)")

@@ -8,45 +8,38 @@
 //   K_E = sum_{k in E} P_k^T A~_k P_k          (coarse element E, fine children k)
 // followed by a stencil assembly over the 8 elements around each coarse node:
 //   * x4: K_E = sum_k rho_k^p M_k with M_k = P_k^T KE P_k precomputed (a
-//     [576 x 64] x [64 x n_E] product), plus, for children touching a support,
+//     [576 x 64] x [64 x n_E] product, one rocBLAS DGEMM), plus, for children touching a support,
 //     P_k^T (A~_k - rho_k^p KE) P_k where A~_k has the support rows/columns
 //     zeroed and 1e6 on their diagonal (Grid.cu:14, :483-488);
 //   * x2: K_E' = sum_{c<8} W_c^T K_c W_c (W_c the child's trilinear weights).
 // Coarse element matrices are stored entry-major, KEc[(24*row + col) * nE + e].
 #include "rt_internal.h"
 
+#include <algorithm>
+
 namespace rt {
 
-// One thread per coarse element. M is [64][576]; its reads are wave-uniform
-// (scalar loads). rho^p of the 64 children is held in registers.
-__global__ void __launch_bounds__(256) k_galerkin4(Geom gf, int ecx, int ecy, int ecz, const float* __restrict__ rhop,
-                                                   const double* __restrict__ M, double* __restrict__ KEc) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+// The x4 product K_E = sum_k rho_k^p M_k for every coarse element is one GEMM,
+// KEc[576 x nE] = M^T[576 x 64] R[64 x nE], run by rocBLAS (FP64 matrix cores)
+// on the hierarchy's stream. This gathers its right operand: R[k][e] = rho^p of
+// child k = (kx, ky, kz), k = kx + 4 ky + 16 kz, of coarse element e, in FP64.
+__global__ void __launch_bounds__(256) k_children64(Geom gf, int ecx, int ecy, int ecz, const float* __restrict__ rhop,
+                                                    double* __restrict__ R) {
     const int nE = ecx * ecy * ecz;
-    if (e >= nE) return;
-    const int x = e % ecx, y = (e / ecx) % ecy, z = e / (ecx * ecy);
-    float r[64];
-#pragma unroll
-    for (int k = 0; k < 64; k++) r[k] = rhop[oct_p(gf, 4 * x + (k & 3), 4 * y + ((k >> 2) & 3), 4 * z + (k >> 4))];
-    for (int c = 0; c < 24; c++) {
-        double acc[24];
-#pragma unroll
-        for (int j = 0; j < 24; j++) acc[j] = 0;
-#pragma unroll 4
-        for (int k = 0; k < 64; k++) {
-            const double rk = (double)r[k];
-            const double* m = M + (size_t)k * 576 + c * 24;
-#pragma unroll
-            for (int j = 0; j < 24; j++) acc[j] += m[j] * rk;
-        }
-#pragma unroll
-        for (int j = 0; j < 24; j++) KEc[(size_t)(c * 24 + j) * nE + e] = acc[j];
+    const long long tot = 64ll * nE;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < tot;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(t / nE), e = (int)(t % nE);
+        const int x = e % ecx, y = (e / ecx) % ecy, z = e / (ecx * ecy);
+        R[t] = (double)rhop[oct_p(gf, 4 * x + (k & 3), 4 * y + ((k >> 2) & 3), 4 * z + (k >> 4))];
     }
 }
 
-void launch_galerkin4(const Geom& gf, const int ec[3], const float* rhop, const double* M, double* KEc, hipStream_t s) {
-    int nE = ec[0] * ec[1] * ec[2];
-    hipLaunchKernelGGL(k_galerkin4, dim3((nE + 255) / 256), dim3(256), 0, s, gf, ec[0], ec[1], ec[2], rhop, M, KEc);
+void launch_children64(const Geom& gf, const int ec[3], const float* rhop, double* R, hipStream_t s) {
+    const long long tot = 64ll * ec[0] * ec[1] * ec[2];
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(k_children64, dim3((unsigned)std::min<long long>((tot + 255) / 256, 65536)), dim3(256), 0, s, gf,
+                       ec[0], ec[1], ec[2], rhop, R);
     RT_HIP(hipGetLastError());
 }
 

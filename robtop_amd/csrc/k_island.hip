@@ -83,22 +83,41 @@ __global__ void k_isl_flatten(int N, int* __restrict__ lab) {
         if (lab[e] >= 0) lab[e] = uf_find(lab, e);
 }
 
-// elements with a support node anchor their component
+// elements with a support node anchor their component. Element counts and
+// anchor flags are aggregated per wave before the atomics: most lanes of a
+// wave share one label (a large component), and one counter hit by every
+// element of the box would serialise.
 __global__ void k_isl_anchor(Geom g, const int* __restrict__ lab, const uint8_t* __restrict__ vflag,
                              uint8_t* __restrict__ anchored, int* __restrict__ count) {
     const int ex = g.n[0] - 1, ey = g.n[1] - 1, ez = g.n[2] - 1;
     const int tot = ex * ey * ez;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += gridDim.x * blockDim.x) {
-        const int x = t % ex, y = (t / ex) % ey, z = t / (ex * ey);
-        const int e = oct_p(g, x, y, z);
-        const int l = lab[e];
-        if (l < 0) continue;
-        atomicAdd(&count[l], 1);
+    const int lane = (int)__lane_id();
+    for (int t0 = blockIdx.x * blockDim.x; t0 < tot; t0 += gridDim.x * blockDim.x) {
+        const int t = t0 + (int)threadIdx.x;
+        int l = -1;
         bool fixed = false;
+        if (t < tot) {
+            const int x = t % ex, y = (t / ex) % ey, z = t / (ex * ey);
+            l = lab[oct_p(g, x, y, z)];
+            if (l >= 0) {
 #pragma unroll
-        for (int k = 0; k < 8; k++)
-            fixed |= (vflag[oct_p(g, x + (k & 1), y + ((k >> 1) & 1), z + (k >> 2))] & NF_FIXED) != 0;
-        if (fixed) anchored[l] = 1;
+                for (int k = 0; k < 8; k++)
+                    fixed |= (vflag[oct_p(g, x + (k & 1), y + ((k >> 1) & 1), z + (k >> 2))] & NF_FIXED) != 0;
+            }
+        }
+        bool todo = l >= 0;
+        while (__any(todo)) {
+            const int lead = __ffsll((unsigned long long)__ballot(todo)) - 1;
+            const int l0 = __shfl(l, lead);
+            const bool mine = todo && l == l0;
+            const unsigned long long m = __ballot(mine);
+            const unsigned long long mf = __ballot(mine && fixed);
+            if (lane == lead) {
+                atomicAdd(&count[l0], (int)__popcll(m));
+                if (mf) anchored[l0] = 1;
+            }
+            todo = todo && !mine;
+        }
     }
 }
 

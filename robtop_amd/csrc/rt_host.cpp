@@ -77,6 +77,7 @@ struct Hier {
     double E = 1e6, nu = 0.4;
     float p = 3.f;
     DBuf dM;                 // galerkin x4 M_k [64][576]
+    DBuf rh64;               // galerkin x4 GEMM operand: rho^p of the 64 children per coarse element, FP64 [64][nE]
     DBuf xfer;               // scratch of the two-stage x4 restriction (sized for FP64)
     // finest level
     DBuf rho, rhop, eflag, sens, sens_tmp;
@@ -631,6 +632,20 @@ static double sum_partials_host(Hier& H, int np) {
 // ---------------------------------------------------------------------------
 // Galerkin + coarsest
 // ---------------------------------------------------------------------------
+// the hierarchy's rocBLAS handle, on its stream
+static void ensure_blas(Hier& H) {
+    if (!H.blas) {
+        if (rocblas_create_handle(&H.blas) != rocblas_status_success) throw std::runtime_error("rocblas_create_handle failed");
+    }
+    if (rocblas_set_stream(H.blas, H.s) != rocblas_status_success) throw std::runtime_error("rocblas_set_stream failed");
+}
+
+static void gemm(Hier& H, rocblas_operation ta, rocblas_operation tb, int M, int N, int K, double alpha,
+                 const double* A, int lda, const double* B, int ldb, double beta, double* C, int ldc) {
+    if (rocblas_dgemm(H.blas, ta, tb, M, N, K, &alpha, A, lda, B, ldb, &beta, C, ldc) != rocblas_status_success)
+        throw std::runtime_error("rocblas_dgemm failed");
+}
+
 static void factor_coarsest(Hier& H) {
     Level& L = H.lv.back();
     const size_t N = (size_t)L.g.N;
@@ -670,10 +685,7 @@ static void factor_coarsest(Hier& H) {
     H.Ainv.ensure(sizeof(double) * (size_t)n * n);
     H.dinfo.ensure(sizeof(int));
     RT_HIP(hipMemcpyAsync(H.Ainv.p, K.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice, H.s));
-    if (!H.blas) {
-        if (rocblas_create_handle(&H.blas) != rocblas_status_success) throw std::runtime_error("rocblas_create_handle failed");
-    }
-    if (rocblas_set_stream(H.blas, H.s) != rocblas_status_success) throw std::runtime_error("rocblas_set_stream failed");
+    ensure_blas(H);
     auto info_ok = [&](const char* what) {
         int info = 0;
         RT_HIP(hipMemcpyAsync(&info, H.dinfo.p, sizeof(int), hipMemcpyDeviceToHost, H.s));
@@ -728,7 +740,16 @@ static void update_stencil(Hier& H) {
     Level& L0 = H.lv[0];
     launch_rhop(L0.g, H.rho.as<float>(), H.eflag.as<uint8_t>(), H.p, H.rhop.as<float>(), H.s);
     Level& L2 = H.lv[1];
-    launch_galerkin4(L0.g, L2.e, H.rhop.as<float>(), H.dM.as<double>(), L2.KEc.as<double>(), H.s);
+    {
+        // KEc[576 x nE] (row stride nE) = M^T R: column-major C(nE x 576) = R(nE x 64) M(64 x 576), where M is
+        // stored [64][576], i.e. column-major 576 x 64 read transposed
+        const int nE = L2.e[0] * L2.e[1] * L2.e[2];
+        H.rh64.ensure(sizeof(double) * 64 * (size_t)nE);
+        launch_children64(L0.g, L2.e, H.rhop.as<float>(), H.rh64.as<double>(), H.s);
+        ensure_blas(H);
+        gemm(H, rocblas_operation_none, rocblas_operation_transpose, nE, 576, 64, 1.0, H.rh64.as<double>(), nE,
+             H.dM.as<double>(), 576, 0.0, L2.KEc.as<double>(), nE);
+    }
     launch_galerkin4_fix(L0.g, L2.e, H.rhop.as<float>(), L0.vflag.as<uint8_t>(), H.eflag.as<uint8_t>(),
                          H.fixce.as<int>(), H.nfixce, L2.KEc.as<double>(), H.s);
     for (size_t li = 2; li < H.lv.size(); li++)
@@ -2133,11 +2154,6 @@ static void jacobi_eigen(int n, std::vector<double>& A, std::vector<double>& w, 
 // against it are rocBLAS GEMMs on the handle's stream; the block tridiagonal T
 // and its eigenproblem ((j + 1) k square) stay on the host.
 // ---------------------------------------------------------------------------
-static void gemm(Hier& H, rocblas_operation ta, rocblas_operation tb, int M, int N, int K, double alpha,
-                 const double* A, int lda, const double* B, int ldb, double beta, double* C, int ldc) {
-    if (rocblas_dgemm(H.blas, ta, tb, M, N, K, &alpha, A, lda, B, ldb, &beta, C, ldc) != rocblas_status_success)
-        throw std::runtime_error("rocblas_dgemm failed");
-}
 static double ddot(Hier& H, int n, const double* x, const double* y) {
     double r = 0;
     if (rocblas_ddot(H.blas, n, x, 1, y, 1, &r) != rocblas_status_success) throw std::runtime_error("rocblas_ddot failed");

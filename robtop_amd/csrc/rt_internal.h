@@ -204,7 +204,9 @@ void launch_oct_to_box(const Geom& g, const T* oct, T* box, int node_or_elem, hi
 
 // k_galerkin.hip
 void set_ke_galerkin(const double* ke576, hipStream_t s);
-void launch_galerkin4(const Geom& gf, const int ec[3], const float* rhop, const double* M, double* KEc, hipStream_t s);
+// R[64][nE]: FP64 rho^p of the 64 children of every x4 coarse element (the
+// right operand of the Galerkin GEMM KEc = M^T R)
+void launch_children64(const Geom& gf, const int ec[3], const float* rhop, double* R, hipStream_t s);
 void launch_galerkin4_fix(const Geom& gf, const int ec[3], const float* rhop, const uint8_t* vflag,
                           const uint8_t* eflag, const int* ce, int nce, double* KEc, hipStream_t s);
 void launch_galerkin2(const double* KEf, const int ef[3], double* KEp, const int ep[3], hipStream_t s);
