@@ -277,7 +277,23 @@ int rt_modified_pm(rt_hier* h, int max_itn, double* worst_compliance, int* vcycl
 typedef struct rt_lanczos_stats {
     int blocks;             /* Lanczos blocks built */
     int solves;             /* operator applications (PCG solves), the final one included */
-    int pcg_iterations;     /* PCG iterations over all solves */
+    int pcg_iterations;     /* No counterpart in the reference (it solves one load at a time): nb <= 8
+ * load cases K u_b = f_b on the same hierarchy in one batched solve, from
+ * zero. f_box and u_box hold nb finest node-box vectors of 3 nn values each
+ * (the layout of rt_set_force_box); support and absent nodes of f are
+ * ignored. Preconditioned CG per load case with FP64 vectors and dots, in
+ * lockstep, preconditioned by one FP32 V(pre, post) cycle for all nb vectors
+ * at once: the coarser levels read each stencil coefficient once per cycle
+ * for all of them, and the cycle is replayed from a HIP graph. iterations[b]
+ * and rel_res[b] (true |f_b - K u_b| / |f_b|) per load case. Needs the V
+ * cycle (gamma 1) and FP32 coarse stencils. */
+int rt_solve_batch(rt_hier* h, int nb, const double* f_box, double* u_box, double tol, int maxit, int pre, int post,
+                   int* iterations, double* rel_res);
+/* whether rt_block_lanczos solves the k right-hand sides of a block together
+ * with the batched solver (1, the default, for k <= 8 and the mixed or
+ * reliable modes) or one after another (0) */
+int rt_set_block_solves(rt_hier* h, int batched);
+/* PCG iterations over all solves */
 } rt_lanczos_stats;
 int rt_block_lanczos(rt_hier* h, int k, int max_blocks, double tol, double solve_tol, int mode, int pre, int post,
                      uint64_t seed, double* worst_compliance, rt_lanczos_stats* stats);

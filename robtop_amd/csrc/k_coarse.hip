@@ -1084,4 +1084,273 @@ RT_INST_PCG(float, float)
 RT_INST_PCG(float, double)
 #undef RT_INST_PCG
 
+// ---------------------------------------------------------------------------
+// Batched forms for nb <= kBatchMax right-hand sides at once (the worst-case
+// search pushes a Lanczos block of load vectors through one V-cycle). Vector b
+// of a batch starts at base + b * bs (bs = 3N), its component c at + c * N.
+// A workgroup loads each stencil coefficient, and the coarsest inverse each
+// matrix entry, once and applies it to all nb vectors; per vector the
+// arithmetic and its order are those of the single-vector kernels, so a
+// batched cycle gives bit for bit the single cycles' results.
+// ---------------------------------------------------------------------------
+template <typename T, typename TS, int C>
+__global__ void __launch_bounds__(576) k_coarse_gs9b(Geom g, T* __restrict__ U, const T* __restrict__ F, int bs,
+                                                     int nb, const TS* __restrict__ st,
+                                                     const uint8_t* __restrict__ vflag, int reverse) {
+    __shared__ T part[kBatchMax][9][3][64];
+    constexpr int cx = C & 1, cy = (C >> 1) & 1, cz = C >> 2;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int dy = w % 3 - 1, dz = w / 3 - 1;
+    int base = 0;
+    bool act = c_colour_thread<C>(g, blockIdx.x * 64 + lane, base);
+    const int v = C * g.cs + base;
+    if (act) act = (vflag[v] & NF_ACTIVE) != 0;
+    const int N = g.N;
+    const int sh = g.h[0] * g.h[1] + g.h[0] + 1;
+    const int vbu = act ? (base - sh) * (int)sizeof(T) : 0, vbs = act ? v * (int)sizeof(TS) : 0;
+    T fo[kBatchMax][3], uc[kBatchMax][3];
+    T S[9];
+    TS a[3][9];
+    int so[3];
+    if (act) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int dx = k - 1;
+            const int nbk = 9 * (dz + 1) + 3 * (dy + 1) + k;
+            const int col = C ^ ((dx & 1) | ((dy & 1) << 1) | ((dz & 1) << 2));
+            so[k] = (col * g.cs + (((cz + dz) >> 1) * g.h[1] + ((cy + dy) >> 1)) * g.h[0] + ((cx + dx) >> 1) + sh) *
+                    (int)sizeof(T);
+            const __amdgpu_buffer_rsrc_t rs = rsrc(st + (size_t)nbk * 9 * N, 9 * N * (int)sizeof(TS));
+#pragma unroll
+            for (int q = 0; q < 9; q++) a[k][q] = bload<TS>(rs, vbs, q * N * (int)sizeof(TS));
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < kBatchMax; b++) {
+        if (b >= nb) break;
+        const T* Ub = U + (size_t)b * bs;
+        T Au[3] = {T(0), T(0), T(0)};
+        if (act && w == 4) {
+            const T* Fb = F + (size_t)b * bs;
+            fo[b][0] = Fb[v]; fo[b][1] = Fb[N + v]; fo[b][2] = Fb[2 * N + v];
+        }
+        if (act) {
+            T u[3][3];
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) u[k][c] = bload<T>(rsrc(Ub + (size_t)c * N, N * (int)sizeof(T)), vbu, so[k]);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                if (w == 4 && k == 1) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) uc[b][c] = u[k][c];
+                    continue;
+                }
+#pragma unroll
+                for (int r = 0; r < 3; r++)
+                    Au[r] += (T)a[k][3 * r + 0] * u[k][0] + (T)a[k][3 * r + 1] * u[k][1] + (T)a[k][3 * r + 2] * u[k][2];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 3; r++) part[b][w][r][lane] = Au[r];
+    }
+    if (act && w == 4)
+#pragma unroll
+        for (int q = 0; q < 9; q++) S[q] = (T)a[1][q];
+    __syncthreads();
+    if (w != 4 || !act) return;
+#pragma unroll
+    for (int b = 0; b < kBatchMax; b++) {
+        if (b >= nb) break;
+        T Au[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            T acc = T(0);
+#pragma unroll
+            for (int q = 0; q < 9; q++) acc += part[b][q][r][lane];
+            Au[r] = acc;
+        }
+        T u[3] = {uc[b][0], uc[b][1], uc[b][2]};
+        const T f[3] = {fo[b][0], fo[b][1], fo[b][2]};
+        if (!reverse) {
+            u[0] = (f[0] - Au[0] - (S[1] * u[1] + S[2] * u[2])) / S[0];
+            u[1] = (f[1] - Au[1] - (S[3] * u[0] + S[5] * u[2])) / S[4];
+            u[2] = (f[2] - Au[2] - (S[6] * u[0] + S[7] * u[1])) / S[8];
+        } else {
+            u[2] = (f[2] - Au[2] - (S[6] * u[0] + S[7] * u[1])) / S[8];
+            u[1] = (f[1] - Au[1] - (S[3] * u[0] + S[5] * u[2])) / S[4];
+            u[0] = (f[0] - Au[0] - (S[1] * u[1] + S[2] * u[2])) / S[0];
+        }
+        T* Ub = U + (size_t)b * bs;
+        Ub[v] = u[0]; Ub[N + v] = u[1]; Ub[2 * N + v] = u[2];
+    }
+}
+
+template <typename T, typename TS>
+__global__ void __launch_bounds__(576) k_coarse_res9b(Geom g, const T* __restrict__ U, const T* __restrict__ F,
+                                                      T* __restrict__ R, int bs, int nb, const TS* __restrict__ st,
+                                                      const uint8_t* __restrict__ vflag) {
+    __shared__ T part[kBatchMax][9][3][64];
+    const int C = blockIdx.y;
+    const int cx = C & 1, cy = (C >> 1) & 1, cz = C >> 2;
+    const int nhx = (g.n[0] - cx + 1) >> 1, nhy = (g.n[1] - cy + 1) >> 1, nhz = (g.n[2] - cz + 1) >> 1;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int dy = w % 3 - 1, dz = w / 3 - 1;
+    const int t = blockIdx.x * 64 + lane;
+    bool act = t < nhx * nhy * nhz;
+    int base = 0, v = 0;
+    if (act) {
+        const int rr = t / nhx;
+        base = ((rr / nhy + 1) * g.h[1] + (rr % nhy + 1)) * g.h[0] + (t % nhx + 1);
+        v = C * g.cs + base;
+        act = (vflag[v] & NF_ACTIVE) != 0;
+    }
+    const int N = g.N;
+    const int sh = g.h[0] * g.h[1] + g.h[0] + 1;
+    const int vbu = act ? (base - sh) * (int)sizeof(T) : 0, vbs = act ? v * (int)sizeof(TS) : 0;
+    TS a[3][9];
+    int so[3];
+    if (act) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int dx = k - 1;
+            const int nbk = 9 * (dz + 1) + 3 * (dy + 1) + k;
+            const int col = C ^ ((dx & 1) | ((dy & 1) << 1) | ((dz & 1) << 2));
+            so[k] = (col * g.cs + (((cz + dz) >> 1) * g.h[1] + ((cy + dy) >> 1)) * g.h[0] + ((cx + dx) >> 1) + sh) *
+                    (int)sizeof(T);
+            const __amdgpu_buffer_rsrc_t rs = rsrc(st + (size_t)nbk * 9 * N, 9 * N * (int)sizeof(TS));
+#pragma unroll
+            for (int q = 0; q < 9; q++) a[k][q] = bload<TS>(rs, vbs, q * N * (int)sizeof(TS));
+        }
+    }
+    T fo[kBatchMax][3];
+#pragma unroll
+    for (int b = 0; b < kBatchMax; b++) {
+        if (b >= nb) break;
+        T Au[3] = {T(0), T(0), T(0)};
+        if (act && w == 0) {
+            const T* Fb = F + (size_t)b * bs;
+            fo[b][0] = Fb[v]; fo[b][1] = Fb[N + v]; fo[b][2] = Fb[2 * N + v];
+        }
+        if (act) {
+            const T* Ub = U + (size_t)b * bs;
+            T u[3][3];
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) u[k][c] = bload<T>(rsrc(Ub + (size_t)c * N, N * (int)sizeof(T)), vbu, so[k]);
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int r = 0; r < 3; r++)
+                    Au[r] += (T)a[k][3 * r + 0] * u[k][0] + (T)a[k][3 * r + 1] * u[k][1] + (T)a[k][3 * r + 2] * u[k][2];
+        }
+#pragma unroll
+        for (int r = 0; r < 3; r++) part[b][w][r][lane] = Au[r];
+    }
+    __syncthreads();
+    if (w != 0 || !act) return;
+#pragma unroll
+    for (int b = 0; b < kBatchMax; b++) {
+        if (b >= nb) break;
+        T* Rb = R + (size_t)b * bs;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            T acc = T(0);
+#pragma unroll
+            for (int q = 0; q < 9; q++) acc += part[b][q][r][lane];
+            Rb[(size_t)r * N + v] = fo[b][r] - acc;
+        }
+    }
+}
+
+template <typename T, typename TS>
+void launch_coarse_gs_batch(const Geom& g, int nb, T* U, const T* F, const TS* st, const uint8_t* vflag, int colour,
+                            bool reverse, hipStream_t s) {
+    if (nb < 1 || nb > kBatchMax) throw std::invalid_argument("batch size out of range");
+    const unsigned blocks = colour_blocks(g, colour, 64);
+    if (!blocks) return;
+    const int bs = 3 * g.N;
+    switch (colour) {
+#define RT_CASE(c) case c: hipLaunchKernelGGL((k_coarse_gs9b<T, TS, c>), dim3(blocks), dim3(576), 0, s, g, U, F, bs, nb, st, vflag, (int)reverse); break;
+        RT_CASE(0) RT_CASE(1) RT_CASE(2) RT_CASE(3) RT_CASE(4) RT_CASE(5) RT_CASE(6) RT_CASE(7)
+#undef RT_CASE
+    }
+    RT_HIP(hipGetLastError());
+}
+
+template <typename T, typename TS>
+void launch_coarse_residual_batch(const Geom& g, int nb, const T* U, const T* F, T* R, const TS* st,
+                                  const uint8_t* vflag, hipStream_t s) {
+    if (nb < 1 || nb > kBatchMax) throw std::invalid_argument("batch size out of range");
+    dim3 grid(colour_blocks(g, 0, 64), 8);
+    if (!grid.x) return;
+    hipLaunchKernelGGL((k_coarse_res9b<T, TS>), grid, dim3(576), 0, s, g, U, F, R, 3 * g.N, nb, st, vflag);
+    RT_HIP(hipGetLastError());
+}
+
+// coarsest for a batch: fb[b][3r + c] = F_b at node r; U_b = Ainv fb[b]
+template <typename T>
+__global__ void k_gather_rows_b(const int* __restrict__ rowslot, int nrows, const T* __restrict__ F, int N, int bs,
+                                int nb, double* __restrict__ fb) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const int s = rowslot[r];
+    for (int b = 0; b < nb; b++) {
+        const T* Fb = F + (size_t)b * bs;
+        double* f = fb + (size_t)b * 3 * nrows;
+        f[3 * r + 0] = (double)Fb[s];
+        f[3 * r + 1] = (double)Fb[N + s];
+        f[3 * r + 2] = (double)Fb[2 * N + s];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_dense_gemv_b(const double* __restrict__ A, int n, const double* __restrict__ f,
+                                                     const int* __restrict__ rowslot, T* __restrict__ U, int N, int bs,
+                                                     int nb) {
+    const int row = blockIdx.x;
+    double s[kBatchMax];
+#pragma unroll
+    for (int b = 0; b < kBatchMax; b++) s[b] = 0;
+    const double* a = A + (size_t)row * n;
+    for (int j = threadIdx.x; j < n; j += 64) {
+        const double aj = a[j];
+#pragma unroll
+        for (int b = 0; b < kBatchMax; b++)
+            if (b < nb) s[b] += aj * f[(size_t)b * n + j];
+    }
+#pragma unroll
+    for (int b = 0; b < kBatchMax; b++) {
+        if (b >= nb) break;
+        double x = s[b];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+        if (threadIdx.x == 0) U[(size_t)b * bs + (size_t)(row % 3) * N + rowslot[row / 3]] = (T)x;
+    }
+}
+
+template <typename T>
+void launch_coarsest_batch(const double* Ainv, int n3, const int* rowslot, int nb, int N, const T* F, T* U,
+                           double* fbuf, hipStream_t s) {
+    if (nb < 1 || nb > kBatchMax) throw std::invalid_argument("batch size out of range");
+    const int nrows = n3 / 3;
+    if (nrows <= 0) return;
+    hipLaunchKernelGGL((k_gather_rows_b<T>), dim3((nrows + 255) / 256), dim3(256), 0, s, rowslot, nrows, F, N, 3 * N,
+                       nb, fbuf);
+    hipLaunchKernelGGL((k_dense_gemv_b<T>), dim3(n3), dim3(64), 0, s, Ainv, n3, (const double*)fbuf, rowslot, U, N,
+                       3 * N, nb);
+    RT_HIP(hipGetLastError());
+}
+
+template void launch_coarse_gs_batch<float, float>(const Geom&, int, float*, const float*, const float*,
+                                                   const uint8_t*, int, bool, hipStream_t);
+template void launch_coarse_residual_batch<float, float>(const Geom&, int, const float*, const float*, float*,
+                                                         const float*, const uint8_t*, hipStream_t);
+template void launch_coarsest_batch<float>(const double*, int, const int*, int, int, const float*, float*, double*,
+                                           hipStream_t);
+
 }  // namespace rt

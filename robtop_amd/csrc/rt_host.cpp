@@ -50,6 +50,19 @@ struct LevelVecs {
     void alloc(int N) { U.alloc(N); F.alloc(N); R.alloc(N); }
 };
 
+// nb vectors of 3N values, vector b at data() + b * 3N (components as in V3)
+template <typename T>
+struct BVec {
+    DBuf b;
+    int N = 0, nb = 0;
+    bool fits(int n, int k) const { return b.p && N == n && nb >= k; }
+    void alloc(int n, int k) { N = n; nb = k; b.alloc(sizeof(T) * 3 * (size_t)n * k); }
+    T* data() const { return b.as<T>(); }
+    T* at(int i) const { return data() + (size_t)i * 3 * N; }
+    Vec3<T> v(int i) const { T* p = at(i); return Vec3<T>{{p, p + N, p + 2 * (size_t)N}}; }
+    Vec3<const T> cv(int i) const { const T* p = at(i); return Vec3<const T>{{p, p + N, p + 2 * (size_t)N}}; }
+};
+
 struct Level {
     int layer = 0, scale = 1;
     int o[3] = {0, 0, 0};
@@ -64,6 +77,7 @@ struct Level {
     DBuf st64, st32, KEc;
     DBuf st16;                            // bfloat16 copy for FP32 preconditioner cycles (optional)
     DBuf tile_act;                        // finest level: mask of residual / apply tiles holding an active node
+    BVec<float> bU, bF, bR;               // batched preconditioner cycle (worst-case block solves)
     size_t nn_box() const { return (size_t)g.n[0] * g.n[1] * g.n[2]; }
     size_t ne_box() const { return (size_t)e[0] * e[1] * e[2]; }
 };
@@ -129,11 +143,21 @@ struct Hier {
     bool use_graphs = true;
     hipGraphExec_t pc_graph = nullptr;
     long long pc_key = -1;
+    // batched solves: FP64 Krylov vectors per right-hand side, the batched
+    // cycle's graph, coarsest gather buffer, per-vector partial sums
+    BVec<double> bx, bb, br, bp, bq;
+    DBuf bfbuf, bpart, bscal;
+    hipGraphExec_t bgraph = nullptr;
+    long long bkey = -1;
+    bool batch_solves = true;        // block Lanczos solves its k right-hand sides together
     bool stencil_ready = false, material_set = false;
     void drop_graph() {
         if (pc_graph) (void)hipGraphExecDestroy(pc_graph);
         pc_graph = nullptr;
         pc_key = -1;
+        if (bgraph) (void)hipGraphExecDestroy(bgraph);
+        bgraph = nullptr;
+        bkey = -1;
     }
     ~Hier() {
         drop_graph();
@@ -2223,6 +2247,238 @@ static void worst_apply(Hier& H, const double* q, double* y, double solve_tol, i
     RT_HIP(hipMemcpyAsync(y, H.pm_fs.p, sizeof(double) * nl3, hipMemcpyDeviceToDevice, H.s));
 }
 
+// ---------------------------------------------------------------------------
+// Batched solves for the worst-case search: nb <= kBatchMax right-hand sides
+// through one preconditioner cycle. The finest level (whose traffic is the
+// vectors themselves) runs the single-vector kernels once per vector; every
+// coarser level runs batched kernels that read each stencil coefficient once
+// for all nb vectors, and the coarsest applies its inverse to all nb at once.
+// The whole batched cycle is captured into one HIP graph. No counterpart in
+// the reference, which solves one load at a time (optimization.cpp:123-261).
+// ---------------------------------------------------------------------------
+static bool batch_supported(const Hier& H) { return H.pc_gamma == 1 && H.coarse_bits == 32; }
+
+static void ensure_batch(Hier& H, int nb) {
+    if (nb < 1 || nb > kBatchMax) throw std::invalid_argument("batch of right-hand sides must be 1..8");
+    bool grown = false;
+    for (Level& L : H.lv)
+        if (!L.bU.fits(L.g.N, nb)) {
+            L.bU.alloc(L.g.N, nb);
+            L.bF.alloc(L.g.N, nb);
+            L.bR.alloc(L.g.N, nb);
+            grown = true;
+        }
+    const int N = H.lv[0].g.N;
+    for (BVec<double>* v : {&H.bx, &H.bb, &H.br, &H.bp, &H.bq})
+        if (!v->fits(N, nb)) { v->alloc(N, nb); grown = true; }
+    H.bfbuf.ensure(sizeof(double) * (size_t)nb * std::max(H.n3, 1));
+    const int npa = std::max(1024, fine_apply_partials_modal(H.lv[0].g));
+    H.bpart.ensure(sizeof(double) * (size_t)nb * npa);
+    H.bscal.ensure(sizeof(double) * kBatchMax);
+    if (grown && H.bgraph) {
+        (void)hipGraphExecDestroy(H.bgraph);
+        H.bgraph = nullptr;
+        H.bkey = -1;
+    }
+}
+
+// V(pre, post) on the nb vectors L0.bF -> L0.bU (L0.bU zero on entry), the
+// preconditioner's cycle (vcycle_core with zero_support and islands)
+static void vcycle_batch(Hier& H, int nb, int pre, int post) {
+    const int n = (int)H.lv.size();
+    auto smooth = [&](int i, int nsw, bool rev) {
+        Level& L = H.lv[i];
+        if (i == 0) {
+            for (int b = 0; b < nb; b++) gs<float>(H, 0, L.bU.v(b), L.bF.cv(b), nsw, rev, true);
+            return;
+        }
+        for (int it = 0; it < nsw; it++)
+            for (int cc = 0; cc < 8; cc++)
+                launch_coarse_gs_batch<float, float>(L.g, nb, L.bU.data(), L.bF.data(), L.st32.as<float>(),
+                                                     L.vflag.as<uint8_t>(), rev ? 7 - cc : cc, rev, H.s);
+    };
+    auto cyc = [&](auto&& self, int i) -> void {
+        Level& L = H.lv[i];
+        if (i == n - 1) {
+            launch_coarsest_batch<float>(H.Ainv.as<double>(), H.n3, H.rowslot.as<int>(), nb, L.g.N, L.bF.data(),
+                                         L.bU.data(), H.bfbuf.as<double>(), H.s);
+            return;
+        }
+        const int m = i == 0 ? 1 : H.pc_mult;
+        smooth(i, pre * m, false);
+        if (i == 0)
+            for (int b = 0; b < nb; b++) residual<float>(H, 0, L.bU.cv(b), L.bF.cv(b), L.bR.v(b));
+        else
+            launch_coarse_residual_batch<float, float>(L.g, nb, L.bU.data(), L.bF.data(), L.bR.data(),
+                                                       L.st32.as<float>(), L.vflag.as<uint8_t>(), H.s);
+        Level& C = H.lv[i + 1];
+        for (int b = 0; b < nb; b++) restrict_to<float>(H, i + 1, L.bR.cv(b), C.bF.v(b));
+        launch_fill<float>(C.bU.data(), 0.f, (size_t)3 * C.g.N * nb, H.s);
+        self(self, i + 1);
+        for (int b = 0; b < nb; b++) prolong_from<float>(H, i, L.bU.v(b), C.bU.cv(b), i == 0);
+        if (i == 0)
+            for (int b = 0; b < nb; b++)
+                island_correct<float>(L.g, H.nisl, H.isl_node.as<int>(), H.isl_cent.as<double>(),
+                                      H.isl_Einv.as<double>(), H.isl_zr.as<double>(), H.isl_coef.as<double>(),
+                                      L.bR.cv(b), L.bU.v(b), H.s);
+        smooth(i, post * m, true);
+    };
+    cyc(cyc, 0);
+}
+
+// L0.bU = M^-1 L0.bF for nb vectors; replayed from a HIP graph after the first
+// (eager) application of a shape
+static void precondition_batch(Hier& H, int nb, int pre, int post) {
+    Level& L0 = H.lv[0];
+    const long long key = (((long long)nb * 64 + pre) * 64 + post) * 64 + H.pc_mult;
+    if (H.use_graphs && H.bgraph && H.bkey == key) {
+        RT_HIP(hipGraphLaunch(H.bgraph, H.s));
+        return;
+    }
+    const size_t n = (size_t)3 * L0.g.N * nb;
+    launch_fill<float>(L0.bU.data(), 0.f, n, H.s);
+    vcycle_batch(H, nb, pre, post);
+    if (!H.use_graphs) return;
+    if (H.bgraph) (void)hipGraphExecDestroy(H.bgraph);
+    H.bgraph = nullptr;
+    H.bkey = -1;
+    RT_HIP(hipStreamBeginCapture(H.s, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    try {
+        launch_fill<float>(L0.bU.data(), 0.f, n, H.s);
+        vcycle_batch(H, nb, pre, post);
+    } catch (...) {
+        (void)hipStreamEndCapture(H.s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    RT_HIP(hipStreamEndCapture(H.s, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    RT_HIP(e);
+    H.bgraph = ex;
+    H.bkey = key;
+}
+
+// out[b] = sum of the np partials of vector b (bpart + b * stride), one sync
+static void batch_sums(Hier& H, int nb, int np, int stride, double* out) {
+    for (int b = 0; b < nb; b++)
+        launch_sum_partials(H.bpart.as<double>() + (size_t)b * stride, np, H.bscal.as<double>() + b, H.s);
+    RT_HIP(hipMemcpyAsync(out, H.bscal.p, sizeof(double) * nb, hipMemcpyDeviceToHost, H.s));
+    RT_HIP(hipStreamSynchronize(H.s));
+}
+
+// nb preconditioned CG solves K x_b = f_b (H.bb -> H.bx, from zero) in
+// lockstep: FP64 Krylov vectors and dots per vector, the batched FP32 cycle as
+// preconditioner; a vector stops updating once its recurrence residual is
+// below tol |f_b|. iters[b] = its iterations, rel[b] = its true residual.
+static void pcg_batch(Hier& H, int nb, double tol, int maxit, int pre, int post, int* iters, double* rel) {
+    if (!H.stencil_ready) throw std::logic_error("update_stencil has not been called");
+    ensure_batch(H, nb);
+    Level& L0 = H.lv[0];
+    const size_t n = (size_t)3 * L0.g.N;
+    const int np = 1024, npa = std::max(1024, fine_apply_partials_modal(L0.g));
+    double nf[kBatchMax], rr[kBatchMax], rz[kBatchMax], pq[kBatchMax];
+    bool act[kBatchMax];
+    for (int b = 0; b < nb; b++) {
+        launch_fill<double>(H.bx.at(b), 0.0, n, H.s);
+        launch_convert<double, double>(H.bb.at(b), H.br.at(b), n, H.s);
+        launch_convert<double, float>(H.bb.at(b), L0.bF.at(b), n, H.s);
+        launch_dot_partials_t<double, double>(H.bb.at(b), H.bb.at(b), n, H.bpart.as<double>() + (size_t)b * npa, np,
+                                              H.s);
+        iters[b] = 0;
+    }
+    batch_sums(H, nb, np, npa, nf);
+    int live = 0;
+    for (int b = 0; b < nb; b++) {
+        nf[b] = std::sqrt(nf[b]);
+        act[b] = nf[b] > 0;
+        live += act[b];
+    }
+    if (live) {
+        precondition_batch(H, nb, pre, post);
+        for (int b = 0; b < nb; b++) {
+            launch_dot_partials_t<double, float>(H.br.at(b), L0.bU.at(b), n, H.bpart.as<double>() + (size_t)b * npa,
+                                                 np, H.s);
+            launch_convert<float, double>(L0.bU.at(b), H.bp.at(b), n, H.s);
+        }
+        batch_sums(H, nb, np, npa, rz);
+    }
+    for (int it = 1; it <= maxit && live; it++) {
+        int npq = 0;
+        for (int b = 0; b < nb; b++) {
+            if (!act[b]) continue;
+            launch_fine_apply_modal<double>(L0.g, H.bp.cv(b), H.bq.v(b), H.rhop.as<float>(), L0.vflag.as<uint8_t>(),
+                                            H.bpart.as<double>() + (size_t)b * npa, &npq, H.s);
+        }
+        batch_sums(H, nb, npq, npa, pq);
+        for (int b = 0; b < nb; b++) {
+            if (!act[b]) continue;
+            if (!(pq[b] > 0)) throw std::domain_error("PCG breakdown: p.Ap <= 0");
+            launch_pcg_update_x_r<double, float>(H.bx.at(b), H.br.at(b), H.bp.at(b), H.bq.at(b), rz[b] / pq[b], n,
+                                                 H.bpart.as<double>() + (size_t)b * npa, np, L0.bF.at(b), H.s);
+        }
+        batch_sums(H, nb, np, npa, rr);
+        live = 0;
+        for (int b = 0; b < nb; b++) {
+            if (!act[b]) continue;
+            iters[b] = it;
+            if (std::sqrt(rr[b]) <= tol * nf[b]) act[b] = false;
+            else live++;
+        }
+        if (!live) break;
+        precondition_batch(H, nb, pre, post);
+        for (int b = 0; b < nb; b++)
+            if (act[b])
+                launch_dot_partials_t<double, float>(H.br.at(b), L0.bU.at(b), n,
+                                                     H.bpart.as<double>() + (size_t)b * npa, np, H.s);
+        double rzn[kBatchMax];
+        batch_sums(H, nb, np, npa, rzn);
+        for (int b = 0; b < nb; b++) {
+            if (!act[b]) continue;
+            launch_pcg_update_p<double, float>(H.bp.at(b), L0.bU.at(b), rzn[b] / rz[b], n, H.s);
+            rz[b] = rzn[b];
+        }
+    }
+    // true residuals |f_b - K x_b| / |f_b|
+    for (int b = 0; b < nb; b++) {
+        launch_fine_apply_modal<double>(L0.g, H.bx.cv(b), H.bq.v(b), H.rhop.as<float>(), L0.vflag.as<uint8_t>(),
+                                        nullptr, nullptr, H.s);
+        launch_sub_conv<double, double>(H.bb.at(b), H.bq.at(b), H.br.at(b), (double*)nullptr, n,
+                                        H.bpart.as<double>() + (size_t)b * npa, np, H.s);
+    }
+    batch_sums(H, nb, np, npa, rr);
+    for (int b = 0; b < nb; b++) rel[b] = nf[b] > 0 ? std::sqrt(rr[b]) / nf[b] : 0.0;
+}
+
+// Y[:, c] = C Q[:, c] for the nb columns of a block (worst_apply for all of
+// them in one batched solve); zero columns give zero
+static void worst_apply_batch(Hier& H, int nb, const double* Q, double* Y, const std::vector<char>& zero,
+                              double solve_tol, int pre, int post, int* iters) {
+    const size_t nl3 = (size_t)3 * H.nload, n = (size_t)3 * H.lv[0].g.N;
+    ensure_batch(H, nb);
+    for (int b = 0; b < nb; b++) {
+        launch_fill<double>(H.bb.at(b), 0.0, n, H.s);
+        if (!zero[b])
+            launch_pm_commit(H.nload, H.load_slots.as<int>(), Q + (size_t)b * nl3, 1.0, H.bb.v(b),
+                             H.pm_old.as<double>(), H.partials.as<double>(), kPmParts, H.s);
+    }
+    int it[kBatchMax];
+    double rel[kBatchMax];
+    pcg_batch(H, nb, solve_tol, 2000, pre, post, it, rel);
+    for (int b = 0; b < nb; b++) {
+        if (zero[b]) {
+            RT_HIP(hipMemsetAsync(Y + (size_t)b * nl3, 0, sizeof(double) * nl3, H.s));
+            continue;
+        }
+        if (!(rel[b] <= 10 * solve_tol)) throw std::runtime_error("worst-case solve did not converge");
+        *iters += it[b];
+        launch_pm_gather(H.nload, H.load_slots.as<int>(), H.bx.cv(b), H.load_normals ? H.load_nrm.as<double>() : nullptr,
+                         Y + (size_t)b * nl3, H.partials.as<double>(), kPmParts, H.s);
+    }
+}
+
 // Block Lanczos with full reorthogonalisation on C restricted to the
 // admissible load space, block size k, from a seeded random block. Stops when
 // the residual of the top Ritz pair, |B_j y_last|, falls below tol * theta, or
@@ -2243,6 +2499,10 @@ static double block_lanczos(Hier& H, int k, int max_blocks, double tol, double s
     const size_t dim = nrm.empty() ? m : nl;   // dimension of the admissible space
     k = (int)std::min<size_t>((size_t)k, dim);
     const int mi = (int)m;
+    // the k right-hand sides of a block are solved together (FP64 Krylov
+    // vectors, batched FP32 cycle) when the cycle has a batched form
+    const bool batched = H.batch_solves && k <= kBatchMax && batch_supported(H) &&
+                         (mode == RT_SOLVE_MIXED || mode == RT_SOLVE_RELIABLE);
     // basis: block j in columns [j k, (j + 1) k); block j + 1 is the work block
     DBuf Qd, Cd, Md, fd;
     Qd.alloc(sizeof(double) * m * (size_t)(max_blocks + 1) * k);
@@ -2285,13 +2545,18 @@ static double block_lanczos(Hier& H, int k, int max_blocks, double tol, double s
     for (j = 0; j < max_blocks; j++) {
         double* blk = col(j * k);
         double* W = col((j + 1) * k);
-        for (int c = 0; c < k; c++) {
-            if (zero[c]) {
-                RT_HIP(hipMemsetAsync(W + (size_t)c * m, 0, sizeof(double) * m, H.s));
-                continue;
+        if (batched) {
+            worst_apply_batch(H, k, blk, W, zero, solve_tol, pre, post, &iters);
+            for (int c = 0; c < k; c++) solves += !zero[c];
+        } else {
+            for (int c = 0; c < k; c++) {
+                if (zero[c]) {
+                    RT_HIP(hipMemsetAsync(W + (size_t)c * m, 0, sizeof(double) * m, H.s));
+                    continue;
+                }
+                worst_apply(H, blk + (size_t)c * m, W + (size_t)c * m, solve_tol, mode, pre, post, &iters);
+                solves++;
             }
-            worst_apply(H, blk + (size_t)c * m, W + (size_t)c * m, solve_tol, mode, pre, post, &iters);
-            solves++;
         }
         // A_j = Q_j^T W (symmetrised); column-major k x k, A(a, b) = q_a . w_b
         std::vector<double> A((size_t)k * k);
@@ -2399,6 +2664,44 @@ int rt_modified_pm(rt_hier* h, int max_itn, double* worst_compliance, int* vcycl
         int nv = 0;
         *worst_compliance = H.vprec == 64 ? modified_pm<double>(H, max_itn, &nv) : modified_pm<float>(H, max_itn, &nv);
         if (vcycles) *vcycles = nv;
+    })
+}
+
+int rt_set_block_solves(rt_hier* h, int batched) {
+    RT_TRY({ H_(h).batch_solves = batched != 0; })
+}
+
+int rt_solve_batch(rt_hier* h, int nb, const double* f_box, double* u_box, double tol, int maxit, int pre, int post,
+                   int* iterations, double* rel_res) {
+    RT_TRY({
+        need(f_box); need(u_box); need(iterations); need(rel_res);
+        Hier& H = H_(h);
+        if (nb < 1 || nb > kBatchMax) throw std::invalid_argument("nb must be 1..8");
+        if (tol <= 0 || maxit < 0 || pre < 0 || post < 0) throw std::invalid_argument("bad tolerance or counts");
+        if (!batch_supported(H))
+            throw std::logic_error("batched solves need a V-cycle (gamma 1) with FP32 coarse stencils");
+        if (!H.stencil_ready) throw std::logic_error("update_stencil has not been called");
+        ensure_batch(H, nb);
+        Level& L0 = H.lv[0];
+        const size_t nbx = L0.nn_box();
+        std::vector<double> box(3 * nbx);
+        for (int b = 0; b < nb; b++) {
+            std::copy(f_box + (size_t)b * 3 * nbx, f_box + (size_t)(b + 1) * 3 * nbx, box.begin());
+            for (size_t v = 0; v < nbx; v++)
+                if (!L0.vact[v] || H.fixed_box[v])
+                    for (int c = 0; c < 3; c++) box[c * nbx + v] = 0;
+            const Vec3<double> fv = H.bb.v(b);
+            for (int c = 0; c < 3; c++) {
+                std::vector<double> comp(box.begin() + c * nbx, box.begin() + (c + 1) * nbx);
+                upload_box<double>(H, L0.g, 0, comp.data(), fv.c[c]);
+            }
+        }
+        pcg_batch(H, nb, tol, maxit, pre, post, iterations, rel_res);
+        for (int b = 0; b < nb; b++) {
+            const Vec3<double> xv = H.bx.v(b);
+            for (int c = 0; c < 3; c++)
+                download_box<double>(H, L0.g, 0, xv.c[c], u_box + (size_t)b * 3 * nbx + (size_t)c * nbx);
+        }
     })
 }
 

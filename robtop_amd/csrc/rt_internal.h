@@ -140,6 +140,18 @@ void launch_coarse_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const TS* st, c
 template <typename T, typename TS>
 void launch_coarse_residual(const Geom& g, Vec3<const T> U, Vec3<const T> F, Vec3<T> R, const TS* st,
                             const uint8_t* vflag, hipStream_t s);
+// batched coarse-level kernels: nb <= kBatchMax vectors, vector b at base + b 3N
+constexpr int kBatchMax = 8;
+template <typename T, typename TS>
+void launch_coarse_gs_batch(const Geom& g, int nb, T* U, const T* F, const TS* st, const uint8_t* vflag, int colour,
+                            bool reverse, hipStream_t s);
+template <typename T, typename TS>
+void launch_coarse_residual_batch(const Geom& g, int nb, const T* U, const T* F, T* R, const TS* st,
+                                  const uint8_t* vflag, hipStream_t s);
+// fbuf must hold nb * n3 doubles
+template <typename T>
+void launch_coarsest_batch(const double* Ainv, int n3, const int* rowslot, int nb, int N, const T* F, T* U,
+                           double* fbuf, hipStream_t s);
 template <typename T>
 void launch_restrict(const Geom& gf, const int of[3], Vec3<const T> Rf, const Geom& gc, const int oc[3], Vec3<T> Fc,
                      const uint8_t* vflag_c, int ratio, T* scratch, hipStream_t s);

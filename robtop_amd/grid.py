@@ -403,6 +403,21 @@ class HierarchyGrid:
         check(lib().rt_modified_pm(self.handle, max_itn, ctypes.byref(c), ctypes.byref(nv)))
         return c.value, nv.value
 
+    def solve_batch(self, f_boxes, tol=1e-6, maxit=1000, pre=2, post=2):
+        """nb <= 8 load cases [nb, 3, nn] solved together from zero (no
+        reference counterpart); returns (u [nb, 3, nn], iterations, rel_res)."""
+        f = as_c(f_boxes, np.float64)
+        nb = f.shape[0]
+        u = np.zeros_like(f)
+        it = np.zeros(nb, np.int32)
+        rel = np.zeros(nb, np.float64)
+        check(lib().rt_solve_batch(self.handle, nb, ptr(f), ptr(u), tol, maxit, pre, post, ptr(it), ptr(rel)))
+        return u, it, rel
+
+    def set_block_solves(self, batched=True):
+        """block_lanczos solves a block's right-hand sides together (True) or one by one"""
+        check(lib().rt_set_block_solves(self.handle, int(bool(batched))))
+
     def block_lanczos(self, k=8, max_blocks=30, tol=1e-6, solve_tol=1e-8, mode=None, pre=2, post=2, seed=0):
         """Top eigenpair of the compliance operator on the admissible loads by
         block Lanczos (no reference counterpart; same answer as modified_pm).

@@ -190,7 +190,10 @@ def test_mesh_problem_supports_loads_and_shell(tmp_path):
     dist = np.min(np.stack([cen[:, 0], 24 - cen[:, 0], cen[:, 1], 12 - cen[:, 1], cen[:, 2], 24 - cen[:, 2]]), axis=0)
     want = eact & (dist * dist < (2 * prob.h) ** 2)
     assert np.array_equal(prob.shell, want) and want.sum() > 0
+    hg.init_rho(0.5)
     hg.update_stencil()
+    st = hg.solve(tol=1e-6, maxit=500)
+    assert st.converged
     hg.close()
 
 
