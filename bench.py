@@ -33,6 +33,8 @@ def parse():
     ap.add_argument("--torus-n", type=int, default=384, help="torus voxels per side")
     ap.add_argument("--no-eigen-check", action="store_true",
                     help="torus workload: skip the FP64 recomputation of the worst compliance (for profiling)")
+    ap.add_argument("--lanczos-sequential", action="store_true",
+                    help="torus: solve a Lanczos block's right-hand sides one after another instead of together")
     ap.add_argument("--lanczos-solve-tol", type=float, default=1e-6,
                     help="relative residual of the PCG solves inside block Lanczos (torus workload)")
     ap.add_argument("--pre", type=int, default=2)
@@ -41,7 +43,7 @@ def parse():
     ap.add_argument("--maxit", type=int, default=5000,
                     help="PCG iteration cap per solve; a timed step that reaches it without the tolerance makes "
                          "the run invalid (exit status 3)")
-    ap.add_argument("--coarse-mult", type=int, default=1, help="sweep multiplier on coarse levels (preconditioner)")
+    ap.add_argument("--coarse-mult", type=int, default=2, help="sweep multiplier on coarse levels (preconditioner)")
     ap.add_argument("--gamma", type=int, default=1, help="preconditioner cycle index: 1 = V, 2 = W")
     ap.add_argument("--fine-smoother", default="pairs", choices=["tiled", "pairs"],
                     help="finest-level sweep: four colour-pair launches (default) or two tiled launches")
@@ -260,6 +262,9 @@ def torus_run(a):
         tuned = hg.autotune(reps=5)
         log(f"autotune: {tuned}")
     t_v = hg.time_vcycle(32, a.pre, a.post, a.vcycles)
+    t_vb = hg.time_vcycle_batch(8, a.pre, a.post, a.vcycles)
+    log(f"V-cycle {t_v:.3f} ms; batched cycle for 8 vectors {t_vb:.3f} ms ({t_vb / 8:.3f} ms per vector)")
+    hg.set_block_solves(not a.lanczos_sequential)
     params = rt.Parameter(volume_ratio=0.3, volume_decrease=0.05)
     lz = dict(k=8, max_blocks=20, tol=1e-6, solve_tol=a.lanczos_solve_tol, pre=a.pre, post=a.post)
     recs = rt.worst_case_topopt(hg, params, n_iter=a.warmup + a.steps, stop=False, method="lanczos", lanczos=lz,
@@ -287,7 +292,9 @@ def torus_run(a):
                                f"filter r=2, OC",
                    "active_elements": ne, "active_nodes": nn, "levels": hg.n_levels,
                    "lanczos_per_iter": [{k: r[k] for k in ("blocks", "solves", "pcg_iterations")} for r in timed],
-                   "autotune": tuned, "vcycle_ms": t_v, "node_box": [int(v) for v in hg[0].nodes]},
+                   "autotune": tuned, "vcycle_ms": t_v, "batched_vcycle_ms_8_vectors": t_vb,
+                   "block_solves": "one after another" if a.lanczos_sequential else "together (batched cycle)",
+                   "node_box": [int(v) for v in hg[0].nodes]},
         "eigen_check": eig,
     }
     hg.close()

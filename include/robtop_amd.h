@@ -430,6 +430,9 @@ int rt_synchronize(rt_hier* h);
  * rt_solve uses) on the current force and returns the mean milliseconds per
  * cycle measured with HIP events on the handle's stream. */
 int rt_time_vcycle(rt_hier* h, int bits, int pre, int post, int n, double* ms_per_cycle);
+/* mean milliseconds of one batched FP32 preconditioner cycle for nb vectors
+ * (the cycle of rt_solve_batch and of block Lanczos's block solves) */
+int rt_time_vcycle_batch(rt_hier* h, int nb, int pre, int post, int n, double* ms_per_cycle);
 /* Times n full fine-level smoothing sweeps (colours 0..7, run as four
  * colour-pair launches) of precision `bits`; returns mean ms per sweep. */
 int rt_time_fine_sweep(rt_hier* h, int bits, int n, double* ms_per_sweep);

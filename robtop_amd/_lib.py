@@ -135,6 +135,7 @@ SIGNATURES = {
     "rt_cg_coefficients": ([P, I, P, P, P], I),
     "rt_synchronize": ([P], I),
     "rt_time_vcycle": ([P, I, I, I, I, P], I),
+    "rt_time_vcycle_batch": ([P, I, I, I, I, P], I),
     "rt_time_fine_sweep": ([P, I, I, P], I),
     "rt_time_fine_residual": ([P, I, I, P], I),
 }

@@ -146,6 +146,7 @@ struct Hier {
     // batched solves: FP64 Krylov vectors per right-hand side, the batched
     // cycle's graph, coarsest gather buffer, per-vector partial sums
     BVec<double> bx, bb, br, bp, bq;
+    BVec<float> bp32, bq32, bdx32;   // reliable-update batched solves: FP32 Krylov vectors and increment
     DBuf bfbuf, bpart, bscal;
     hipGraphExec_t bgraph = nullptr;
     long long bkey = -1;
@@ -2258,6 +2259,11 @@ static void worst_apply(Hier& H, const double* q, double* y, double solve_tol, i
 // ---------------------------------------------------------------------------
 static bool batch_supported(const Hier& H) { return H.pc_gamma == 1 && H.coarse_bits == 32; }
 
+// room per vector for the partial sums of any kernel the batched solves use
+static int batch_part_stride(const Hier& H) {
+    return std::max(std::max(1024, kStreamBlocks), fine_apply_partials_modal(H.lv[0].g));
+}
+
 static void ensure_batch(Hier& H, int nb) {
     if (nb < 1 || nb > kBatchMax) throw std::invalid_argument("batch of right-hand sides must be 1..8");
     bool grown = false;
@@ -2272,8 +2278,9 @@ static void ensure_batch(Hier& H, int nb) {
     for (BVec<double>* v : {&H.bx, &H.bb, &H.br, &H.bp, &H.bq})
         if (!v->fits(N, nb)) { v->alloc(N, nb); grown = true; }
     H.bfbuf.ensure(sizeof(double) * (size_t)nb * std::max(H.n3, 1));
-    const int npa = std::max(1024, fine_apply_partials_modal(H.lv[0].g));
-    H.bpart.ensure(sizeof(double) * (size_t)nb * npa);
+    for (BVec<float>* v : {&H.bp32, &H.bq32, &H.bdx32})
+        if (!v->fits(N, nb)) { v->alloc(N, nb); grown = true; }
+    H.bpart.ensure(sizeof(double) * (size_t)nb * batch_part_stride(H));
     H.bscal.ensure(sizeof(double) * kBatchMax);
     if (grown && H.bgraph) {
         (void)hipGraphExecDestroy(H.bgraph);
@@ -2378,7 +2385,7 @@ static void pcg_batch(Hier& H, int nb, double tol, int maxit, int pre, int post,
     ensure_batch(H, nb);
     Level& L0 = H.lv[0];
     const size_t n = (size_t)3 * L0.g.N;
-    const int np = 1024, npa = std::max(1024, fine_apply_partials_modal(L0.g));
+    const int np = 1024, npa = batch_part_stride(H);
     double nf[kBatchMax], rr[kBatchMax], rz[kBatchMax], pq[kBatchMax];
     bool act[kBatchMax];
     for (int b = 0; b < nb; b++) {
@@ -2452,10 +2459,151 @@ static void pcg_batch(Hier& H, int nb, double tol, int maxit, int pre, int post,
     for (int b = 0; b < nb; b++) rel[b] = nf[b] > 0 ? std::sqrt(rr[b]) / nf[b] : 0.0;
 }
 
+// The same nb solves with reliable updates (pcg_reliable's scheme per
+// vector): FP32 r, p, q and FP32 operator, the solution increment in FP32,
+// flushed into the FP64 solution with an FP64 true residual whenever a
+// vector's FP32 residual has dropped 1/RT_REPLACE_DROP since its last
+// replacement, and before it stops. A vector whose replaced residual comes
+// out 10x above its best one (the FP32 recurrence lost it) is solved again
+// from zero with FP64 Krylov vectors (pcg_batch).
+static void pcg_batch_reliable(Hier& H, int nb, double tol, int maxit, int pre, int post, int* iters, double* rel) {
+    if (!H.stencil_ready) throw std::logic_error("update_stencil has not been called");
+    ensure_batch(H, nb);
+    Level& L0 = H.lv[0];
+    const size_t n = (size_t)3 * L0.g.N;
+    const int np = 1024, ns = kStreamBlocks, npa = batch_part_stride(H);
+    double* part = H.bpart.as<double>();
+    double nf[kBatchMax], rmax[kBatchMax], best[kBatchMax], rz[kBatchMax], pq[kBatchMax], rn[kBatchMax],
+        rt[kBatchMax];
+    bool act[kBatchMax], bad[kBatchMax], dirty[kBatchMax];
+    for (int b = 0; b < nb; b++) {
+        launch_fill<double>(H.bx.at(b), 0.0, n, H.s);
+        launch_fill<float>(H.bdx32.at(b), 0.f, n, H.s);
+        launch_convert<double, float>(H.bb.at(b), L0.bF.at(b), n, H.s);
+        launch_dot_partials_t<double, double>(H.bb.at(b), H.bb.at(b), n, part + (size_t)b * npa, np, H.s);
+        iters[b] = 0;
+        bad[b] = dirty[b] = false;
+    }
+    batch_sums(H, nb, np, npa, nf);
+    int live = 0;
+    for (int b = 0; b < nb; b++) {
+        nf[b] = std::sqrt(nf[b]);
+        rmax[b] = nf[b];
+        best[b] = 1.0;
+        rel[b] = 0.0;
+        act[b] = nf[b] > 0;
+        live += act[b];
+    }
+    // true residuals of the vectors in `which`: r64 = f - K x64, rt[b] = |r64| / |f|
+    auto true_res = [&](const bool* which) {
+        for (int b = 0; b < nb; b++) {
+            if (!which[b]) continue;
+            launch_fine_apply_modal<double>(L0.g, H.bx.cv(b), H.bq.v(b), H.rhop.as<float>(), L0.vflag.as<uint8_t>(),
+                                            nullptr, nullptr, H.s);
+            launch_sub_conv<double, double>(H.bb.at(b), H.bq.at(b), H.br.at(b), (double*)nullptr, n,
+                                            part + (size_t)b * npa, np, H.s);
+        }
+        double ss[kBatchMax];
+        batch_sums(H, nb, np, npa, ss);
+        for (int b = 0; b < nb; b++)
+            if (which[b]) rt[b] = std::sqrt(ss[b]) / nf[b];
+    };
+    if (live) {
+        precondition_batch(H, nb, pre, post);
+        for (int b = 0; b < nb; b++) {
+            launch_dot_partials_t<float, float>(L0.bF.at(b), L0.bU.at(b), n, part + (size_t)b * npa, ns, H.s);
+            launch_convert<float, float>(L0.bU.at(b), H.bp32.at(b), n, H.s);
+        }
+        batch_sums(H, nb, ns, npa, rz);
+    }
+    for (int it = 1; it <= maxit && live; it++) {
+        int npq = 0;
+        for (int b = 0; b < nb; b++)
+            if (act[b])
+                launch_fine_apply_modal<float>(L0.g, H.bp32.cv(b), H.bq32.v(b), H.rhop.as<float>(),
+                                               L0.vflag.as<uint8_t>(), part + (size_t)b * npa, &npq, H.s);
+        batch_sums(H, nb, npq, npa, pq);
+        for (int b = 0; b < nb; b++) {
+            if (!act[b]) continue;
+            if (!(pq[b] > 0)) throw std::domain_error("PCG breakdown: p.Ap <= 0");
+            launch_pcg_ru_dxr(L0.bF.at(b), H.bq32.at(b), H.bdx32.at(b), H.bp32.at(b), rz[b] / pq[b], n,
+                              part + (size_t)b * npa, ns, H.s);
+            dirty[b] = true;
+        }
+        batch_sums(H, nb, ns, npa, rn);
+        bool rep[kBatchMax] = {};
+        int nrep = 0;
+        for (int b = 0; b < nb; b++) {
+            if (!act[b]) continue;
+            iters[b] = it;
+            rn[b] = std::sqrt(rn[b]);
+            if (rn[b] / nf[b] <= tol || rn[b] < RT_REPLACE_DROP * rmax[b] || it == maxit) {
+                launch_pcg_flush_x(H.bx.at(b), H.bdx32.at(b), n, H.s);
+                dirty[b] = false;
+                rep[b] = true;
+                nrep++;
+            }
+        }
+        if (nrep) {
+            true_res(rep);
+            for (int b = 0; b < nb; b++) {
+                if (!rep[b]) continue;
+                rel[b] = rt[b];
+                if (rt[b] <= tol || it == maxit) { act[b] = false; continue; }
+                if (!(rt[b] <= 10 * best[b])) { act[b] = false; bad[b] = true; continue; }
+                best[b] = std::min(best[b], rt[b]);
+                launch_convert<double, float>(H.br.at(b), L0.bF.at(b), n, H.s);
+                rmax[b] = rt[b] * nf[b];
+            }
+        }
+        live = 0;
+        for (int b = 0; b < nb; b++) live += act[b];
+        if (!live) break;
+        precondition_batch(H, nb, pre, post);
+        for (int b = 0; b < nb; b++)
+            if (act[b]) launch_dot_partials_t<float, float>(L0.bF.at(b), L0.bU.at(b), n, part + (size_t)b * npa, ns, H.s);
+        double rzn[kBatchMax];
+        batch_sums(H, nb, ns, npa, rzn);
+        for (int b = 0; b < nb; b++) {
+            if (!act[b]) continue;
+            launch_pcg_update_p<float, float>(H.bp32.at(b), L0.bU.at(b), rzn[b] / rz[b], n, H.s);
+            rz[b] = rzn[b];
+        }
+    }
+    for (int b = 0; b < nb; b++)
+        if (dirty[b]) launch_pcg_flush_x(H.bx.at(b), H.bdx32.at(b), n, H.s);
+    // a vector whose FP32 recurrence failed: all of them again with FP64
+    // Krylov vectors (pcg_batch zero-starts every vector; the others' results
+    // are kept and restored)
+    int nbad = 0;
+    for (int b = 0; b < nb; b++) nbad += bad[b];
+    if (nbad) {
+        int it2[kBatchMax];
+        double rel2[kBatchMax];
+        std::vector<std::vector<double>> keep(nb);
+        for (int b = 0; b < nb; b++)
+            if (!bad[b]) {
+                keep[b].resize(n);
+                RT_HIP(hipMemcpyAsync(keep[b].data(), H.bx.at(b), sizeof(double) * n, hipMemcpyDeviceToHost, H.s));
+            }
+        RT_HIP(hipStreamSynchronize(H.s));
+        pcg_batch(H, nb, tol, maxit, pre, post, it2, rel2);
+        for (int b = 0; b < nb; b++) {
+            if (bad[b]) {
+                iters[b] += it2[b];
+                rel[b] = rel2[b];
+            } else {
+                RT_HIP(hipMemcpyAsync(H.bx.at(b), keep[b].data(), sizeof(double) * n, hipMemcpyHostToDevice, H.s));
+            }
+        }
+        RT_HIP(hipStreamSynchronize(H.s));
+    }
+}
+
 // Y[:, c] = C Q[:, c] for the nb columns of a block (worst_apply for all of
 // them in one batched solve); zero columns give zero
 static void worst_apply_batch(Hier& H, int nb, const double* Q, double* Y, const std::vector<char>& zero,
-                              double solve_tol, int pre, int post, int* iters) {
+                              double solve_tol, int mode, int pre, int post, int* iters) {
     const size_t nl3 = (size_t)3 * H.nload, n = (size_t)3 * H.lv[0].g.N;
     ensure_batch(H, nb);
     for (int b = 0; b < nb; b++) {
@@ -2466,7 +2614,8 @@ static void worst_apply_batch(Hier& H, int nb, const double* Q, double* Y, const
     }
     int it[kBatchMax];
     double rel[kBatchMax];
-    pcg_batch(H, nb, solve_tol, 2000, pre, post, it, rel);
+    if (mode == RT_SOLVE_RELIABLE) pcg_batch_reliable(H, nb, solve_tol, 2000, pre, post, it, rel);
+    else pcg_batch(H, nb, solve_tol, 2000, pre, post, it, rel);
     for (int b = 0; b < nb; b++) {
         if (zero[b]) {
             RT_HIP(hipMemsetAsync(Y + (size_t)b * nl3, 0, sizeof(double) * nl3, H.s));
@@ -2546,7 +2695,7 @@ static double block_lanczos(Hier& H, int k, int max_blocks, double tol, double s
         double* blk = col(j * k);
         double* W = col((j + 1) * k);
         if (batched) {
-            worst_apply_batch(H, k, blk, W, zero, solve_tol, pre, post, &iters);
+            worst_apply_batch(H, k, blk, W, zero, solve_tol, mode, pre, post, &iters);
             for (int c = 0; c < k; c++) solves += !zero[c];
         } else {
             for (int c = 0; c < k; c++) {
@@ -2855,6 +3004,32 @@ int rt_time_vcycle(rt_hier* h, int bits, int pre, int post, int n, double* ms) {
         need(ms);
         Hier& H = H_(h);
         *ms = bits == 64 ? time_vcycle<double>(H, pre, post, n) : time_vcycle<float>(H, pre, post, n);
+    })
+}
+
+int rt_time_vcycle_batch(rt_hier* h, int nb, int pre, int post, int n, double* ms) {
+    RT_TRY({
+        need(ms);
+        Hier& H = H_(h);
+        if (!H.stencil_ready) throw std::logic_error("update_stencil has not been called");
+        if (!batch_supported(H)) throw std::logic_error("the batched cycle needs gamma 1 and FP32 coarse stencils");
+        ensure_batch(H, nb);
+        Level& L0 = H.lv[0];
+        const size_t nn = (size_t)3 * L0.g.N;
+        for (int b = 0; b < nb; b++) launch_convert<double, float>(H.f64.data(), L0.bF.at(b), nn, H.s);
+        precondition_batch(H, nb, pre, post);   // eager first application (and graph capture)
+        hipEvent_t a, e;
+        RT_HIP(hipEventCreate(&a));
+        RT_HIP(hipEventCreate(&e));
+        RT_HIP(hipEventRecord(a, H.s));
+        for (int i = 0; i < n; i++) precondition_batch(H, nb, pre, post);
+        RT_HIP(hipEventRecord(e, H.s));
+        RT_HIP(hipEventSynchronize(e));
+        float t = 0;
+        RT_HIP(hipEventElapsedTime(&t, a, e));
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(e);
+        *ms = t / std::max(1, n);
     })
 }
 

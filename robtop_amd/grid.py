@@ -444,6 +444,12 @@ class HierarchyGrid:
         check(lib().rt_time_fine_sweep(self.handle, bits, n, ctypes.byref(ms)))
         return ms.value
 
+    def time_vcycle_batch(self, nb=8, pre=2, post=2, n=10):
+        """ms of one batched preconditioner cycle for nb vectors"""
+        ms = ctypes.c_double()
+        check(lib().rt_time_vcycle_batch(self.handle, nb, pre, post, n, ctypes.byref(ms)))
+        return ms.value
+
     def time_fine_residual(self, bits=32, n=10):
         ms = ctypes.c_double()
         check(lib().rt_time_fine_residual(self.handle, bits, n, ctypes.byref(ms)))
