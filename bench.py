@@ -43,7 +43,8 @@ def parse():
     ap.add_argument("--maxit", type=int, default=5000,
                     help="PCG iteration cap per solve; a timed step that reaches it without the tolerance makes "
                          "the run invalid (exit status 3)")
-    ap.add_argument("--coarse-mult", type=int, default=2, help="sweep multiplier on coarse levels (preconditioner)")
+    ap.add_argument("--coarse-mult", type=int, default=None,
+                    help="sweep multiplier on coarse levels (preconditioner); default 2 for the cantilever, 1 for the torus")
     ap.add_argument("--gamma", type=int, default=1, help="preconditioner cycle index: 1 = V, 2 = W")
     ap.add_argument("--fine-smoother", default="pairs", choices=["tiled", "pairs"],
                     help="finest-level sweep: four colour-pair launches (default) or two tiled launches")
@@ -321,6 +322,8 @@ def net_forward_run(a):
 
 def main():
     a = parse()
+    if a.coarse_mult is None:
+        a.coarse_mult = 1 if a.workload == "torus" else 2
     if a.pmc_summary:
         print(json.dumps(pmc_summary(*a.pmc_summary, tuple(int(v) for v in a.size.split(",")))))
         return
