@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--coarse-mult", type=int, default=None,
                     help="sweep multiplier on coarse levels (preconditioner); default 2 for the cantilever, 1 for the torus")
     ap.add_argument("--gamma", type=int, default=1, help="preconditioner cycle index: 1 = V, 2 = W")
+    ap.add_argument("--coarse-direct-dofs", type=int, default=6000,
+                    help="the preconditioner solves the level above the coarsest exactly (dense FP64 inverse) "
+                         "when it has at most this many unknowns (0: cycle over every level)")
     ap.add_argument("--fine-smoother", default="pairs", choices=["tiled", "pairs"],
                     help="finest-level sweep: four colour-pair launches (default) or two tiled launches")
     ap.add_argument("--warm-start", type=int, default=0,
@@ -253,6 +256,7 @@ def torus_run(a):
     nl = hg.set_load_nodes(load.astype(np.uint8), normals)
     hg.set_precision(32)
     hg.set_preconditioner(a.coarse_mult, a.gamma)
+    hg.set_coarse_direct_dofs(a.coarse_direct_dofs)
     log(f"hierarchy: {hg.n_levels} levels, {nl} load nodes")
     hg.set_graphs(not a.no_graphs)
     tuned = None
@@ -345,6 +349,7 @@ def main():
     hg, bits, R, fixed, f, rho = build_problem(ex, ey, ez, a.seed)
     log(f"hierarchy: {hg.n_levels} levels")
     hg.set_preconditioner(a.coarse_mult, a.gamma)
+    hg.set_coarse_direct_dofs(a.coarse_direct_dofs)
     if a.coarse_bf16:
         hg.set_coarse_stencil_bits(16)
     hg.set_fine_smoother(1 if a.fine_smoother == "tiled" else 0)
@@ -387,7 +392,6 @@ def main():
     for i in range(hg.n_levels):
         ne, nn = hg.num_active(i)
         levels.append(dict(nodes=nn, elements=ne))
-    coarsest_dofs = 3 * levels[-1]["nodes"]
 
     vgoal, step, damp, rmin, rfilt = 0.4, 0.03, 0.5, 1e-3, 2
     mode = {"mixed": rt.RT_SOLVE_MIXED, "reliable": rt.RT_SOLVE_RELIABLE, "fp64": rt.RT_SOLVE_FP64}[a.mode]
@@ -440,8 +444,10 @@ def main():
     hg.update_stencil()
     t_v = hg.time_vcycle(32, a.pre, a.post, a.vcycles)
     t_sweep = hg.time_fine_sweep(32, 10)
-    B_V = algorithmic_vcycle_bytes(levels, a.pre, a.post, coarsest_dofs, mult=a.coarse_mult, gamma=a.gamma,
-                                   Ps=2 if a.coarse_bf16 else 4)
+    # the preconditioner visits pc_levels levels and solves the last exactly
+    pc_levels = hg.preconditioner_levels()
+    B_V = algorithmic_vcycle_bytes(levels[:pc_levels], a.pre, a.post, 3 * levels[pc_levels - 1]["nodes"],
+                                   mult=a.coarse_mult, gamma=a.gamma, Ps=2 if a.coarse_bf16 else 4)
     # Compulsory bytes of a sweep: u (read + write), f and the flag once per node,
     # rho^p once per element. The tiled smoother runs a sweep as two launches
     # (one z-parity class each, half the nodes), the colour-pair smoother as four
@@ -488,7 +494,8 @@ def main():
         },
         "vcycle": {"ms": t_v, "algorithmic_bytes": B_V, "GBps": B_V / (t_v * 1e-3) / 1e9,
                    "frac_of_peak": B_V / (t_v * 1e-3) / 1e9 / HBM_PEAK_GBS, "pre": a.pre, "post": a.post,
-                   "coarse_mult": a.coarse_mult, "gamma": a.gamma,
+                   "coarse_mult": a.coarse_mult, "gamma": a.gamma, "levels_visited": pc_levels,
+                   "exact_solve_dofs": 3 * levels[pc_levels - 1]["nodes"],
                    "coarse_stencil_bits": 16 if a.coarse_bf16 else 32},
         "roofline": {"kernel": f"{kname} (one of the {n_launch} launches of a finest-level Gauss-Seidel sweep)",
                      "bound": "hbm",

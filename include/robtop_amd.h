@@ -205,6 +205,15 @@ int rt_set_coarse_stencil_bits(rt_hier* h, int bits);
  * coarse level is visited gamma times (1 = V-cycle, 2 = W-cycle). Default
  * (1, 1); no reference counterpart. rt_vcycle is always the reference's V. */
 int rt_set_preconditioner(rt_hier* h, int coarse_sweep_mult, int gamma);
+/* The preconditioner's cycle (rt_solve, rt_solve_batch, the worst-case
+ * solves) stops one level above the coarsest and solves that level exactly
+ * (FP64 dense inverse) when it has at most max_dofs unknowns (default 6000:
+ * at 511 x 256 x 256 this is the 4131-unknown level of 16 x 8 x 8 elements).
+ * 0 keeps the whole hierarchy. rt_vcycle always visits every level. */
+int rt_set_coarse_direct_dofs(rt_hier* h, int max_dofs);
+/* number of levels the preconditioner's cycle visits (the last one solved
+ * exactly); 0 before the first rt_update_stencil */
+int rt_preconditioner_levels(const rt_hier* h, int* levels);
 
 /* Replay the preconditioner cycle of rt_solve as a captured HIP graph (on,
  * the default) or launch its kernels one by one (off); results are identical. */

@@ -94,6 +94,8 @@ SIGNATURES = {
     "rt_voxelize_mesh": ([P, I, P, I, I, P, SZ, P, P], I),
     "rt_solve_batch": ([P, I, P, P, D, I, I, I, P, P], I),
     "rt_set_block_solves": ([P, I], I),
+    "rt_set_coarse_direct_dofs": ([P, I], I),
+    "rt_preconditioner_levels": ([P, ctypes.POINTER(I)], I),
     "rt_voxelize_mesh_noalloc": ([P, I, P, I, I, P, SZ, P, P], I),
     "rt_set_preconditioner": ([P, I, I], I),
     "rt_set_graphs": ([P, I], I),

@@ -729,6 +729,31 @@ void launch_mirror_upper(double* A, int n, hipStream_t s) {
     RT_HIP(hipGetLastError());
 }
 
+// dense n x n (n = 3 nr) matrix of a level from its FP64 stencils: row block r
+// (slot slots[r]) takes neighbour nb's 3 x 3 block at column block
+// nbcol[27 r + nb] (-1: outside the box or inactive). A must be zero on entry.
+__global__ void k_dense_from_stencil(const double* __restrict__ st, size_t N, const int* __restrict__ slots,
+                                     const int* __restrict__ nbcol, int nr, double* __restrict__ A) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= (size_t)nr * 243) return;
+    const int r = (int)(i / 243), k = (int)(i % 243);
+    const int nb = k / 9, a = k % 9 / 3, b = k % 3;
+    const int c = nbcol[27 * r + nb];
+    if (c < 0) return;
+    const size_t n = 3 * (size_t)nr;
+    A[(3 * (size_t)r + a) * n + 3 * (size_t)c + b] = st[(size_t)k * N + slots[r]];
+}
+void launch_dense_from_stencil(const double* st, size_t N, const int* slots, const int* nbcol, int nr, double* A,
+                               hipStream_t s) {
+    const size_t n = 3 * (size_t)nr;
+    RT_HIP(hipMemsetAsync(A, 0, sizeof(double) * n * n, s));
+    const size_t work = (size_t)nr * 243;
+    if (work == 0) return;
+    hipLaunchKernelGGL(k_dense_from_stencil, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, st, N, slots,
+                       nbcol, nr, A);
+    RT_HIP(hipGetLastError());
+}
+
 // ---------------------------------------------------------------------------
 // vector helpers and deterministic two-stage FP64 reductions
 // ---------------------------------------------------------------------------

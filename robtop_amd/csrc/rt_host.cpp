@@ -119,6 +119,12 @@ struct Hier {
     bool load_normals = false;
     int vprec = 64;
     int pc_mult = 1, pc_gamma = 1;   // preconditioner cycle: coarse-level sweep multiplier, cycle index
+    // The preconditioner cycle may stop one level early and solve that level
+    // exactly when it has at most pc_direct_dofs unknowns (pc_levels = the
+    // number of levels its cycle visits); its dense inverse lives here.
+    int pc_direct_dofs = 6000, pc_levels = 0;
+    DBuf pcAinv, pcrowslot, pcfbuf, pcbfbuf;
+    int pcn3 = 0;
     int coarse_bits = 32;            // stencil storage of FP32 coarse-level cycles: 32 or 16 (bfloat16)
     int fine_smoother = 0;           // 0: four colour-pair launches in place; 1: two tiled launches, two buffers
     int ws_m = 0;                    // solves start from the best combination of the last ws_m solutions (0: off)
@@ -560,7 +566,10 @@ template <typename T>
 // cycle stays symmetric when its smoothing is)
 static void vcycle_core(Hier& H, V3<T>& U0, V3<T>& F0, V3<T>& R0, int pre, int post, bool symmetric,
                         int coarse_mult = 1, int gamma = 1, bool zero_support = false, bool islands = false) {
-    const int n = (int)H.lv.size();
+    // the preconditioner (zero_support) may stop a level early and solve it
+    // exactly (pc_levels); the reference-semantics cycle visits every level
+    const int n = zero_support && H.pc_levels > 0 ? H.pc_levels : (int)H.lv.size();
+    const bool early = n < (int)H.lv.size();
     auto Uv = [&](int i) { return i == 0 ? U0.v() : vecs<T>(H.lv[i]).U.v(); };
     auto Fv = [&](int i) { return i == 0 ? F0.cv() : vecs<T>(H.lv[i]).F.cv(); };
     auto Rv = [&](int i) { return i == 0 ? R0.v() : vecs<T>(H.lv[i]).R.v(); };
@@ -570,7 +579,11 @@ static void vcycle_core(Hier& H, V3<T>& U0, V3<T>& F0, V3<T>& R0, int pre, int p
     // levels below the finest smooth coarse_mult times as often
     auto cyc = [&](auto&& self, int i) -> void {
         if (i == n - 1) {
-            coarsest<T>(H, Fv(i), Uv(i));
+            if (early)
+                launch_coarsest<T>(H.pcAinv.as<double>(), H.pcn3, H.pcrowslot.as<int>(), 0, Fv(i), Uv(i),
+                                   H.pcfbuf.as<double>(), H.s);
+            else
+                coarsest<T>(H, Fv(i), Uv(i));
             return;
         }
         const int m = i == 0 ? 1 : coarse_mult;
@@ -671,12 +684,12 @@ static void gemm(Hier& H, rocblas_operation ta, rocblas_operation tb, int M, int
         throw std::runtime_error("rocblas_dgemm failed");
 }
 
-static void factor_coarsest(Hier& H) {
-    Level& L = H.lv.back();
+// dense exact solve of level li: the matrix assembled on the device from its
+// FP64 stencils, Cholesky and inverse there too (rocSOLVER, FP64); Ainv
+// n3 x n3, rowslot[r] = slot of the level's r-th active node
+static void factor_level(Hier& H, int li, DBuf& Ainv, DBuf& rowslot, DBuf& fbuf, int& n3_out) {
+    Level& L = H.lv[li];
     const size_t N = (size_t)L.g.N;
-    std::vector<double> st(243 * N);
-    RT_HIP(hipMemcpyAsync(st.data(), L.st64.p, st.size() * 8, hipMemcpyDeviceToHost, H.s));
-    RT_HIP(hipStreamSynchronize(H.s));
     const int nx = L.g.n[0], ny = L.g.n[1], nz = L.g.n[2];
     std::vector<int> slots, rowof(N, -1);
     for (int z = 0; z < nz; z++)
@@ -688,28 +701,28 @@ static void factor_coarsest(Hier& H) {
                     slots.push_back(s);
                 }
     const int nr = (int)slots.size(), n = 3 * nr;
-    std::vector<double> K((size_t)n * n, 0.0);
+    std::vector<int> nbcol((size_t)27 * nr, -1);
     for (int z = 0; z < nz; z++)
         for (int y = 0; y < ny; y++)
             for (int x = 0; x < nx; x++) {
-                const int s = oct_p(L.g, x, y, z);
-                const int r = rowof[s];
+                const int r = rowof[oct_p(L.g, x, y, z)];
                 if (r < 0) continue;
                 for (int nb = 0; nb < 27; nb++) {
                     const int jx = x + nb % 3 - 1, jy = y + nb / 3 % 3 - 1, jz = z + nb / 9 - 1;
                     if (jx < 0 || jy < 0 || jz < 0 || jx >= nx || jy >= ny || jz >= nz) continue;
-                    const int c = rowof[oct_p(L.g, jx, jy, jz)];
-                    if (c < 0) continue;
-                    for (int a = 0; a < 3; a++)
-                        for (int b = 0; b < 3; b++)
-                            K[(size_t)(3 * r + a) * n + 3 * c + b] = st[((size_t)nb * 9 + 3 * a + b) * N + s];
+                    nbcol[(size_t)27 * r + nb] = rowof[oct_p(L.g, jx, jy, jz)];
                 }
             }
-    // Cholesky and inverse on the device (rocSOLVER, FP64), then mirror the
-    // triangle the inverse was written to so the GEMV can read whole rows
-    H.Ainv.ensure(sizeof(double) * (size_t)n * n);
+    rowslot.ensure(sizeof(int) * (size_t)std::max(nr, 1));
+    DBuf nbc;
+    nbc.alloc(sizeof(int) * std::max<size_t>(nbcol.size(), 1));
+    RT_HIP(hipMemcpyAsync(rowslot.p, slots.data(), sizeof(int) * (size_t)nr, hipMemcpyHostToDevice, H.s));
+    RT_HIP(hipMemcpyAsync(nbc.p, nbcol.data(), sizeof(int) * nbcol.size(), hipMemcpyHostToDevice, H.s));
+    Ainv.ensure(sizeof(double) * std::max<size_t>((size_t)n * n, 1));
+    launch_dense_from_stencil(L.st64.as<double>(), N, rowslot.as<int>(), nbc.as<int>(), nr, Ainv.as<double>(), H.s);
+    // Cholesky and inverse (rocSOLVER, FP64), then mirror the triangle the
+    // inverse was written to so the GEMV can read whole rows
     H.dinfo.ensure(sizeof(int));
-    RT_HIP(hipMemcpyAsync(H.Ainv.p, K.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice, H.s));
     ensure_blas(H);
     auto info_ok = [&](const char* what) {
         int info = 0;
@@ -718,20 +731,31 @@ static void factor_coarsest(Hier& H) {
         if (info != 0)
             throw std::domain_error(std::string("coarsest system: ") + what + " failed (not positive definite; no supports?)");
     };
-    if (rocsolver_dpotrf(H.blas, rocblas_fill_lower, n, H.Ainv.as<double>(), n, H.dinfo.as<int>()) !=
+    if (rocsolver_dpotrf(H.blas, rocblas_fill_lower, n, Ainv.as<double>(), n, H.dinfo.as<int>()) !=
         rocblas_status_success)
         throw std::runtime_error("rocsolver_dpotrf failed");
     info_ok("Cholesky factorisation");
-    if (rocsolver_dpotri(H.blas, rocblas_fill_lower, n, H.Ainv.as<double>(), n, H.dinfo.as<int>()) !=
+    if (rocsolver_dpotri(H.blas, rocblas_fill_lower, n, Ainv.as<double>(), n, H.dinfo.as<int>()) !=
         rocblas_status_success)
         throw std::runtime_error("rocsolver_dpotri failed");
     info_ok("inverse");
-    launch_mirror_upper(H.Ainv.as<double>(), n, H.s);
-    H.n3 = n;
-    H.rowslot.ensure(sizeof(int) * (size_t)nr);
-    H.fbuf.ensure(sizeof(double) * (size_t)n);
-    RT_HIP(hipMemcpyAsync(H.rowslot.p, slots.data(), sizeof(int) * (size_t)nr, hipMemcpyHostToDevice, H.s));
+    launch_mirror_upper(Ainv.as<double>(), n, H.s);
+    n3_out = n;
+    fbuf.ensure(sizeof(double) * (size_t)std::max(n, 1));
     RT_HIP(hipStreamSynchronize(H.s));
+}
+
+static void factor_coarsest(Hier& H) {
+    factor_level(H, (int)H.lv.size() - 1, H.Ainv, H.rowslot, H.fbuf, H.n3);
+    // the preconditioner's cycle stops one level early when that level is
+    // small enough to solve exactly (its smoothing passes cost launches, not
+    // bandwidth); the reference-semantics V-cycle keeps the whole hierarchy
+    const int n = (int)H.lv.size();
+    H.pc_levels = n;
+    if (n >= 3 && 3 * (long long)H.lv[n - 2].n_vact <= H.pc_direct_dofs) {
+        factor_level(H, n - 2, H.pcAinv, H.pcrowslot, H.pcfbuf, H.pcn3);
+        H.pc_levels = n - 1;
+    }
 }
 
 static void build_islands(Hier& H) {
@@ -1809,6 +1833,23 @@ int rt_set_preconditioner(rt_hier* h, int coarse_sweep_mult, int gamma) {
     })
 }
 
+int rt_set_coarse_direct_dofs(rt_hier* h, int max_dofs) {
+    RT_TRY({
+        if (max_dofs < 0) throw std::invalid_argument("max_dofs must be >= 0");
+        Hier& H = H_(h);
+        H.pc_direct_dofs = max_dofs;
+        H.drop_graph();
+        if (H.stencil_ready) factor_coarsest(H);
+    })
+}
+
+int rt_preconditioner_levels(const rt_hier* h, int* levels) {
+    RT_TRY({
+        if (!levels) throw std::invalid_argument("null output");
+        *levels = H_(h).stencil_ready ? H_(h).pc_levels : 0;
+    })
+}
+
 int rt_set_vcycle_precision(rt_hier* h, int bits) {
     RT_TRY({
         if (bits != 32 && bits != 64) throw std::invalid_argument("precision must be 32 or 64");
@@ -2277,7 +2318,7 @@ static void ensure_batch(Hier& H, int nb) {
     const int N = H.lv[0].g.N;
     for (BVec<double>* v : {&H.bx, &H.bb, &H.br, &H.bp, &H.bq})
         if (!v->fits(N, nb)) { v->alloc(N, nb); grown = true; }
-    H.bfbuf.ensure(sizeof(double) * (size_t)nb * std::max(H.n3, 1));
+    H.bfbuf.ensure(sizeof(double) * (size_t)nb * std::max(std::max(H.n3, H.pcn3), 1));
     for (BVec<float>* v : {&H.bp32, &H.bq32, &H.bdx32})
         if (!v->fits(N, nb)) { v->alloc(N, nb); grown = true; }
     H.bpart.ensure(sizeof(double) * (size_t)nb * batch_part_stride(H));
@@ -2292,7 +2333,8 @@ static void ensure_batch(Hier& H, int nb) {
 // V(pre, post) on the nb vectors L0.bF -> L0.bU (L0.bU zero on entry), the
 // preconditioner's cycle (vcycle_core with zero_support and islands)
 static void vcycle_batch(Hier& H, int nb, int pre, int post) {
-    const int n = (int)H.lv.size();
+    const int n = H.pc_levels > 0 ? H.pc_levels : (int)H.lv.size();
+    const bool early = n < (int)H.lv.size();
     auto smooth = [&](int i, int nsw, bool rev) {
         Level& L = H.lv[i];
         if (i == 0) {
@@ -2307,8 +2349,9 @@ static void vcycle_batch(Hier& H, int nb, int pre, int post) {
     auto cyc = [&](auto&& self, int i) -> void {
         Level& L = H.lv[i];
         if (i == n - 1) {
-            launch_coarsest_batch<float>(H.Ainv.as<double>(), H.n3, H.rowslot.as<int>(), nb, L.g.N, L.bF.data(),
-                                         L.bU.data(), H.bfbuf.as<double>(), H.s);
+            launch_coarsest_batch<float>(early ? H.pcAinv.as<double>() : H.Ainv.as<double>(), early ? H.pcn3 : H.n3,
+                                         early ? H.pcrowslot.as<int>() : H.rowslot.as<int>(), nb, L.g.N,
+                                         L.bF.data(), L.bU.data(), H.bfbuf.as<double>(), H.s);
             return;
         }
         const int m = i == 0 ? 1 : H.pc_mult;

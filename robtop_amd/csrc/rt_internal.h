@@ -164,6 +164,8 @@ template <typename T>
 void launch_coarsest(const double* Ainv, int n3, const int* rowslot, int unused, Vec3<const T> F, Vec3<T> U,
                      double* fbuf, hipStream_t s);
 void launch_mirror_upper(double* A, int n, hipStream_t s);
+void launch_dense_from_stencil(const double* st, size_t N, const int* slots, const int* nbcol, int nr, double* A,
+                               hipStream_t s);
 // x = sum_i c[i] h[i], k <= 8 FP64 vectors of length n
 void launch_lincomb(double* x, const double* const* h, const double* c, int k, size_t n, hipStream_t s);
 // floating-cluster deflation (k_island.hip): builds the islands of stiff

@@ -414,6 +414,17 @@ class HierarchyGrid:
         check(lib().rt_solve_batch(self.handle, nb, ptr(f), ptr(u), tol, maxit, pre, post, ptr(it), ptr(rel)))
         return u, it, rel
 
+    def set_coarse_direct_dofs(self, max_dofs=6000):
+        """the preconditioner solves the level above the coarsest exactly when
+        it has at most max_dofs unknowns (0: the whole hierarchy)"""
+        check(lib().rt_set_coarse_direct_dofs(self.handle, int(max_dofs)))
+
+    def preconditioner_levels(self):
+        """levels the preconditioner's cycle visits (0 before update_stencil)"""
+        n = ctypes.c_int()
+        check(lib().rt_preconditioner_levels(self.handle, ctypes.byref(n)))
+        return n.value
+
     def set_block_solves(self, batched=True):
         """block_lanczos solves a block's right-hand sides together (True) or one by one"""
         check(lib().rt_set_block_solves(self.handle, int(bool(batched))))
