@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--coarse-mult", type=int, default=None,
                     help="sweep multiplier on coarse levels (preconditioner); default 2 for the cantilever, 1 for the torus")
     ap.add_argument("--gamma", type=int, default=1, help="preconditioner cycle index: 1 = V, 2 = W")
+    ap.add_argument("--level-sweeps", default=None,
+                    help="per-level sweep multipliers of the preconditioner, 'level:mult,...' (overrides --coarse-mult "
+                         "on those levels); default '1:1' for the cantilever (the x4 level smooths once, the smaller "
+                         "levels --coarse-mult times), none for the torus")
     ap.add_argument("--coarse-direct-dofs", type=int, default=6000,
                     help="the preconditioner solves the level above the coarsest exactly (dense FP64 inverse) "
                          "when it has at most this many unknowns (0: cycle over every level)")
@@ -88,7 +92,7 @@ def parse():
     return ap.parse_args()
 
 
-def algorithmic_vcycle_bytes(levels, pre, post, coarsest_dofs, P=4, mult=1, gamma=1, Ps=4):
+def algorithmic_vcycle_bytes(levels, pre, post, coarsest_dofs, P=4, mult=1, gamma=1, Ps=4, level_mult=None):
     """Compulsory bytes of one preconditioner cycle: each array element an
     operation must read or write, counted once per operation; u/f/r in FP32 (P),
     stencils Ps bytes each (FP32 or bfloat16), 1 flag byte per node, rho^p 4 B per element. Coarse
@@ -107,13 +111,25 @@ def algorithmic_vcycle_bytes(levels, pre, post, coarsest_dofs, P=4, mult=1, gamm
         if i < nlev - 1:
             nl = levels[i]["nodes"]
             per = nl * (243 * Ps + 9 * P + 1)
-            per_visit += mult * (pre + post) * per + per   # sweeps + residual
+            m = (level_mult or {}).get(i, 0) or mult
+            per_visit += m * (pre + post) * per + per   # sweeps + residual
         else:
             per_visit += coarsest_dofs ** 2 * 8
         total += visits * per_visit
         if i < nlev - 2:
             visits *= gamma
     return total
+
+
+def parse_level_sweeps(spec, n_levels):
+    """'1:1,2:4' -> {1: 1, 2: 4}"""
+    out = {}
+    for item in filter(None, spec.split(",")):
+        lv, m = (int(t) for t in item.split(":"))
+        if not 1 <= lv < n_levels:
+            raise SystemExit(f"--level-sweeps: level {lv} is not a coarse level (1..{n_levels - 1})")
+        out[lv] = m
+    return out
 
 
 def build_problem(ex, ey, ez, seed):
@@ -328,6 +344,8 @@ def main():
     a = parse()
     if a.coarse_mult is None:
         a.coarse_mult = 1 if a.workload == "torus" else 2
+    if a.level_sweeps is None:
+        a.level_sweeps = "" if a.workload == "torus" else "1:1"
     if a.pmc_summary:
         print(json.dumps(pmc_summary(*a.pmc_summary, tuple(int(v) for v in a.size.split(",")))))
         return
@@ -350,6 +368,9 @@ def main():
     log(f"hierarchy: {hg.n_levels} levels")
     hg.set_preconditioner(a.coarse_mult, a.gamma)
     hg.set_coarse_direct_dofs(a.coarse_direct_dofs)
+    level_mults = parse_level_sweeps(a.level_sweeps, hg.n_levels)
+    for lv, m in level_mults.items():
+        hg.set_level_sweeps(lv, m)
     if a.coarse_bf16:
         hg.set_coarse_stencil_bits(16)
     hg.set_fine_smoother(1 if a.fine_smoother == "tiled" else 0)
@@ -447,7 +468,8 @@ def main():
     # the preconditioner visits pc_levels levels and solves the last exactly
     pc_levels = hg.preconditioner_levels()
     B_V = algorithmic_vcycle_bytes(levels[:pc_levels], a.pre, a.post, 3 * levels[pc_levels - 1]["nodes"],
-                                   mult=a.coarse_mult, gamma=a.gamma, Ps=2 if a.coarse_bf16 else 4)
+                                   mult=a.coarse_mult, gamma=a.gamma, Ps=2 if a.coarse_bf16 else 4,
+                                   level_mult=level_mults)
     # Compulsory bytes of a sweep: u (read + write), f and the flag once per node,
     # rho^p once per element. The tiled smoother runs a sweep as two launches
     # (one z-parity class each, half the nodes), the colour-pair smoother as four
@@ -495,6 +517,7 @@ def main():
         "vcycle": {"ms": t_v, "algorithmic_bytes": B_V, "GBps": B_V / (t_v * 1e-3) / 1e9,
                    "frac_of_peak": B_V / (t_v * 1e-3) / 1e9 / HBM_PEAK_GBS, "pre": a.pre, "post": a.post,
                    "coarse_mult": a.coarse_mult, "gamma": a.gamma, "levels_visited": pc_levels,
+                   "level_sweeps": {str(k): v for k, v in level_mults.items()},
                    "exact_solve_dofs": 3 * levels[pc_levels - 1]["nodes"],
                    "coarse_stencil_bits": 16 if a.coarse_bf16 else 32},
         "roofline": {"kernel": f"{kname} (one of the {n_launch} launches of a finest-level Gauss-Seidel sweep)",

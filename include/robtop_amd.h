@@ -211,6 +211,10 @@ int rt_set_preconditioner(rt_hier* h, int coarse_sweep_mult, int gamma);
  * at 511 x 256 x 256 this is the 4131-unknown level of 16 x 8 x 8 elements).
  * 0 keeps the whole hierarchy. rt_vcycle always visits every level. */
 int rt_set_coarse_direct_dofs(rt_hier* h, int max_dofs);
+/* Sweep multiplier of one coarse level (1 .. levels-1) in the preconditioner's
+ * cycle, overriding rt_set_preconditioner's coarse_sweep_mult there (0 puts
+ * the level back on it). */
+int rt_set_level_sweeps(rt_hier* h, int level, int mult);
 /* number of levels the preconditioner's cycle visits (the last one solved
  * exactly); 0 before the first rt_update_stencil */
 int rt_preconditioner_levels(const rt_hier* h, int* levels);

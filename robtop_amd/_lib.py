@@ -95,6 +95,7 @@ SIGNATURES = {
     "rt_solve_batch": ([P, I, P, P, D, I, I, I, P, P], I),
     "rt_set_block_solves": ([P, I], I),
     "rt_set_coarse_direct_dofs": ([P, I], I),
+    "rt_set_level_sweeps": ([P, I, I], I),
     "rt_preconditioner_levels": ([P, ctypes.POINTER(I)], I),
     "rt_voxelize_mesh_noalloc": ([P, I, P, I, I, P, SZ, P, P], I),
     "rt_set_preconditioner": ([P, I, I], I),

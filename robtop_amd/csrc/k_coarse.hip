@@ -215,6 +215,125 @@ __global__ void __launch_bounds__(576) k_coarse_gs9(Geom g, Vec3<T> U, Vec3<cons
     U.c[0][v] = u[0]; U.c[1][v] = u[1]; U.c[2][v] = u[2];
 }
 
+// Colour-pair form of k_coarse_gs9 for levels whose rows hold at most 64
+// nodes of one colour: colours C and C^1 (same y, z parity) couple only
+// through the x-neighbours of their own row, so a workgroup owns one row
+// (y, z) of the pair's class and relaxes its nodes of colour CA, then, after a
+// barrier that makes those values visible to the workgroup, its nodes of
+// colour CA^1. Each node's arithmetic is that of k_coarse_gs9, and CA, CA^1
+// are consecutive in the sweep's colour order, so a sweep gives the same bits
+// in half the launches.
+template <typename T, typename TS, int C>
+__device__ __forceinline__ void coarse_gs9_row(const Geom& g, int j, int k, Vec3<T> U, Vec3<const T> F,
+                                               const TS* __restrict__ st, const uint8_t* __restrict__ vflag,
+                                               bool reverse, T (&part)[9][3][64]) {
+    constexpr int cx = C & 1, cy = (C >> 1) & 1, cz = C >> 2;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int dy = w % 3 - 1, dz = w / 3 - 1;
+    const int nhx = (g.n[0] - cx + 1) >> 1;
+    const int base = ((k + 1) * g.h[1] + (j + 1)) * g.h[0] + (lane + 1);
+    bool act = lane < nhx;
+    const int v = C * g.cs + base;
+    if (act) act = (vflag[v] & NF_ACTIVE) != 0;
+    const int N = g.N;
+    const int sh = g.h[0] * g.h[1] + g.h[0] + 1;
+    const int vbu = act ? (base - sh) * (int)sizeof(T) : 0, vbs = act ? v * (int)sizeof(TS) : 0;
+    T fo[3] = {T(0), T(0), T(0)};
+    if (act && w == 4) { fo[0] = F.c[0][v]; fo[1] = F.c[1][v]; fo[2] = F.c[2][v]; }
+    T Au[3] = {T(0), T(0), T(0)};
+    T S[9], uc[3] = {T(0), T(0), T(0)};
+    if (act) {
+        T u[3][3];
+        TS a[3][9];
+#pragma unroll
+        for (int kk = 0; kk < 3; kk++) {
+            const int dx = kk - 1;
+            const int nb = 9 * (dz + 1) + 3 * (dy + 1) + kk;
+            const int col = C ^ ((dx & 1) | ((dy & 1) << 1) | ((dz & 1) << 2));
+            const int so = (col * g.cs + (((cz + dz) >> 1) * g.h[1] + ((cy + dy) >> 1)) * g.h[0] + ((cx + dx) >> 1) +
+                            sh) * (int)sizeof(T);
+            const __amdgpu_buffer_rsrc_t rs = rsrc(st + (size_t)nb * 9 * N, 9 * N * (int)sizeof(TS));
+#pragma unroll
+            for (int c = 0; c < 3; c++) u[kk][c] = bload<T>(rsrc(U.c[c], N * (int)sizeof(T)), vbu, so);
+#pragma unroll
+            for (int q = 0; q < 9; q++) a[kk][q] = bload<TS>(rs, vbs, q * N * (int)sizeof(TS));
+        }
+#pragma unroll
+        for (int kk = 0; kk < 3; kk++) {
+            if (w == 4 && kk == 1) {
+#pragma unroll
+                for (int q = 0; q < 9; q++) S[q] = (T)a[kk][q];
+#pragma unroll
+                for (int c = 0; c < 3; c++) uc[c] = u[kk][c];
+                continue;
+            }
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                Au[r] += (T)a[kk][3 * r + 0] * u[kk][0] + (T)a[kk][3 * r + 1] * u[kk][1] + (T)a[kk][3 * r + 2] * u[kk][2];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++) part[w][r][lane] = Au[r];
+    __syncthreads();
+    if (w == 4 && act) {
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            T acc = T(0);
+#pragma unroll
+            for (int q = 0; q < 9; q++) acc += part[q][r][lane];
+            Au[r] = acc;
+        }
+        T u[3] = {uc[0], uc[1], uc[2]};
+        const T f[3] = {fo[0], fo[1], fo[2]};
+        if (!reverse) {
+            u[0] = (f[0] - Au[0] - (S[1] * u[1] + S[2] * u[2])) / S[0];
+            u[1] = (f[1] - Au[1] - (S[3] * u[0] + S[5] * u[2])) / S[4];
+            u[2] = (f[2] - Au[2] - (S[6] * u[0] + S[7] * u[1])) / S[8];
+        } else {
+            u[2] = (f[2] - Au[2] - (S[6] * u[0] + S[7] * u[1])) / S[8];
+            u[1] = (f[1] - Au[1] - (S[3] * u[0] + S[5] * u[2])) / S[4];
+            u[0] = (f[0] - Au[0] - (S[1] * u[1] + S[2] * u[2])) / S[0];
+        }
+        U.c[0][v] = u[0]; U.c[1][v] = u[1]; U.c[2][v] = u[2];
+    }
+}
+
+template <typename T, typename TS, int CA>
+__global__ void __launch_bounds__(576) k_coarse_gs9_pair(Geom g, Vec3<T> U, Vec3<const T> F,
+                                                         const TS* __restrict__ st,
+                                                         const uint8_t* __restrict__ vflag, int reverse) {
+    __shared__ T part[9][3][64];
+    constexpr int cy = (CA >> 1) & 1;
+    const int nhy = (g.n[1] - cy + 1) >> 1;
+    const int j = blockIdx.x % nhy, k = blockIdx.x / nhy;
+    coarse_gs9_row<T, TS, CA>(g, j, k, U, F, st, vflag, reverse != 0, part);
+    __syncthreads();   // the first colour's new values, and part free again
+    coarse_gs9_row<T, TS, CA ^ 1>(g, j, k, U, F, st, vflag, reverse != 0, part);
+}
+
+// whether a level's sweeps can use the colour-pair smoother: at most 64 nodes
+// of a colour per row, and the sliced smoother selected
+static inline bool coarse_pairs_fit(const Geom& g) { return g.coarse_gs == 1 && ((g.n[0] + 1) >> 1) <= 64; }
+
+template <typename T, typename TS>
+void launch_coarse_gs_pair(const Geom& g, Vec3<T> U, Vec3<const T> F, const TS* st, const uint8_t* vflag, int pair,
+                           bool reverse, hipStream_t s) {
+    if (!coarse_pairs_fit(g)) throw std::logic_error("launch_coarse_gs_pair: rows too long or smoother not sliced");
+    const int cy = pair & 1, cz = pair >> 1;
+    const long long rows = (long long)((g.n[1] - cy + 1) >> 1) * ((g.n[2] - cz + 1) >> 1);
+    if (rows <= 0) return;
+    // forward: colour 2 pair first (x even), reverse: 2 pair + 1 first
+    const int ca = 2 * pair + (reverse ? 1 : 0);
+    switch (ca) {
+#define RT_CASE(c) case c: hipLaunchKernelGGL((k_coarse_gs9_pair<T, TS, c>), dim3((unsigned)rows), dim3(576), 0, s, g, U, F, st, vflag, (int)reverse); break;
+        RT_CASE(0) RT_CASE(1) RT_CASE(2) RT_CASE(3) RT_CASE(4) RT_CASE(5) RT_CASE(6) RT_CASE(7)
+#undef RT_CASE
+    }
+    RT_HIP(hipGetLastError());
+}
+
+bool coarse_gs_pairs(const Geom& g) { return coarse_pairs_fit(g); }
 
 template <typename T, typename TS>
 void launch_coarse_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const TS* st, const uint8_t* vflag, int colour,
@@ -1065,6 +1184,8 @@ void launch_oct_to_box(const Geom& g, const T* oct, T* box, int node_or_elem, hi
 // instantiations
 // ---------------------------------------------------------------------------
 #define RT_INST_CS(T, TS)                                                                                           \
+    template void launch_coarse_gs_pair<T, TS>(const Geom&, Vec3<T>, Vec3<const T>, const TS*, const uint8_t*, int, \
+                                               bool, hipStream_t);                                               \
     template void launch_coarse_gs<T, TS>(const Geom&, Vec3<T>, Vec3<const T>, const TS*, const uint8_t*, int, bool, \
                                           hipStream_t);                                                             \
     template void launch_coarse_residual<T, TS>(const Geom&, Vec3<const T>, Vec3<const T>, Vec3<T>, const TS*,      \

@@ -119,6 +119,7 @@ struct Hier {
     bool load_normals = false;
     int vprec = 64;
     int pc_mult = 1, pc_gamma = 1;   // preconditioner cycle: coarse-level sweep multiplier, cycle index
+    std::vector<int> pc_lv_mult;     // per-level overrides of pc_mult (0: pc_mult)
     // The preconditioner cycle may stop one level early and solve that level
     // exactly when it has at most pc_direct_dofs unknowns (pc_levels = the
     // number of levels its cycle visits); its dense inverse lives here.
@@ -497,6 +498,19 @@ static void gs(Hier& H, int li, Vec3<T> U, Vec3<const T> F, int n, bool reverse,
             for (int pp = 0; pp < 4; pp++)
                 launch_fine_gs<T>(L.g, U, F, H.rhop.as<float>(), L.vflag.as<uint8_t>(), reverse ? 3 - pp : pp,
                                   reverse, zero_support, H.s);
+        } else if (coarse_gs_pairs(L.g)) {
+            // four launches, each relaxing the colour pair (2p, 2p+1) row by row
+            for (int pp = 0; pp < 4; pp++) {
+                const int p = reverse ? 3 - pp : pp;
+                if constexpr (std::is_same<T, float>::value) {
+                    if (H.coarse_bits == 16) {
+                        launch_coarse_gs_pair<float, bf16>(L.g, U, F, L.st16.as<bf16>(), L.vflag.as<uint8_t>(), p,
+                                                           reverse, H.s);
+                        continue;
+                    }
+                }
+                launch_coarse_gs_pair<T, T>(L.g, U, F, stencil<T>(L), L.vflag.as<uint8_t>(), p, reverse, H.s);
+            }
         } else {
             for (int cc = 0; cc < 8; cc++) {
                 const int c = reverse ? 7 - cc : cc;
@@ -555,6 +569,11 @@ static void coarsest(Hier& H, Vec3<const T> F, Vec3<T> U) {
     launch_coarsest<T>(H.Ainv.as<double>(), H.n3, H.rowslot.as<int>(), 0, F, U, H.fbuf.as<double>(), H.s);
 }
 
+// sweep multiplier of coarse level i in the preconditioner's cycle
+static int level_mult(const Hier& H, int i) {
+    return i < (int)H.pc_lv_mult.size() && H.pc_lv_mult[i] > 0 ? H.pc_lv_mult[i] : H.pc_mult;
+}
+
 // V(pre, post) with caller-owned finest vectors (Grid.cpp:1068-1113 ordering).
 template <typename T>
 // zero_support: F0 and U0 vanish at support nodes (the PCG preconditioner), so
@@ -586,7 +605,7 @@ static void vcycle_core(Hier& H, V3<T>& U0, V3<T>& F0, V3<T>& R0, int pre, int p
                 coarsest<T>(H, Fv(i), Uv(i));
             return;
         }
-        const int m = i == 0 ? 1 : coarse_mult;
+        const int m = i == 0 ? 1 : zero_support ? level_mult(H, i) : coarse_mult;
         const bool zs = i == 0 && zero_support;
         gs<T>(H, i, Uv(i), Fv(i), pre * m, false, zs);
         residual<T>(H, i, cvt(Uv(i)), Fv(i), Rv(i));
@@ -1833,6 +1852,17 @@ int rt_set_preconditioner(rt_hier* h, int coarse_sweep_mult, int gamma) {
     })
 }
 
+int rt_set_level_sweeps(rt_hier* h, int level, int mult) {
+    RT_TRY({
+        Hier& H = H_(h);
+        if (level < 1 || level >= (int)H.lv.size()) throw std::out_of_range("level must be a coarse level");
+        if (mult < 0 || mult > 16) throw std::invalid_argument("mult must be 0..16");
+        if ((int)H.pc_lv_mult.size() < (int)H.lv.size()) H.pc_lv_mult.resize(H.lv.size(), 0);
+        H.pc_lv_mult[level] = mult;
+        H.drop_graph();
+    })
+}
+
 int rt_set_coarse_direct_dofs(rt_hier* h, int max_dofs) {
     RT_TRY({
         if (max_dofs < 0) throw std::invalid_argument("max_dofs must be >= 0");
@@ -2354,7 +2384,7 @@ static void vcycle_batch(Hier& H, int nb, int pre, int post) {
                                          L.bF.data(), L.bU.data(), H.bfbuf.as<double>(), H.s);
             return;
         }
-        const int m = i == 0 ? 1 : H.pc_mult;
+        const int m = i == 0 ? 1 : level_mult(H, i);
         smooth(i, pre * m, false);
         if (i == 0)
             for (int b = 0; b < nb; b++) residual<float>(H, 0, L.bU.cv(b), L.bF.cv(b), L.bR.v(b));

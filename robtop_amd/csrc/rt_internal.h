@@ -135,6 +135,12 @@ void modal_tile_dims(int* tx, int* ty);
 
 // k_coarse.hip
 template <typename T, typename TS>
+void launch_coarse_gs_pair(const Geom& g, Vec3<T> U, Vec3<const T> F, const TS* st, const uint8_t* vflag, int pair,
+                           bool reverse, hipStream_t s);
+// whether launch_coarse_gs_pair applies to this level (rows of <= 64 nodes of a
+// colour, sliced smoother selected)
+bool coarse_gs_pairs(const Geom& g);
+template <typename T, typename TS>
 void launch_coarse_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const TS* st, const uint8_t* vflag, int colour,
                       bool reverse, hipStream_t s);
 template <typename T, typename TS>

@@ -419,6 +419,11 @@ class HierarchyGrid:
         it has at most max_dofs unknowns (0: the whole hierarchy)"""
         check(lib().rt_set_coarse_direct_dofs(self.handle, int(max_dofs)))
 
+    def set_level_sweeps(self, level, mult):
+        """sweep multiplier of coarse level `level` in the preconditioner (0: the
+        multiplier of set_preconditioner)"""
+        check(lib().rt_set_level_sweeps(self.handle, int(level), int(mult)))
+
     def preconditioner_levels(self):
         """levels the preconditioner's cycle visits (0 before update_stencil)"""
         n = ctypes.c_int()

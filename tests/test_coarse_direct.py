@@ -78,3 +78,39 @@ def test_batch_bit_identical_with_early_solve():
         U1, it1, _ = hg.solve_batch(F[b:b + 1], tol=1e-8, maxit=1000)
         assert it1[0] == it4[b] and np.array_equal(U1[0], U4[b])
     hg.close()
+
+
+def test_level_sweeps_override_the_multiplier():
+    """rt_set_level_sweeps: overriding every coarse level with multiplier m is
+    the same preconditioner, bit for bit, as rt_set_preconditioner(m); a
+    single override changes only its level (different iterates, same
+    tolerance reached)."""
+    import robtop_amd as rt
+    hg, f, _ = _cantilever(seed=5)
+    hg.set_coarse_direct_dofs(0)     # every coarse level smoothed
+    n = len(hg)
+    assert hg.preconditioner_levels() == n
+
+    def solve():
+        hg.set_displacement(np.zeros_like(f))
+        st = hg.solve(tol=1e-8, maxit=1000, mode=rt.RT_SOLVE_RELIABLE, pre=2, post=2)
+        return st.iterations, st.rel_res, hg.get_displacement()
+
+    hg.set_preconditioner(2, 1)
+    i2, _, u2 = solve()
+    hg.set_preconditioner(1, 1)
+    for lv in range(1, n):
+        hg.set_level_sweeps(lv, 2)
+    io, _, uo = solve()
+    assert io == i2 and np.array_equal(uo, u2)
+    for lv in range(1, n):
+        hg.set_level_sweeps(lv, 0)
+    i1, _, u1 = solve()
+    hg.set_level_sweeps(1, 2)
+    im, qm, um = solve()
+    print(f"[sweeps] x1 {i1}, x2 {i2}, level 1 only x2 {im} iterations", flush=True)
+    assert qm < 1.5e-8
+    assert not np.array_equal(um, u1)
+    with pytest.raises(Exception):
+        hg.set_level_sweeps(0, 2)
+    hg.close()
