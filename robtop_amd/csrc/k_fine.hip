@@ -115,9 +115,16 @@ struct GsBufs {
     __amdgpu_buffer_rsrc_t u0, u1, u2, rho, flag;
 };
 
-template <typename T, int C, bool CHECK_FIX, int NB>
+// colour of the neighbour at (DX, DY, DZ) of a colour-C node
+template <int C, int DX, int DY, int DZ>
+constexpr int nb_colour() { return C ^ ((DX & 1) | ((DY & 1) << 1) | ((DZ & 1) << 2)); }
+
+// Z0 ("zero start"): the sweep is the first of a forward cycle from u = 0, so
+// a neighbour whose colour comes later in the order 0..7 still holds 0 and is
+// neither loaded nor multiplied (adding KE * 0 leaves the sums unchanged).
+template <typename T, int C, bool CHECK_FIX, int NB, bool Z0 = false>
 __device__ __forceinline__ void nb_step(const Geom& g, int vb, const KeC<T>& K, const GsBufs& B, T (&acc)[8][3]) {
-    if constexpr (NB != 13) {
+    if constexpr (NB != 13 && !(Z0 && nb_colour<C, NB % 3 - 1, NB / 3 % 3 - 1, NB / 9 - 1>() > C)) {
         const int so = rel_off<C, NB % 3 - 1, NB / 3 % 3 - 1, NB / 9 - 1>(g);
         const int vo = vb * (int)sizeof(T), sob = so * (int)sizeof(T);
         T u0 = bload<T>(B.u0, vo, sob);
@@ -146,12 +153,12 @@ __device__ __forceinline__ Two<double> bload2<double>(__amdgpu_buffer_rsrc_t r, 
 // The x-neighbours (dx = -1, +1) of a row (dy, dz) have the opposite x parity,
 // so both sit in one colour block at consecutive slots (offset of dx = -1, then
 // + 1): one two-wide load fetches both. The dx = 0 neighbour is a single load.
-template <typename T, int C, bool CHECK_FIX, int ROW>
+template <typename T, int C, bool CHECK_FIX, int ROW, bool Z0 = false>
 __device__ __forceinline__ void nb_row(const Geom& g, int vb, const KeC<T>& K, const GsBufs& B, T (&acc)[8][3]) {
     constexpr int DY = ROW % 3 - 1, DZ = ROW / 3 - 1;
     constexpr int NBM = 9 * (DZ + 1) + 3 * (DY + 1);   // dx = -1; dx = 0, +1 follow
     const int vo = vb * (int)sizeof(T);
-    {
+    if constexpr (!(Z0 && nb_colour<C, -1, DY, DZ>() > C)) {
         const int so = rel_off<C, -1, DY, DZ>(g), sob = so * (int)sizeof(T);
         const Two<T> a = bload2<T>(B.u0, vo, sob), b = bload2<T>(B.u1, vo, sob), c = bload2<T>(B.u2, vo, sob);
         T l0 = a.lo, l1 = b.lo, l2 = c.lo, r0 = a.hi, r1 = b.hi, r2 = c.hi;
@@ -162,13 +169,13 @@ __device__ __forceinline__ void nb_row(const Geom& g, int vb, const KeC<T>& K, c
         nb_elems<T, NBM>(acc, K, l0, l1, l2, std::make_integer_sequence<int, 8>{});
         nb_elems<T, NBM + 2>(acc, K, r0, r1, r2, std::make_integer_sequence<int, 8>{});
     }
-    nb_step<T, C, CHECK_FIX, NBM + 1>(g, vb, K, B, acc);   // skips the node itself
+    nb_step<T, C, CHECK_FIX, NBM + 1, Z0>(g, vb, K, B, acc);   // skips the node itself
 }
 
-template <typename T, int C, bool CHECK_FIX, int... ROWs>
+template <typename T, int C, bool CHECK_FIX, bool Z0, int... ROWs>
 __device__ __forceinline__ void nb_all(const Geom& g, int vb, const KeC<T>& K, const GsBufs& B, T (&acc)[8][3],
                                        std::integer_sequence<int, ROWs...>) {
-    (nb_row<T, C, CHECK_FIX, ROWs>(g, vb, K, B, acc), ...);
+    (nb_row<T, C, CHECK_FIX, ROWs, Z0>(g, vb, K, B, acc), ...);
 }
 
 template <typename T, int C, int W>
@@ -197,13 +204,13 @@ __device__ __forceinline__ void elem_all(const Geom& g, int vb, const KeC<T>& K,
     (elem_step<T, C, Ws>(g, vb, K, B, acc, KU, S), ...);
 }
 
-template <typename T, int C, bool CHECK_FIX>
+template <typename T, int C, bool CHECK_FIX, bool Z0 = false>
 __device__ __forceinline__ void gs_node(const Geom& g, int base, const KeC<T>& K, const GsBufs& B, T KU[3], T S[9]) {
     const int vb = base - g.h[0] * g.h[1] - g.h[0] - 1;
     T acc[8][3];
 #pragma unroll
     for (int w = 0; w < 8; w++) acc[w][0] = acc[w][1] = acc[w][2] = T(0);
-    nb_all<T, C, CHECK_FIX>(g, vb, K, B, acc, std::make_integer_sequence<int, 9>{});
+    nb_all<T, C, CHECK_FIX, Z0>(g, vb, K, B, acc, std::make_integer_sequence<int, 9>{});
     KU[0] = KU[1] = KU[2] = T(0);
 #pragma unroll
     for (int i = 0; i < 9; i++) S[i] = T(0);
@@ -224,13 +231,18 @@ template <> __device__ __forceinline__ float gs_div<float>(float a, float b) {
 // ZS ("zero supports"): the caller guarantees u = 0 at every support node for
 // the whole sweep (f = 0 there, so the relaxation keeps it 0, and the
 // prolongation skips supports), so a neighbour never needs its flag checked.
-template <typename T, int C, bool ZS>
+// Z0: the node's own old value is 0 as well, and an inactive node, which the
+// sweep otherwise leaves alone, gets the 0 that a cleared vector would hold
+template <typename T, int C, bool ZS, bool Z0 = false>
 __device__ __forceinline__ void gs_update(const Geom& g, int base, Vec3<T> U, Vec3<const T> F,
                                           const uint8_t* __restrict__ vflag, const KeC<T>& K, const GsBufs& B,
                                           bool reverse) {
     const int v = C * g.cs + base;
     const uint8_t fl = vflag[v];
-    if (!(fl & NF_ACTIVE)) return;
+    if (!(fl & NF_ACTIVE)) {
+        if constexpr (Z0) { U.c[0][v] = T(0); U.c[1][v] = T(0); U.c[2][v] = T(0); }
+        return;
+    }
     const T f[3] = {F.c[0][v], F.c[1][v], F.c[2][v]};
     if (fl & NF_FIXED) {
         // S = n_e I, KeU = 0 (Grid.cu:1007-1014)
@@ -239,9 +251,14 @@ __device__ __forceinline__ void gs_update(const Geom& g, int base, Vec3<T> U, Ve
         return;
     }
     T KU[3], S[9];
-    if (!ZS && (fl & NF_NEARFIX)) gs_node<T, C, true>(g, base, K, B, KU, S);
-    else gs_node<T, C, false>(g, base, K, B, KU, S);
-    T u[3] = {U.c[0][v], U.c[1][v], U.c[2][v]};
+    if (!ZS && (fl & NF_NEARFIX)) gs_node<T, C, true, Z0>(g, base, K, B, KU, S);
+    else gs_node<T, C, false, Z0>(g, base, K, B, KU, S);
+    T u[3];
+    if constexpr (Z0) {
+        u[0] = u[1] = u[2] = T(0);
+    } else {
+        u[0] = U.c[0][v]; u[1] = U.c[1][v]; u[2] = U.c[2][v];
+    }
     // component-wise Gauss-Seidel with the 3x3 self block (Grid.cu:1056-1062)
     if (!reverse) {
         u[0] = gs_div<T>(f[0] - S[1] * u[1] - S[2] * u[2] - KU[0], S[0]);
@@ -262,14 +279,14 @@ __device__ __forceinline__ void gs_update(const Geom& g, int base, Vec3<T> U, Ve
 // Every other neighbour belongs to a row of another colour pair and is
 // read-only during the launch. Four launches make a sweep, in the reference's
 // colour order 0..7 (or 7..0).
-template <typename T, int C, bool ZS>
+template <typename T, int C, bool ZS, bool Z0 = false>
 __device__ __forceinline__ void gs_row_phase(const Geom& g, int j, int k, Vec3<T> U, Vec3<const T> F,
                                              const uint8_t* __restrict__ vflag, const KeC<T>& K, const GsBufs& B,
                                              bool reverse) {
     const int nhx = (g.n[0] - (C & 1) + 1) >> 1;
     const int rowbase = ((k + 1) * g.h[1] + (j + 1)) * g.h[0] + 1;
     for (int i = threadIdx.x; i < nhx; i += blockDim.x)
-        gs_update<T, C, ZS>(g, rowbase + i, U, F, vflag, K, B, reverse);
+        gs_update<T, C, ZS, Z0>(g, rowbase + i, U, F, vflag, K, B, reverse);
 }
 
 // rows relaxed one after another by each workgroup of the smoother (rows of one
@@ -290,10 +307,11 @@ __device__ __forceinline__ void gs_row_phase(const Geom& g, int j, int k, Vec3<T
 #define RT_GS_ATTR
 #endif
 
-template <typename T, int CY, int CZ, bool REV, bool ZS>
+template <typename T, int CY, int CZ, bool REV, bool ZS, bool Z0 = false>
 __global__ void __launch_bounds__(256) RT_GS_ATTR k_fine_gs_pair(Geom g, Vec3<T> U, Vec3<const T> F,
                                                       const float* __restrict__ rhop,
                                                       const uint8_t* __restrict__ vflag, KeC<T> K, int nrows) {
+    static_assert(!(Z0 && REV), "a zero-start sweep runs forward");
     const int nhy = (g.n[1] - CY + 1) >> 1;
     const int first = xcd_row(blockIdx.x, gridDim.x) * RT_GS_ROWS;
     const int nbytes = g.N * (int)sizeof(T);
@@ -304,9 +322,9 @@ __global__ void __launch_bounds__(256) RT_GS_ATTR k_fine_gs_pair(Geom g, Vec3<T>
         const int row = first + r;   // uniform across the workgroup
         if (row >= nrows) break;
         const int j = row % nhy, k = row / nhy;
-        gs_row_phase<T, CA, ZS>(g, j, k, U, F, vflag, K, B, REV);
+        gs_row_phase<T, CA, ZS, Z0>(g, j, k, U, F, vflag, K, B, REV);
         __syncthreads();
-        gs_row_phase<T, CA ^ 1, ZS>(g, j, k, U, F, vflag, K, B, REV);
+        gs_row_phase<T, CA ^ 1, ZS, Z0>(g, j, k, U, F, vflag, K, B, REV);
     }
 }
 
@@ -532,15 +550,27 @@ template void launch_fine_gs_sweep2<double>(const Geom&, Vec3<const double>, Vec
                                             const float*, const uint8_t*, bool, bool, hipStream_t);
 
 // pair p = 0..3 relaxes colours (2p, 2p+1), or (2p+1, 2p) when reverse
+// zero_start: the first forward sweep from u = 0 (with zero supports); U may
+// hold anything on entry, and every slot the sweep visits is written
 template <typename T>
 void launch_fine_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* rhop, const uint8_t* vflag, int pair,
-                    bool reverse, bool zero_support, hipStream_t s) {
+                    bool reverse, bool zero_support, hipStream_t s, bool zero_start) {
     check_buffer_range<T>(g);
     const int cy = pair & 1, cz = pair >> 1;
     const long long rows = (long long)((g.n[1] - cy + 1) >> 1) * ((g.n[2] - cz + 1) >> 1);
     if (rows <= 0) return;
     dim3 grid((unsigned)((rows + RT_GS_ROWS - 1) / RT_GS_ROWS));
     const KeC<T> K = kec<T>();
+    if (zero_start) {
+        if (reverse || !zero_support) throw std::logic_error("a zero-start sweep is forward with zero supports");
+        switch (pair) {
+#define RT_CASE(p) case p: hipLaunchKernelGGL((k_fine_gs_pair<T, ((p) & 1), ((p) >> 1), false, true, true>), grid, dim3(256), 0, s, g, U, F, rhop, vflag, K, (int)rows); break;
+            RT_CASE(0) RT_CASE(1) RT_CASE(2) RT_CASE(3)
+#undef RT_CASE
+        }
+        RT_HIP(hipGetLastError());
+        return;
+    }
     switch (pair * 4 + (reverse ? 2 : 0) + (zero_support ? 1 : 0)) {
 #define RT_CASE(p, r, z) case p * 4 + r * 2 + z: hipLaunchKernelGGL((k_fine_gs_pair<T, ((p) & 1), ((p) >> 1), (r) != 0, (z) != 0>), grid, dim3(256), 0, s, g, U, F, rhop, vflag, K, (int)rows); break;
         RT_CASE(0, 0, 0) RT_CASE(0, 1, 0) RT_CASE(1, 0, 0) RT_CASE(1, 1, 0) RT_CASE(2, 0, 0) RT_CASE(2, 1, 0)
@@ -552,8 +582,8 @@ void launch_fine_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* rhop
 }
 
 template void launch_fine_gs<float>(const Geom&, Vec3<float>, Vec3<const float>, const float*, const uint8_t*, int,
-                                    bool, bool, hipStream_t);
+                                    bool, bool, hipStream_t, bool);
 template void launch_fine_gs<double>(const Geom&, Vec3<double>, Vec3<const double>, const float*, const uint8_t*, int,
-                                     bool, bool, hipStream_t);
+                                     bool, bool, hipStream_t, bool);
 
 }  // namespace rt

@@ -480,9 +480,16 @@ template <> V3<float>& gs_tmp<float>(Hier& H) { if (!H.gs_tmp32.ok()) H.gs_tmp32
 // ---------------------------------------------------------------------------
 template <typename T>
 // zero_support: u and f are 0 at every support node of the finest level (the
-// preconditioner's case), so the smoother skips neighbour support checks
-static void gs(Hier& H, int li, Vec3<T> U, Vec3<const T> F, int n, bool reverse, bool zero_support = false) {
+// preconditioner's case), so the smoother skips neighbour support checks.
+// zero_start: U holds no valid values and the smoothing starts from u = 0; the
+// colour-pair fine smoother's first forward sweep takes that start itself
+// (reading no neighbour that is still 0), otherwise U is cleared first.
+static void gs(Hier& H, int li, Vec3<T> U, Vec3<const T> F, int n, bool reverse, bool zero_support = false,
+               bool zero_start = false) {
     Level& L = H.lv[li];
+    const bool z0 = zero_start && li == 0 && H.fine_smoother == 0 && n > 0 && !reverse && zero_support;
+    if (zero_start && !z0)
+        for (int c = 0; c < 3; c++) launch_fill<T>(U.c[c], T(0), (size_t)L.g.N, H.s);
     for (int it = 0; it < n; it++) {
         if (li == 0 && H.fine_smoother == 1) {
             // two launches per sweep, ping-ponging between U and the second buffer
@@ -497,7 +504,7 @@ static void gs(Hier& H, int li, Vec3<T> U, Vec3<const T> F, int n, bool reverse,
             // four launches, each relaxing the colour pair (2p, 2p+1) row by row
             for (int pp = 0; pp < 4; pp++)
                 launch_fine_gs<T>(L.g, U, F, H.rhop.as<float>(), L.vflag.as<uint8_t>(), reverse ? 3 - pp : pp,
-                                  reverse, zero_support, H.s);
+                                  reverse, zero_support, H.s, z0 && it == 0);
         } else if (coarse_gs_pairs(L.g)) {
             // four launches, each relaxing the colour pair (2p, 2p+1) row by row
             for (int pp = 0; pp < 4; pp++) {
@@ -583,8 +590,11 @@ template <typename T>
 // the coarse-grid correction of the finest level, from the same residual R0
 // (an additive pair of coarse spaces inside the multiplicative cycle, so the
 // cycle stays symmetric when its smoothing is)
+// u0_unset: U0 holds no valid values and the cycle starts from U0 = 0 (the
+// first smoothing pass clears it or starts from zero itself)
 static void vcycle_core(Hier& H, V3<T>& U0, V3<T>& F0, V3<T>& R0, int pre, int post, bool symmetric,
-                        int coarse_mult = 1, int gamma = 1, bool zero_support = false, bool islands = false) {
+                        int coarse_mult = 1, int gamma = 1, bool zero_support = false, bool islands = false,
+                        bool u0_unset = false) {
     // the preconditioner (zero_support) may stop a level early and solve it
     // exactly (pc_levels); the reference-semantics cycle visits every level
     const int n = zero_support && H.pc_levels > 0 ? H.pc_levels : (int)H.lv.size();
@@ -607,7 +617,7 @@ static void vcycle_core(Hier& H, V3<T>& U0, V3<T>& F0, V3<T>& R0, int pre, int p
         }
         const int m = i == 0 ? 1 : zero_support ? level_mult(H, i) : coarse_mult;
         const bool zs = i == 0 && zero_support;
-        gs<T>(H, i, Uv(i), Fv(i), pre * m, false, zs);
+        gs<T>(H, i, Uv(i), Fv(i), pre * m, false, zs, i == 0 && u0_unset);
         residual<T>(H, i, cvt(Uv(i)), Fv(i), Rv(i));
         LevelVecs<T>& lvv = vecs<T>(H.lv[i + 1]);
         restrict_to<T>(H, i + 1, cvt(Rv(i)), lvv.F.v());
@@ -624,28 +634,26 @@ static void vcycle_core(Hier& H, V3<T>& U0, V3<T>& F0, V3<T>& R0, int pre, int p
     cyc(cyc, 0);
 }
 
-// z = M^-1 r for the PCG: pc.U = 0, then one preconditioner cycle on pc.F.
-// After an eager first application (which also performs lazy allocations) the
-// cycle is captured once into a HIP graph and replayed; the graph holds no
-// memset nodes (pc.U is cleared by a fill kernel) and is dropped whenever
-// stencils, material, supports or the cycle shape change.
+// z = M^-1 r for the PCG: one preconditioner cycle on pc.F from pc.U = 0 (the
+// first fine sweep starts from zero without reading pc.U, or pc.U is cleared
+// by a fill kernel). After an eager first application (which also performs
+// lazy allocations) the cycle is captured once into a HIP graph and replayed;
+// the graph holds no memset nodes and is dropped whenever stencils, material,
+// supports or the cycle shape change.
 template <typename TC>
 static void precondition(Hier& H, LevelVecs<TC>& pc, int pre, int post) {
-    const size_t n = (size_t)3 * pc.U.N;
     const long long key = ((((long long)sizeof(TC) * 64 + pre) * 64 + post) * 64 + H.pc_mult) * 4 + H.pc_gamma;
     if (H.use_graphs && H.pc_graph && H.pc_key == key) {
         RT_HIP(hipGraphLaunch(H.pc_graph, H.s));
         return;
     }
-    launch_fill<TC>(pc.U.data(), TC(0), n, H.s);
-    vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma, true, true);
+    vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma, true, true, true);
     if (!H.use_graphs) return;
     H.drop_graph();
     RT_HIP(hipStreamBeginCapture(H.s, hipStreamCaptureModeThreadLocal));
     hipGraph_t g = nullptr;
     try {
-        launch_fill<TC>(pc.U.data(), TC(0), n, H.s);
-        vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma, true, true);
+        vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma, true, true, true);
     } catch (...) {
         (void)hipStreamEndCapture(H.s, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -2365,10 +2373,11 @@ static void ensure_batch(Hier& H, int nb) {
 static void vcycle_batch(Hier& H, int nb, int pre, int post) {
     const int n = H.pc_levels > 0 ? H.pc_levels : (int)H.lv.size();
     const bool early = n < (int)H.lv.size();
-    auto smooth = [&](int i, int nsw, bool rev) {
+    // zero_start: the fine vectors hold no valid values yet (first pre-smoothing)
+    auto smooth = [&](int i, int nsw, bool rev, bool zero_start) {
         Level& L = H.lv[i];
         if (i == 0) {
-            for (int b = 0; b < nb; b++) gs<float>(H, 0, L.bU.v(b), L.bF.cv(b), nsw, rev, true);
+            for (int b = 0; b < nb; b++) gs<float>(H, 0, L.bU.v(b), L.bF.cv(b), nsw, rev, true, zero_start);
             return;
         }
         for (int it = 0; it < nsw; it++)
@@ -2385,7 +2394,7 @@ static void vcycle_batch(Hier& H, int nb, int pre, int post) {
             return;
         }
         const int m = i == 0 ? 1 : level_mult(H, i);
-        smooth(i, pre * m, false);
+        smooth(i, pre * m, false, i == 0);
         if (i == 0)
             for (int b = 0; b < nb; b++) residual<float>(H, 0, L.bU.cv(b), L.bF.cv(b), L.bR.v(b));
         else
@@ -2401,22 +2410,20 @@ static void vcycle_batch(Hier& H, int nb, int pre, int post) {
                 island_correct<float>(L.g, H.nisl, H.isl_node.as<int>(), H.isl_cent.as<double>(),
                                       H.isl_Einv.as<double>(), H.isl_zr.as<double>(), H.isl_coef.as<double>(),
                                       L.bR.cv(b), L.bU.v(b), H.s);
-        smooth(i, post * m, true);
+        smooth(i, post * m, true, false);
     };
     cyc(cyc, 0);
 }
 
-// L0.bU = M^-1 L0.bF for nb vectors; replayed from a HIP graph after the first
-// (eager) application of a shape
+// L0.bU = M^-1 L0.bF for nb vectors (from L0.bU = 0: the first fine sweep starts
+// from zero itself); replayed from a HIP graph after the first (eager)
+// application of a shape
 static void precondition_batch(Hier& H, int nb, int pre, int post) {
-    Level& L0 = H.lv[0];
     const long long key = (((long long)nb * 64 + pre) * 64 + post) * 64 + H.pc_mult;
     if (H.use_graphs && H.bgraph && H.bkey == key) {
         RT_HIP(hipGraphLaunch(H.bgraph, H.s));
         return;
     }
-    const size_t n = (size_t)3 * L0.g.N * nb;
-    launch_fill<float>(L0.bU.data(), 0.f, n, H.s);
     vcycle_batch(H, nb, pre, post);
     if (!H.use_graphs) return;
     if (H.bgraph) (void)hipGraphExecDestroy(H.bgraph);
@@ -2425,7 +2432,6 @@ static void precondition_batch(Hier& H, int nb, int pre, int post) {
     RT_HIP(hipStreamBeginCapture(H.s, hipStreamCaptureModeThreadLocal));
     hipGraph_t g = nullptr;
     try {
-        launch_fill<float>(L0.bU.data(), 0.f, n, H.s);
         vcycle_batch(H, nb, pre, post);
     } catch (...) {
         (void)hipStreamEndCapture(H.s, &g);

@@ -112,9 +112,11 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, int bytes)
 // ---------------------------------------------------------------------------
 // k_fine.hip
 void set_ke_constants(const double* ke576, hipStream_t s);
+// zero_start: the first forward sweep of a cycle from u = 0 with zero supports;
+// U's old contents are ignored, and every node slot of the pair is written
 template <typename T>
 void launch_fine_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* rhop, const uint8_t* vflag, int pair,
-                    bool reverse, bool zero_support, hipStream_t s);
+                    bool reverse, bool zero_support, hipStream_t s, bool zero_start = false);
 // one full fine sweep in two launches, old values from Uin, new ones to Uout
 // (Uin is not written; the two must not overlap)
 template <typename T>

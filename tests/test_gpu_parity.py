@@ -123,7 +123,10 @@ def test_fine_gs_tiled_matches_colour_pairs(pair, bits, reverse):
 def test_pcg_tiled_smoother_matches_colour_pairs(pair, mode):
     """Through the preconditioner (support-free smoothing, forward then reverse
     sweeps): five PCG iterations give bit-identical iterates with either fine
-    smoother."""
+    smoother. The colour-pair smoother's first sweep of each cycle starts from
+    zero by itself (it neither clears nor reads the correction, and skips the
+    neighbours that are still zero); the tiled one clears the correction and
+    reads every neighbour, so this also pins the zero start."""
     g = pair.g
     g.update_stencil()
     outs = []
