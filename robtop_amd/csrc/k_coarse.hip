@@ -956,14 +956,24 @@ void launch_dot_partials(const double* a, const double* b, size_t n, double* par
     launch_dot_partials_t<double, double>(a, b, n, partials, nparts, s);
 }
 
-__global__ void __launch_bounds__(256) k_sum_partials(const double* __restrict__ p, int n, double* __restrict__ out) {
-    double s = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+// one workgroup of 1024 threads; each thread keeps four running sums over
+// interleaved runs of the partials (independent loads in flight, fixed order)
+__global__ void __launch_bounds__(1024) k_sum_partials(const double* __restrict__ p, int n, double* __restrict__ out) {
+    double a[4] = {0, 0, 0, 0};
+    const int step = 4 * (int)blockDim.x;
+    for (int i = threadIdx.x; i < n; i += step) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int j = i + k * (int)blockDim.x;
+            if (j < n) a[k] += p[j];
+        }
+    }
+    double s = (a[0] + a[1]) + (a[2] + a[3]);
     s = blk_sum(s);
     if (threadIdx.x == 0) out[0] = s;
 }
 void launch_sum_partials(const double* partials, int nparts, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partials, nparts, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, partials, nparts, out);
     RT_HIP(hipGetLastError());
 }
 

@@ -232,11 +232,13 @@ template <> __device__ __forceinline__ float gs_div<float>(float a, float b) {
 // the whole sweep (f = 0 there, so the relaxation keeps it 0, and the
 // prolongation skips supports), so a neighbour never needs its flag checked.
 // Z0: the node's own old value is 0 as well, and an inactive node, which the
-// sweep otherwise leaves alone, gets the 0 that a cleared vector would hold
+// sweep otherwise leaves alone, gets the 0 that a cleared vector would hold.
+// fu: f . u_new of the node is added to it (the last sweep of a cycle, whose
+// values are final: the preconditioned residual's r . z without another pass).
 template <typename T, int C, bool ZS, bool Z0 = false>
 __device__ __forceinline__ void gs_update(const Geom& g, int base, Vec3<T> U, Vec3<const T> F,
                                           const uint8_t* __restrict__ vflag, const KeC<T>& K, const GsBufs& B,
-                                          bool reverse) {
+                                          bool reverse, double* fu = nullptr) {
     const int v = C * g.cs + base;
     const uint8_t fl = vflag[v];
     if (!(fl & NF_ACTIVE)) {
@@ -247,7 +249,9 @@ __device__ __forceinline__ void gs_update(const Geom& g, int base, Vec3<T> U, Ve
     if (fl & NF_FIXED) {
         // S = n_e I, KeU = 0 (Grid.cu:1007-1014)
         const T ne = (T)(fl >> 4);
-        U.c[0][v] = f[0] / ne; U.c[1][v] = f[1] / ne; U.c[2][v] = f[2] / ne;
+        const T u0 = f[0] / ne, u1 = f[1] / ne, u2 = f[2] / ne;
+        U.c[0][v] = u0; U.c[1][v] = u1; U.c[2][v] = u2;
+        if (fu) *fu += ((double)f[0] * (double)u0 + (double)f[1] * (double)u1) + (double)f[2] * (double)u2;
         return;
     }
     T KU[3], S[9];
@@ -270,6 +274,7 @@ __device__ __forceinline__ void gs_update(const Geom& g, int base, Vec3<T> U, Ve
         u[0] = gs_div<T>(f[0] - S[1] * u[1] - S[2] * u[2] - KU[0], S[0]);
     }
     U.c[0][v] = u[0]; U.c[1][v] = u[1]; U.c[2][v] = u[2];
+    if (fu) *fu += ((double)f[0] * (double)u[0] + (double)f[1] * (double)u[1]) + (double)f[2] * (double)u[2];
 }
 
 // Colours C and C^1 (same y, z parity, opposite x parity) couple only through
@@ -282,11 +287,25 @@ __device__ __forceinline__ void gs_update(const Geom& g, int base, Vec3<T> U, Ve
 template <typename T, int C, bool ZS, bool Z0 = false>
 __device__ __forceinline__ void gs_row_phase(const Geom& g, int j, int k, Vec3<T> U, Vec3<const T> F,
                                              const uint8_t* __restrict__ vflag, const KeC<T>& K, const GsBufs& B,
-                                             bool reverse) {
+                                             bool reverse, double* fu = nullptr) {
     const int nhx = (g.n[0] - (C & 1) + 1) >> 1;
     const int rowbase = ((k + 1) * g.h[1] + (j + 1)) * g.h[0] + 1;
     for (int i = threadIdx.x; i < nhx; i += blockDim.x)
-        gs_update<T, C, ZS, Z0>(g, rowbase + i, U, F, vflag, K, B, reverse);
+        gs_update<T, C, ZS, Z0>(g, rowbase + i, U, F, vflag, K, B, reverse, fu);
+}
+
+// sum of a double over the workgroup, valid in thread 0
+__device__ __forceinline__ double gs_blk_sum(double v) {
+    __shared__ double red[16];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)((blockDim.x + 63) >> 6); i++) t += red[i];
+    return t;
 }
 
 // rows relaxed one after another by each workgroup of the smoother (rows of one
@@ -307,11 +326,16 @@ __device__ __forceinline__ void gs_row_phase(const Geom& g, int j, int k, Vec3<T
 #define RT_GS_ATTR
 #endif
 
-template <typename T, int CY, int CZ, bool REV, bool ZS, bool Z0 = false>
+// DOT: partial sums of f . u_new over each workgroup's rows, one per workgroup
+// in fu_part[blockIdx.x]
+template <typename T, int CY, int CZ, bool REV, bool ZS, bool Z0 = false, bool DOT = false>
 __global__ void __launch_bounds__(256) RT_GS_ATTR k_fine_gs_pair(Geom g, Vec3<T> U, Vec3<const T> F,
                                                       const float* __restrict__ rhop,
-                                                      const uint8_t* __restrict__ vflag, KeC<T> K, int nrows) {
+                                                      const uint8_t* __restrict__ vflag, KeC<T> K, int nrows,
+                                                      double* __restrict__ fu_part = nullptr) {
     static_assert(!(Z0 && REV), "a zero-start sweep runs forward");
+    double fu = 0;
+    double* fup = DOT ? &fu : nullptr;
     const int nhy = (g.n[1] - CY + 1) >> 1;
     const int first = xcd_row(blockIdx.x, gridDim.x) * RT_GS_ROWS;
     const int nbytes = g.N * (int)sizeof(T);
@@ -322,9 +346,13 @@ __global__ void __launch_bounds__(256) RT_GS_ATTR k_fine_gs_pair(Geom g, Vec3<T>
         const int row = first + r;   // uniform across the workgroup
         if (row >= nrows) break;
         const int j = row % nhy, k = row / nhy;
-        gs_row_phase<T, CA, ZS, Z0>(g, j, k, U, F, vflag, K, B, REV);
+        gs_row_phase<T, CA, ZS, Z0>(g, j, k, U, F, vflag, K, B, REV, fup);
         __syncthreads();
-        gs_row_phase<T, CA ^ 1, ZS, Z0>(g, j, k, U, F, vflag, K, B, REV);
+        gs_row_phase<T, CA ^ 1, ZS, Z0>(g, j, k, U, F, vflag, K, B, REV, fup);
+    }
+    if constexpr (DOT) {
+        const double t = gs_blk_sum(fu);
+        if (threadIdx.x == 0) fu_part[blockIdx.x] = t;
     }
 }
 
@@ -552,15 +580,33 @@ template void launch_fine_gs_sweep2<double>(const Geom&, Vec3<const double>, Vec
 // pair p = 0..3 relaxes colours (2p, 2p+1), or (2p+1, 2p) when reverse
 // zero_start: the first forward sweep from u = 0 (with zero supports); U may
 // hold anything on entry, and every slot the sweep visits is written
+// workgroups of one colour-pair launch (the partials a fu launch writes)
+int fine_gs_pair_blocks(const Geom& g, int pair) {
+    const int cy = pair & 1, cz = pair >> 1;
+    const long long rows = (long long)((g.n[1] - cy + 1) >> 1) * ((g.n[2] - cz + 1) >> 1);
+    return rows <= 0 ? 0 : (int)((rows + RT_GS_ROWS - 1) / RT_GS_ROWS);
+}
+
 template <typename T>
 void launch_fine_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* rhop, const uint8_t* vflag, int pair,
-                    bool reverse, bool zero_support, hipStream_t s, bool zero_start) {
+                    bool reverse, bool zero_support, hipStream_t s, bool zero_start, double* fu_part) {
     check_buffer_range<T>(g);
     const int cy = pair & 1, cz = pair >> 1;
     const long long rows = (long long)((g.n[1] - cy + 1) >> 1) * ((g.n[2] - cz + 1) >> 1);
     if (rows <= 0) return;
     dim3 grid((unsigned)((rows + RT_GS_ROWS - 1) / RT_GS_ROWS));
     const KeC<T> K = kec<T>();
+    if (fu_part) {
+        if (!reverse || !zero_support || zero_start)
+            throw std::logic_error("f.u partials come from a reverse sweep with zero supports");
+        switch (pair) {
+#define RT_CASE(p) case p: hipLaunchKernelGGL((k_fine_gs_pair<T, ((p) & 1), ((p) >> 1), true, true, false, true>), grid, dim3(256), 0, s, g, U, F, rhop, vflag, K, (int)rows, fu_part); break;
+            RT_CASE(0) RT_CASE(1) RT_CASE(2) RT_CASE(3)
+#undef RT_CASE
+        }
+        RT_HIP(hipGetLastError());
+        return;
+    }
     if (zero_start) {
         if (reverse || !zero_support) throw std::logic_error("a zero-start sweep is forward with zero supports");
         switch (pair) {
@@ -582,8 +628,8 @@ void launch_fine_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* rhop
 }
 
 template void launch_fine_gs<float>(const Geom&, Vec3<float>, Vec3<const float>, const float*, const uint8_t*, int,
-                                    bool, bool, hipStream_t, bool);
+                                    bool, bool, hipStream_t, bool, double*);
 template void launch_fine_gs<double>(const Geom&, Vec3<double>, Vec3<const double>, const float*, const uint8_t*, int,
-                                     bool, bool, hipStream_t, bool);
+                                     bool, bool, hipStream_t, bool, double*);
 
 }  // namespace rt

@@ -120,6 +120,10 @@ struct Hier {
     int vprec = 64;
     int pc_mult = 1, pc_gamma = 1;   // preconditioner cycle: coarse-level sweep multiplier, cycle index
     std::vector<int> pc_lv_mult;     // per-level overrides of pc_mult (0: pc_mult)
+    // f . u partials of the preconditioner's last fine sweep (its r . z), one
+    // run per colour pair at rzpart_off[pair]
+    DBuf rzpart;
+    int rzpart_n = 0, rzpart_off[4] = {0, 0, 0, 0};
     // The preconditioner cycle may stop one level early and solve that level
     // exactly when it has at most pc_direct_dofs unknowns (pc_levels = the
     // number of levels its cycle visits); its dense inverse lives here.
@@ -484,8 +488,10 @@ template <typename T>
 // zero_start: U holds no valid values and the smoothing starts from u = 0; the
 // colour-pair fine smoother's first forward sweep takes that start itself
 // (reading no neighbour that is still 0), otherwise U is cleared first.
+// fu_part: the last sweep also leaves the partial sums of f . u_new (the
+// colour-pair fine smoother only; see rz_fused)
 static void gs(Hier& H, int li, Vec3<T> U, Vec3<const T> F, int n, bool reverse, bool zero_support = false,
-               bool zero_start = false) {
+               bool zero_start = false, double* fu_part = nullptr) {
     Level& L = H.lv[li];
     const bool z0 = zero_start && li == 0 && H.fine_smoother == 0 && n > 0 && !reverse && zero_support;
     if (zero_start && !z0)
@@ -502,9 +508,11 @@ static void gs(Hier& H, int li, Vec3<T> U, Vec3<const T> F, int n, bool reverse,
                 for (int c = 0; c < 3; c++) launch_convert<T, T>(Tv.c[c], U.c[c], (size_t)L.g.N, H.s);
         } else if (li == 0) {
             // four launches, each relaxing the colour pair (2p, 2p+1) row by row
-            for (int pp = 0; pp < 4; pp++)
-                launch_fine_gs<T>(L.g, U, F, H.rhop.as<float>(), L.vflag.as<uint8_t>(), reverse ? 3 - pp : pp,
-                                  reverse, zero_support, H.s, z0 && it == 0);
+            for (int pp = 0; pp < 4; pp++) {
+                const int p = reverse ? 3 - pp : pp;
+                launch_fine_gs<T>(L.g, U, F, H.rhop.as<float>(), L.vflag.as<uint8_t>(), p, reverse, zero_support,
+                                  H.s, z0 && it == 0, fu_part && it == n - 1 ? fu_part + H.rzpart_off[p] : nullptr);
+            }
         } else if (coarse_gs_pairs(L.g)) {
             // four launches, each relaxing the colour pair (2p, 2p+1) row by row
             for (int pp = 0; pp < 4; pp++) {
@@ -576,6 +584,23 @@ static void coarsest(Hier& H, Vec3<const T> F, Vec3<T> U) {
     launch_coarsest<T>(H.Ainv.as<double>(), H.n3, H.rowslot.as<int>(), 0, F, U, H.fbuf.as<double>(), H.s);
 }
 
+// whether the preconditioner's last fine sweep yields r . z itself (the
+// colour-pair smoother and at least one post-smoothing sweep)
+static bool rz_fused(const Hier& H, int post) { return H.fine_smoother == 0 && post > 0; }
+
+static double* ensure_rzpart(Hier& H) {
+    if (!H.rzpart.p) {
+        int off = 0;
+        for (int p = 0; p < 4; p++) {
+            H.rzpart_off[p] = off;
+            off += fine_gs_pair_blocks(H.lv[0].g, p);
+        }
+        H.rzpart_n = off;
+        H.rzpart.alloc(sizeof(double) * (size_t)std::max(off, 1));
+    }
+    return H.rzpart.as<double>();
+}
+
 // sweep multiplier of coarse level i in the preconditioner's cycle
 static int level_mult(const Hier& H, int i) {
     return i < (int)H.pc_lv_mult.size() && H.pc_lv_mult[i] > 0 ? H.pc_lv_mult[i] : H.pc_mult;
@@ -594,7 +619,7 @@ template <typename T>
 // first smoothing pass clears it or starts from zero itself)
 static void vcycle_core(Hier& H, V3<T>& U0, V3<T>& F0, V3<T>& R0, int pre, int post, bool symmetric,
                         int coarse_mult = 1, int gamma = 1, bool zero_support = false, bool islands = false,
-                        bool u0_unset = false) {
+                        bool u0_unset = false, double* fu_part = nullptr) {
     // the preconditioner (zero_support) may stop a level early and solve it
     // exactly (pc_levels); the reference-semantics cycle visits every level
     const int n = zero_support && H.pc_levels > 0 ? H.pc_levels : (int)H.lv.size();
@@ -629,7 +654,7 @@ static void vcycle_core(Hier& H, V3<T>& U0, V3<T>& F0, V3<T>& R0, int pre, int p
             island_correct<T>(H.lv[0].g, H.nisl, H.isl_node.as<int>(), H.isl_cent.as<double>(),
                               H.isl_Einv.as<double>(), H.isl_zr.as<double>(), H.isl_coef.as<double>(),
                               cvt(Rv(0)), Uv(0), H.s);
-        gs<T>(H, i, Uv(i), Fv(i), post * m, symmetric, zs);
+        gs<T>(H, i, Uv(i), Fv(i), post * m, symmetric, zs, false, i == 0 ? fu_part : nullptr);
     };
     cyc(cyc, 0);
 }
@@ -647,13 +672,14 @@ static void precondition(Hier& H, LevelVecs<TC>& pc, int pre, int post) {
         RT_HIP(hipGraphLaunch(H.pc_graph, H.s));
         return;
     }
-    vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma, true, true, true);
+    double* fu = std::is_same<TC, float>::value && rz_fused(H, post) ? ensure_rzpart(H) : nullptr;
+    vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma, true, true, true, fu);
     if (!H.use_graphs) return;
     H.drop_graph();
     RT_HIP(hipStreamBeginCapture(H.s, hipStreamCaptureModeThreadLocal));
     hipGraph_t g = nullptr;
     try {
-        vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma, true, true, true);
+        vcycle_core<TC>(H, pc.U, pc.F, pc.R, pre, post, true, H.pc_mult, H.pc_gamma, true, true, true, fu);
     } catch (...) {
         (void)hipStreamEndCapture(H.s, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -688,6 +714,17 @@ static double sum_partials_at(Hier& H, const double* p, int np) {
 }
 static double sum_partials_host(Hier& H, int np) {
     launch_sum_partials(H.partials.as<double>(), np, H.scal.as<double>(), H.s);
+    double v;
+    RT_HIP(hipMemcpyAsync(&v, H.scal.p, sizeof(double), hipMemcpyDeviceToHost, H.s));
+    RT_HIP(hipStreamSynchronize(H.s));
+    return v;
+}
+
+// r . z after precondition<float>(H, pc, pre, post): from the last fine
+// sweep's partials when it wrote them, else a separate pass over both vectors
+static double precond_rz(Hier& H, LevelVecs<float>& pc, int post, size_t n) {
+    if (!rz_fused(H, post)) return dot<float, float>(H, pc.F.data(), pc.U.data(), n);
+    launch_sum_partials(H.rzpart.as<double>(), H.rzpart_n, H.scal.as<double>(), H.s);
     double v;
     RT_HIP(hipMemcpyAsync(&v, H.scal.p, sizeof(double), hipMemcpyDeviceToHost, H.s));
     RT_HIP(hipStreamSynchronize(H.s));
@@ -1178,7 +1215,7 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
         H.cg_beta.clear();
         launch_fill<float>(H.dx32.data(), 0.f, n, H.s);
         precondition<float>(H, pc, pre, post);
-        double rz = dot<float, float>(H, pc.F.data(), pc.U.data(), n);
+        double rz = precond_rz(H, pc, post, n);
         launch_convert<float, float>(pc.U.data(), H.p32.data(), n, H.s);
         bool flushed = false;
         for (it = 1; it <= maxit; it++) {
@@ -1204,7 +1241,7 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
             }
             if (it == maxit) break;
             precondition<float>(H, pc, pre, post);
-            const double rz_new = dot<float, float>(H, pc.F.data(), pc.U.data(), n);
+            const double rz_new = precond_rz(H, pc, post, n);
             const double beta = rz_new / rz;
             H.cg_beta.push_back(beta);
             rz = rz_new;

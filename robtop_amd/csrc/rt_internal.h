@@ -113,10 +113,14 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, int bytes)
 // k_fine.hip
 void set_ke_constants(const double* ke576, hipStream_t s);
 // zero_start: the first forward sweep of a cycle from u = 0 with zero supports;
-// U's old contents are ignored, and every node slot of the pair is written
+// U's old contents are ignored, and every node slot of the pair is written.
+// fu_part (a reverse sweep with zero supports): each workgroup also writes the
+// sum of f . u_new over the nodes it relaxes, fine_gs_pair_blocks(g, pair) of them.
 template <typename T>
 void launch_fine_gs(const Geom& g, Vec3<T> U, Vec3<const T> F, const float* rhop, const uint8_t* vflag, int pair,
-                    bool reverse, bool zero_support, hipStream_t s, bool zero_start = false);
+                    bool reverse, bool zero_support, hipStream_t s, bool zero_start = false,
+                    double* fu_part = nullptr);
+int fine_gs_pair_blocks(const Geom& g, int pair);
 // one full fine sweep in two launches, old values from Uin, new ones to Uout
 // (Uin is not written; the two must not overlap)
 template <typename T>

@@ -122,11 +122,14 @@ def test_fine_gs_tiled_matches_colour_pairs(pair, bits, reverse):
 @pytest.mark.parametrize("mode", [rt.RT_SOLVE_FP64, rt.RT_SOLVE_RELIABLE])
 def test_pcg_tiled_smoother_matches_colour_pairs(pair, mode):
     """Through the preconditioner (support-free smoothing, forward then reverse
-    sweeps): five PCG iterations give bit-identical iterates with either fine
-    smoother. The colour-pair smoother's first sweep of each cycle starts from
-    zero by itself (it neither clears nor reads the correction, and skips the
-    neighbours that are still zero); the tiled one clears the correction and
-    reads every neighbour, so this also pins the zero start."""
+    sweeps): five PCG iterations with either fine smoother. The colour-pair
+    smoother's first sweep of each cycle starts from zero by itself (it neither
+    clears nor reads the correction, and skips the neighbours that are still
+    zero); the tiled one clears the correction and reads every neighbour.
+    With the FP64 cycle the iterates are bit-identical, which pins the zero
+    start. With the FP32 cycle (reliable mode) the colour-pair smoother's last
+    sweep also sums r . z as it relaxes, in another order than the separate
+    dot the tiled path takes, so the iterates agree to rounding."""
     g = pair.g
     g.update_stencil()
     outs = []
@@ -136,7 +139,10 @@ def test_pcg_tiled_smoother_matches_colour_pairs(pair, mode):
         g.solve(tol=0.0, maxit=5, mode=mode, pre=2, post=2)
         outs.append(g.get_displacement())
     g.set_fine_smoother(0)
-    assert np.array_equal(outs[0], outs[1])
+    if mode == rt.RT_SOLVE_FP64:
+        assert np.array_equal(outs[0], outs[1])
+    else:
+        assert rel_err(outs[0], outs[1]) < 1e-5
 
 
 def test_apply_k(pair):
