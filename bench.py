@@ -24,8 +24,8 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--size", default="511,256,256")
     ap.add_argument("--workload", default="cantilever", choices=["cantilever", "torus"],
                     help="cantilever: the headline s/SIMP-iteration run; torus: the worst-case loop on a "
