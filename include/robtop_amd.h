@@ -371,6 +371,11 @@ int rt_net_gradient(rt_net* n, const int* dims, const float* x, const float* tar
 int rt_net_add_sample(rt_net* n, rt_hier* h);
 /* loss of the prediction for the hierarchy's current triple (not stored) */
 int rt_net_evaluate(rt_net* n, rt_hier* h, double* loss);
+/* The network as a coarse-grid correction: the first coarse (x4) level's U
+ * becomes the prediction from the density and that level's current F (a
+ * restricted residual, say), shape only (unit scale up to the network's
+ * error). Works on the FP64 level vectors (rt_set_vcycle_precision(h, 64)). */
+int rt_net_predict_coarse(rt_net* n, rt_hier* h);
 int rt_net_num_samples(const rt_net* n, int* count);
 int rt_net_clear_samples(rt_net* n);
 /* steps Adam updates (beta 0.9 / 0.999), each on the mean loss of `batch`

@@ -126,6 +126,7 @@ SIGNATURES = {
     "rt_net_gradient": ([P, P, P, P, P, P], I),
     "rt_net_add_sample": ([P, P], I),
     "rt_net_evaluate": ([P, P, P], I),
+    "rt_net_predict_coarse": ([P, P], I),
     "rt_net_time_forward": ([P, P, I, P], I),
     "rt_net_num_samples": ([P, P], I),
     "rt_net_clear_samples": ([P], I),

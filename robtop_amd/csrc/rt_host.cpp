@@ -1854,6 +1854,22 @@ static double net_sample(rt_net* net, rt_hier* h, bool store) {
 
 int rt_net_add_sample(rt_net* net, rt_hier* h) { RT_TRY({ net_sample(net, h, true); }) }
 
+int rt_net_predict_coarse(rt_net* net, rt_hier* h) {
+    RT_TRY({
+        Hier& H = H_(h);
+        if (!net || !net->n) throw std::invalid_argument("null rt_net handle");
+        need_x4_level(H);
+        if (H.vprec != 64)
+            throw std::logic_error("rt_net_predict_coarse works on the FP64 level vectors (rt_set_vcycle_precision 64)");
+        Level& L0 = H.lv[0];
+        Level& C = H.lv[1];
+        LevelVecs<double>& cv = vecs<double>(C);
+        launch_fill<double>(cv.U.data(), 0.0, (size_t)3 * C.g.N, H.s);
+        net_predict(*net->n, L0.g, L0.e, H.rho.as<float>(), C.g, cv.F.cv(), C.vflag.as<uint8_t>(), cv.U.v(), H.s);
+        RT_HIP(hipStreamSynchronize(H.s));
+    })
+}
+
 int rt_net_evaluate(rt_net* net, rt_hier* h, double* loss) {
     RT_TRY({
         need(loss);

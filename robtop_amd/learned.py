@@ -94,6 +94,11 @@ class WarmStartNet:
         check(lib().rt_net_evaluate(self.handle, hg.handle, ctypes.byref(loss)))
         return loss.value
 
+    def predict_coarse(self, hg):
+        """the x4 level's U of hg = the network's prediction from the density
+        and that level's current F (FP64 level vectors)"""
+        check(lib().rt_net_predict_coarse(self.handle, hg.handle))
+
     def time_forward(self, dims_xyz, reps=20):
         """Mean ms of one forward pass on an (nx, ny, nz) box."""
         dims = np.array(dims_xyz, np.int32)
