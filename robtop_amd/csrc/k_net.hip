@@ -7,6 +7,9 @@
 // v_mfma_f32_16x16x16_bf16 operand layout (lane l, i = l % 16, q = l / 16):
 //   A: lane l holds A[i][4q .. 4q+3]   B: lane l holds B[4q .. 4q+3][i]
 //   D: lane l holds D[4q .. 4q+3][i]
+// v_mfma_f32_16x16x32_bf16 (gfx950; layers with 32 or 64 input channels):
+//   A: lane l holds A[i][8q .. 8q+7]   B: lane l holds B[8q .. 8q+7][i]
+//   D: as above
 #include "net_internal.h"
 
 #include <algorithm>
@@ -15,6 +18,7 @@ namespace rt {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
     uint32_t u = __float_as_uint(f);
@@ -61,6 +65,9 @@ __global__ void __launch_bounds__(256) k_conv(Dims D, int flip, const uint16_t* 
         const int dx = sg * (t % 3 - 1), dy = sg * ((t / 3) % 3 - 1), dz = sg * (t / 9 - 1);
         const int xx = xm + dx, zz = z + dz;
         const bool okxz = xm < nx && xx >= 0 && xx < nx && zz >= 0 && zz < nz;
+        // K per instruction: 32 input channels (8 per lane) when the layer has
+        // them, else 16 (4 per lane)
+        constexpr int KL = CIN >= 32 ? 8 : 4;
         const uint16_t* xp[MT];
         bool ok[MT];
 #pragma unroll
@@ -68,20 +75,36 @@ __global__ void __launch_bounds__(256) k_conv(Dims D, int flip, const uint16_t* 
             const int yy = y0 + r + dy;
             ok[r] = okxz && y0 + r < ny && yy >= 0 && yy < ny;
             const size_t pn = ok[r] ? (size_t)xx + (size_t)nx * ((size_t)yy + (size_t)ny * zz) : 0;
-            xp[r] = X + pn * CIN + kq * 4;
+            xp[r] = X + pn * CIN + kq * KL;
         }
-        const uint16_t* wp = W + ((size_t)t * COUT + m) * CIN + kq * 4;
+        const uint16_t* wp = W + ((size_t)t * COUT + m) * CIN + kq * KL;
+        if constexpr (CIN >= 32) {
 #pragma unroll
-        for (int kb = 0; kb < CIN / 16; kb++) {
-            s4 a[MT];
+            for (int kb = 0; kb < CIN / 32; kb++) {
+                b8 a[MT];
 #pragma unroll
-            for (int r = 0; r < MT; r++) a[r] = ok[r] ? *(const s4*)(xp[r] + kb * 16) : s4{0, 0, 0, 0};
+                for (int r = 0; r < MT; r++) a[r] = ok[r] ? *(const b8*)(xp[r] + kb * 32) : b8{};
 #pragma unroll
-            for (int nb = 0; nb < COUT / 16; nb++) {
-                const s4 b = *(const s4*)(wp + (size_t)nb * 16 * CIN + kb * 16);
+                for (int nb = 0; nb < COUT / 16; nb++) {
+                    const b8 b = *(const b8*)(wp + (size_t)nb * 16 * CIN + kb * 32);
 #pragma unroll
-                for (int r = 0; r < MT; r++)
-                    acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[r], b, acc[r][nb], 0, 0, 0);
+                    for (int r = 0; r < MT; r++)
+                        acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], b, acc[r][nb], 0, 0, 0);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int kb = 0; kb < CIN / 16; kb++) {
+                s4 a[MT];
+#pragma unroll
+                for (int r = 0; r < MT; r++) a[r] = ok[r] ? *(const s4*)(xp[r] + kb * 16) : s4{0, 0, 0, 0};
+#pragma unroll
+                for (int nb = 0; nb < COUT / 16; nb++) {
+                    const s4 b = *(const s4*)(wp + (size_t)nb * 16 * CIN + kb * 16);
+#pragma unroll
+                    for (int r = 0; r < MT; r++)
+                        acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[r], b, acc[r][nb], 0, 0, 0);
+                }
             }
         }
     }
