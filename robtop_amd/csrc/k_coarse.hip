@@ -917,12 +917,17 @@ __device__ __forceinline__ double blk_sum(double v) {
 }
 
 // partials of a.b, four FP32 values per lane
+// chunk (optional): one byte per 1024 float4 (4096 floats) of the vectors,
+// 0 where every value is known to be 0 (no active node): those are skipped,
+// which leaves every sum and vector unchanged
 __global__ void __launch_bounds__(256) k_dot4_partials(const float* __restrict__ a, const float* __restrict__ b,
-                                                       size_t n4, double* __restrict__ partials) {
+                                                       size_t n4, double* __restrict__ partials,
+                                                       const uint8_t* __restrict__ chunk) {
     const float4* a4 = reinterpret_cast<const float4*>(a);
     const float4* b4 = reinterpret_cast<const float4*>(b);
     double s = 0;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
+        if (chunk && !chunk[i >> 10]) continue;
         const float4 x = a4[i], y = b4[i];
         s += ((double)x.x * y.x + (double)x.y * y.y) + ((double)x.z * y.z + (double)x.w * y.w);
     }
@@ -941,10 +946,11 @@ __global__ void __launch_bounds__(256) k_dot_partials(const TA* __restrict__ a, 
 }
 
 template <typename TA, typename TB>
-void launch_dot_partials_t(const TA* a, const TB* b, size_t n, double* partials, int nparts, hipStream_t s) {
+void launch_dot_partials_t(const TA* a, const TB* b, size_t n, double* partials, int nparts, hipStream_t s,
+                           const uint8_t* chunk) {
     if constexpr (std::is_same<TA, float>::value && std::is_same<TB, float>::value) {
         if (n % 4 == 0) {
-            hipLaunchKernelGGL(k_dot4_partials, dim3(nparts), dim3(256), 0, s, a, b, n / 4, partials);
+            hipLaunchKernelGGL(k_dot4_partials, dim3(nparts), dim3(256), 0, s, a, b, n / 4, partials, chunk);
             RT_HIP(hipGetLastError());
             return;
         }
@@ -953,7 +959,20 @@ void launch_dot_partials_t(const TA* a, const TB* b, size_t n, double* partials,
     RT_HIP(hipGetLastError());
 }
 void launch_dot_partials(const double* a, const double* b, size_t n, double* partials, int nparts, hipStream_t s) {
-    launch_dot_partials_t<double, double>(a, b, n, partials, nparts, s);
+    launch_dot_partials_t<double, double>(a, b, n, partials, nparts, s, nullptr);
+}
+
+// chunk mask of a finest-level vector (3 components of N slots): byte c = 1 if
+// floats [4096 c, 4096 (c + 1)) hold an active node's value
+__global__ void k_vec_chunks(const uint8_t* __restrict__ vflag, int N, uint8_t* __restrict__ chunk) {
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < N; s += gridDim.x * blockDim.x)
+        if (vflag[s] & NF_ACTIVE)
+            for (int k = 0; k < 3; k++) chunk[((size_t)k * N + s) >> 12] = 1;
+}
+void launch_vec_chunks(const uint8_t* vflag, int N, uint8_t* chunk, hipStream_t s) {
+    hipLaunchKernelGGL(k_vec_chunks, dim3((unsigned)std::min<long long>(((long long)N + 255) / 256, 16384)), dim3(256),
+                       0, s, vflag, N, chunk);
+    RT_HIP(hipGetLastError());
 }
 
 // one workgroup of 1024 threads; each thread keeps four running sums over
@@ -1008,13 +1027,15 @@ void launch_pcg_update_x_r(T* x, T* r, const T* p, const T* q, double alpha, siz
 // the HBM rate.
 __global__ void __launch_bounds__(256) k_pcg_ru_dxr(float* __restrict__ r, const float* __restrict__ q,
                                                     float* __restrict__ dx, const float* __restrict__ p, float a,
-                                                    size_t n4, double* __restrict__ partials) {
+                                                    size_t n4, double* __restrict__ partials,
+                                                    const uint8_t* __restrict__ chunk) {
     float4* r4 = reinterpret_cast<float4*>(r);
     float4* d4 = reinterpret_cast<float4*>(dx);
     const float4* q4 = reinterpret_cast<const float4*>(q);
     const float4* p4 = reinterpret_cast<const float4*>(p);
     double s = 0;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
+        if (chunk && !chunk[i >> 10]) continue;
         float4 ri = r4[i];
         const float4 qi = q4[i], pi = p4[i];
         float4 di = d4[i];
@@ -1028,9 +1049,9 @@ __global__ void __launch_bounds__(256) k_pcg_ru_dxr(float* __restrict__ r, const
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 void launch_pcg_ru_dxr(float* r, const float* q, float* dx, const float* p, double alpha, size_t n, double* partials,
-                       int nparts, hipStream_t s) {
+                       int nparts, hipStream_t s, const uint8_t* chunk) {
     if (n % 4) throw std::logic_error("launch_pcg_ru_dxr: n must be a multiple of 4");
-    hipLaunchKernelGGL(k_pcg_ru_dxr, dim3(nparts), dim3(256), 0, s, r, q, dx, p, (float)alpha, n / 4, partials);
+    hipLaunchKernelGGL(k_pcg_ru_dxr, dim3(nparts), dim3(256), 0, s, r, q, dx, p, (float)alpha, n / 4, partials, chunk);
     RT_HIP(hipGetLastError());
 }
 
@@ -1049,10 +1070,11 @@ void launch_pcg_flush_x(double* x, float* dx, size_t n, hipStream_t s) {
 
 // p = z + b p, four FP32 values per lane
 __global__ void __launch_bounds__(256) k_pcg_p4(float* __restrict__ p, const float* __restrict__ z, float b,
-                                                size_t n4) {
+                                                size_t n4, const uint8_t* __restrict__ chunk) {
     float4* p4 = reinterpret_cast<float4*>(p);
     const float4* z4 = reinterpret_cast<const float4*>(z);
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
+        if (chunk && !chunk[i >> 10]) continue;
         const float4 zi = z4[i];
         float4 pi = p4[i];
         pi.x = zi.x + b * pi.x; pi.y = zi.y + b * pi.y; pi.z = zi.z + b * pi.z; pi.w = zi.w + b * pi.w;
@@ -1069,10 +1091,10 @@ __global__ void k_pcg_p(T* __restrict__ p, const TZ* __restrict__ z, double beta
         p[i] = (T)z[i] + b * p[i];
 }
 template <typename T, typename TZ>
-void launch_pcg_update_p(T* p, const TZ* z, double beta, size_t n, hipStream_t s) {
+void launch_pcg_update_p(T* p, const TZ* z, double beta, size_t n, hipStream_t s, const uint8_t* chunk) {
     if constexpr (std::is_same<T, float>::value && std::is_same<TZ, float>::value) {
         if (n % 4 == 0) {
-            hipLaunchKernelGGL(k_pcg_p4, dim3(kStreamBlocks), dim3(256), 0, s, p, z, (float)beta, n / 4);
+            hipLaunchKernelGGL(k_pcg_p4, dim3(kStreamBlocks), dim3(256), 0, s, p, z, (float)beta, n / 4, chunk);
             RT_HIP(hipGetLastError());
             return;
         }
@@ -1225,13 +1247,16 @@ template void launch_convert<double, float>(const double*, float*, size_t, hipSt
 template void launch_convert<float, double>(const float*, double*, size_t, hipStream_t);
 template void launch_convert<float, float>(const float*, float*, size_t, hipStream_t);
 template void launch_convert<double, double>(const double*, double*, size_t, hipStream_t);
-template void launch_dot_partials_t<double, double>(const double*, const double*, size_t, double*, int, hipStream_t);
-template void launch_dot_partials_t<double, float>(const double*, const float*, size_t, double*, int, hipStream_t);
-template void launch_dot_partials_t<float, float>(const float*, const float*, size_t, double*, int, hipStream_t);
+template void launch_dot_partials_t<double, double>(const double*, const double*, size_t, double*, int, hipStream_t,
+                                                  const uint8_t*);
+template void launch_dot_partials_t<double, float>(const double*, const float*, size_t, double*, int, hipStream_t,
+                                                  const uint8_t*);
+template void launch_dot_partials_t<float, float>(const float*, const float*, size_t, double*, int, hipStream_t,
+                                                  const uint8_t*);
 #define RT_INST_PCG(T, TZ)                                                                                        \
     template void launch_pcg_update_x_r<T, TZ>(T*, T*, const T*, const T*, double, size_t, double*, int, TZ*,        \
                                                hipStream_t);                                                       \
-    template void launch_pcg_update_p<T, TZ>(T*, const TZ*, double, size_t, hipStream_t);                          \
+    template void launch_pcg_update_p<T, TZ>(T*, const TZ*, double, size_t, hipStream_t, const uint8_t*);                          \
     template void launch_sub_conv<T, TZ>(const T*, const T*, T*, TZ*, size_t, double*, int, hipStream_t);          \
     template void launch_add_conv<T, TZ>(T*, const TZ*, size_t, hipStream_t);
 RT_INST_PCG(double, float)

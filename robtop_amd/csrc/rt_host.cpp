@@ -124,6 +124,7 @@ struct Hier {
     // run per colour pair at rzpart_off[pair]
     DBuf rzpart;
     int rzpart_n = 0, rzpart_off[4] = {0, 0, 0, 0};
+    DBuf vchunk;                     // finest-level vector chunks holding an active node (launch_vec_chunks)
     // The preconditioner cycle may stop one level early and solve that level
     // exactly when it has at most pc_direct_dofs unknowns (pc_levels = the
     // number of levels its cycle visits); its dense inverse lives here.
@@ -601,6 +602,19 @@ static double* ensure_rzpart(Hier& H) {
     return H.rzpart.as<double>();
 }
 
+// the chunk mask of the finest level's vectors (built once: node activity is
+// fixed with the hierarchy); the reliable-update PCG's streaming kernels skip
+// the chunks whose values are all 0
+static const uint8_t* vec_chunks(Hier& H) {
+    if (!H.vchunk.p) {
+        const Level& L0 = H.lv[0];
+        const size_t nch = ((size_t)3 * L0.g.N + 4095) / 4096;
+        H.vchunk.alloc(nch);
+        launch_vec_chunks(L0.vflag.as<uint8_t>(), L0.g.N, H.vchunk.as<uint8_t>(), H.s);
+    }
+    return H.vchunk.as<uint8_t>();
+}
+
 // sweep multiplier of coarse level i in the preconditioner's cycle
 static int level_mult(const Hier& H, int i) {
     return i < (int)H.pc_lv_mult.size() && H.pc_lv_mult[i] > 0 ? H.pc_lv_mult[i] : H.pc_mult;
@@ -723,7 +737,11 @@ static double sum_partials_host(Hier& H, int np) {
 // r . z after precondition<float>(H, pc, pre, post): from the last fine
 // sweep's partials when it wrote them, else a separate pass over both vectors
 static double precond_rz(Hier& H, LevelVecs<float>& pc, int post, size_t n) {
-    if (!rz_fused(H, post)) return dot<float, float>(H, pc.F.data(), pc.U.data(), n);
+    if (!rz_fused(H, post)) {
+        launch_dot_partials_t<float, float>(pc.F.data(), pc.U.data(), n, H.partials.as<double>(), kStreamBlocks, H.s,
+                                            vec_chunks(H));
+        return sum_partials_host(H, kStreamBlocks);
+    }
     launch_sum_partials(H.rzpart.as<double>(), H.rzpart_n, H.scal.as<double>(), H.s);
     double v;
     RT_HIP(hipMemcpyAsync(&v, H.scal.p, sizeof(double), hipMemcpyDeviceToHost, H.s));
@@ -1224,7 +1242,7 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
             const double alpha = rz / pq;
             H.cg_alpha.push_back(alpha);
             launch_pcg_ru_dxr(pc.F.data(), H.q32.data(), H.dx32.data(), H.p32.data(), alpha, n,
-                              H.partials.as<double>(), ns, H.s);
+                              H.partials.as<double>(), ns, H.s, vec_chunks(H));
             flushed = false;
             const double rn = std::sqrt(sum_partials_host(H, ns));
             rel = rn / nb;
@@ -1245,7 +1263,7 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
             const double beta = rz_new / rz;
             H.cg_beta.push_back(beta);
             rz = rz_new;
-            launch_pcg_update_p<float, float>(H.p32.data(), pc.U.data(), beta, n, H.s);
+            launch_pcg_update_p<float, float>(H.p32.data(), pc.U.data(), beta, n, H.s, vec_chunks(H));
         }
         if (!flushed) launch_pcg_flush_x(H.x64.data(), H.dx32.data(), n, H.s);
         if (it > maxit) it = maxit;
@@ -2643,7 +2661,8 @@ static void pcg_batch_reliable(Hier& H, int nb, double tol, int maxit, int pre, 
     if (live) {
         precondition_batch(H, nb, pre, post);
         for (int b = 0; b < nb; b++) {
-            launch_dot_partials_t<float, float>(L0.bF.at(b), L0.bU.at(b), n, part + (size_t)b * npa, ns, H.s);
+            launch_dot_partials_t<float, float>(L0.bF.at(b), L0.bU.at(b), n, part + (size_t)b * npa, ns, H.s,
+                                                vec_chunks(H));
             launch_convert<float, float>(L0.bU.at(b), H.bp32.at(b), n, H.s);
         }
         batch_sums(H, nb, ns, npa, rz);
@@ -2659,7 +2678,7 @@ static void pcg_batch_reliable(Hier& H, int nb, double tol, int maxit, int pre, 
             if (!act[b]) continue;
             if (!(pq[b] > 0)) throw std::domain_error("PCG breakdown: p.Ap <= 0");
             launch_pcg_ru_dxr(L0.bF.at(b), H.bq32.at(b), H.bdx32.at(b), H.bp32.at(b), rz[b] / pq[b], n,
-                              part + (size_t)b * npa, ns, H.s);
+                              part + (size_t)b * npa, ns, H.s, vec_chunks(H));
             dirty[b] = true;
         }
         batch_sums(H, nb, ns, npa, rn);
@@ -2693,12 +2712,14 @@ static void pcg_batch_reliable(Hier& H, int nb, double tol, int maxit, int pre, 
         if (!live) break;
         precondition_batch(H, nb, pre, post);
         for (int b = 0; b < nb; b++)
-            if (act[b]) launch_dot_partials_t<float, float>(L0.bF.at(b), L0.bU.at(b), n, part + (size_t)b * npa, ns, H.s);
+            if (act[b])
+                launch_dot_partials_t<float, float>(L0.bF.at(b), L0.bU.at(b), n, part + (size_t)b * npa, ns, H.s,
+                                                    vec_chunks(H));
         double rzn[kBatchMax];
         batch_sums(H, nb, ns, npa, rzn);
         for (int b = 0; b < nb; b++) {
             if (!act[b]) continue;
-            launch_pcg_update_p<float, float>(H.bp32.at(b), L0.bU.at(b), rzn[b] / rz[b], n, H.s);
+            launch_pcg_update_p<float, float>(H.bp32.at(b), L0.bU.at(b), rzn[b] / rz[b], n, H.s, vec_chunks(H));
             rz[b] = rzn[b];
         }
     }

@@ -199,14 +199,15 @@ template <typename T>
 void launch_fill(T* a, T v, size_t n, hipStream_t s);
 void launch_dot_partials(const double* a, const double* b, size_t n, double* partials, int nparts, hipStream_t s);
 template <typename TA, typename TB>
-void launch_dot_partials_t(const TA* a, const TB* b, size_t n, double* partials, int nparts, hipStream_t s);
+void launch_dot_partials_t(const TA* a, const TB* b, size_t n, double* partials, int nparts, hipStream_t s,
+                           const uint8_t* chunk = nullptr);
 void launch_sum_partials(const double* partials, int nparts, double* out, hipStream_t s);
 template <typename T, typename TZ>
 void launch_pcg_update_x_r(T* x, T* r, const T* p, const T* q, double alpha, size_t n, double* partials, int nparts,
                            TZ* rz_out, hipStream_t s);
 // reliable-update PCG (FP32 vectors): r -= alpha q, dx += alpha p, partials of r.r
 void launch_pcg_ru_dxr(float* r, const float* q, float* dx, const float* p, double alpha, size_t n, double* partials,
-                       int nparts, hipStream_t s);
+                       int nparts, hipStream_t s, const uint8_t* chunk = nullptr);
 // x64 += dx, dx = 0
 void launch_pcg_flush_x(double* x, float* dx, size_t n, hipStream_t s);
 // streaming kernels' grid size (partials written by the reducing ones)
@@ -214,8 +215,11 @@ void launch_pcg_flush_x(double* x, float* dx, size_t n, hipStream_t s);
 #define RT_STREAM_BLOCKS 2048
 #endif
 constexpr int kStreamBlocks = RT_STREAM_BLOCKS;
+// chunk mask of the finest level's vectors for the streaming PCG kernels (one
+// byte per 4096 floats: 0 where no active node's value lies)
+void launch_vec_chunks(const uint8_t* vflag, int N, uint8_t* chunk, hipStream_t s);
 template <typename T, typename TZ>
-void launch_pcg_update_p(T* p, const TZ* z, double beta, size_t n, hipStream_t s);
+void launch_pcg_update_p(T* p, const TZ* z, double beta, size_t n, hipStream_t s, const uint8_t* chunk = nullptr);
 template <typename T, typename TZ>
 void launch_sub_conv(const T* b, const T* q, T* r, TZ* rz, size_t n, double* partials, int nparts, hipStream_t s);
 template <typename T, typename TE>
