@@ -255,27 +255,33 @@ int rt_volume(rt_hier* h, double* v);
 /* --- worst-case load search --------------------------------------------- */
 
 /* Load nodes for the worst-case search. Replaces setLoadNodes
- * (optimization/projection.cpp:129-269) for the with-support modes: load_box
- * flags load nodes on the finest node box (supports and absent nodes are
- * dropped); normals_box (FP64, 3 x n) gives outward normals for the
- * constrained-direction mode wscf (only the normal component of a load is
- * admissible), or is NULL for free loads (wsff). */
+ * (optimization/projection.cpp:129-269): load_box flags load nodes on the
+ * finest node box (supports and absent nodes are dropped); normals_box (FP64,
+ * 3 x n) gives outward normals for the constrained-direction modes wscf / nscf
+ * (only the normal component of a load is admissible), or is NULL for free
+ * loads (wsff / nsff). The no-support modes (nsff, nscf) are the ones of a
+ * hierarchy without supports: their loads must also be balanced (no net force
+ * or moment over the load nodes, projection.cpp:208-257, :322-335). */
 int rt_set_load_nodes(rt_hier* h, const uint8_t* load_box, const double* normals_box);
 int rt_num_load_nodes(const rt_hier* h, int* n);
 /* forceProject (optimization/projection.cu:165-231): the finest force is
- * cleared off the load nodes and, with normals, reduced to its normal part. */
+ * cleared off the load nodes and, with normals, reduced to its normal part;
+ * without supports its rigid part over the load nodes is removed as well. */
 int rt_force_project(rt_hier* h);
 /* Grid::randForce (grid/Grid.cu:2242-2245): force components uniform in
  * [-1, 1) on existing nodes, from a counter-based generator seeded by `seed`
  * (the reference seeds cuRAND from the clock). */
 int rt_rand_force(rt_hier* h, uint64_t seed);
-/* modifiedPM (optimization/optimization.cpp:123-261), with-support branch,
- * starting from the current force: project and normalise, u = 0, then per
- * step one V(1,1) cycle (precision of rt_set_vcycle_precision) and
- * f <- P u / |P u|, until the load change |fs_new - fs_old| / |fs_old| is at
- * most 1e-4 and the cycle's relative residual at most 1e-2, or max_itn steps.
- * The worst load and its displacement stay in the force / displacement;
- * returns the worst compliance f.u and the number of V-cycles. */
+/* modifiedPM (optimization/optimization.cpp:123-261), starting from the
+ * current force: project and normalise, u = 0, then per step one V(1,1) cycle
+ * (precision of rt_set_vcycle_precision) and f <- P u / |P u|, until the load
+ * change |fs_new - fs_old| / |fs_old| is at most 1e-4 and the cycle's
+ * relative residual at most 1e-2, or max_itn steps. Without supports (the
+ * no-support modes) the coarsest levels are solved for the minimum-norm
+ * solution and the rigid motion is removed from u after every cycle
+ * (displacementProject, projection.cpp:374-394). The worst load and its
+ * displacement stay in the force / displacement; returns the worst
+ * compliance f.u and the number of V-cycles. */
 int rt_modified_pm(rt_hier* h, int max_itn, double* worst_compliance, int* vcycles);
 
 /* Worst-case load search by block Lanczos (no counterpart in the reference,

@@ -72,7 +72,70 @@ struct Hier {
     std::vector<int> lastrow;   // node -> row block, -1 if absent
     int nlastrows = 0;
     std::vector<double> Lfac;   // Cholesky factor (row-major, lower)
+    // no supports: orthonormal rigid motions of the coarsest rows (N x 6,
+    // row-major) and of the finest nodes (6 box vectors), and of the load
+    // nodes (the admissible ones, nrd of them; 6 box vectors)
+    std::vector<double> Rc;
+    std::vector<double> Rf[6][3];
+    std::vector<double> Rd[6][3];
+    int nrd = 0;
 };
+
+// whether any finest node is a support
+static bool has_support(const Hier* h) {
+    for (uint8_t f : h->lv[0].vfix) if (f) return true;
+    return false;
+}
+
+// rigid motions at points p (n x 3): columns e_x, e_y, e_z and the rotations
+// (projection.cpp:46-112: [I | phat(p)] per node, phat(p) x = p x (rows
+// (0, -p2, p1), (p2, 0, -p0), (-p1, p0, 0))), dof 3 i + c; out: 3n x 6 row-major
+static void rigid_columns(const std::vector<double>& p, std::vector<double>& R) {
+    const size_t n = p.size() / 3;
+    R.assign(3 * n * 6, 0.0);
+    for (size_t i = 0; i < n; i++) {
+        const double x = p[3 * i], y = p[3 * i + 1], z = p[3 * i + 2];
+        const double ph[3][3] = {{0, -z, y}, {z, 0, -x}, {-y, x, 0}};
+        for (int c = 0; c < 3; c++) {
+            R[(3 * i + c) * 6 + c] = 1.0;
+            for (int k = 0; k < 3; k++) R[(3 * i + c) * 6 + 3 + k] = ph[c][k];
+        }
+    }
+}
+
+// modified Gram-Schmidt on the 6 columns in order (projection.cpp:96-101, and
+// :232-257 dropping a column whose norm falls below 1e-6 of its own or 1e-6);
+// returns the number of columns kept, compacted to the front
+static int orthonormalize(std::vector<double>& R, size_t rows) {
+    bool keep[6];
+    for (int i = 0; i < 6; i++) {
+        double old = 0;
+        for (size_t r = 0; r < rows; r++) old += R[r * 6 + i] * R[r * 6 + i];
+        old = std::sqrt(old);
+        for (int j = 0; j < i; j++) {
+            if (!keep[j]) continue;
+            double w = 0;
+            for (size_t r = 0; r < rows; r++) w += R[r * 6 + i] * R[r * 6 + j];
+            for (size_t r = 0; r < rows; r++) R[r * 6 + i] -= w * R[r * 6 + j];
+        }
+        double res = 0;
+        for (size_t r = 0; r < rows; r++) res += R[r * 6 + i] * R[r * 6 + i];
+        res = std::sqrt(res);
+        keep[i] = !(res < 1e-6 * old || old < 1e-6);
+        if (keep[i])
+            for (size_t r = 0; r < rows; r++) R[r * 6 + i] /= res;
+    }
+    int m = 0;
+    for (int i = 0; i < 6; i++) {
+        if (!keep[i]) continue;
+        if (m != i)
+            for (size_t r = 0; r < rows; r++) R[r * 6 + m] = R[r * 6 + i];
+        m++;
+    }
+    for (size_t r = 0; r < rows; r++)
+        for (int i = m; i < 6; i++) R[r * 6 + i] = 0.0;
+    return m;
+}
 
 // ---------------------------------------------------------------------------
 // KE: element/templateMatrix.cpp:15-29 (dN) and :31-124 (2x2x2 Gauss, D(E,nu))
@@ -516,6 +579,28 @@ void build_coarsest(Hier* h) {
                 K[(size_t)(h->lastrow[v] * 3 + r) * N + h->lastrow[j] * 3 + q] = L.st[((size_t)nb * 9 + r * 3 + q) * nn + v];
         }
     }
+    // No supports (the no-support modes, projection.cpp:374-411): K has the
+    // rigid motions R of the level's nodes as its null space. The reference
+    // takes BDCSVD's minimum-norm solution (Grid.cpp:1981-2043); the same
+    // solution is P (K + a R R^T)^-1 P b with P = I - R R^T and a > 0 (the mean
+    // diagonal), which stays a Cholesky solve.
+    h->Rc.clear();
+    if (!has_support(h)) {
+        std::vector<double> pos;
+        for (int z = 0; z < L.n[2]; z++) for (int y = 0; y < L.n[1]; y++) for (int x = 0; x < L.n[0]; x++)
+            if (L.vact[L.vid(x, y, z)]) { pos.push_back(x); pos.push_back(y); pos.push_back(z); }
+        rigid_columns(pos, h->Rc);
+        if (orthonormalize(h->Rc, (size_t)N) != 6) throw std::runtime_error("coarsest level has fewer than 6 rigid motions");
+        double a = 0;
+        for (int i = 0; i < N; i++) a += K[(size_t)i * N + i];
+        a /= N;
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < N; j++) {
+                double s = 0;
+                for (int k = 0; k < 6; k++) s += h->Rc[(size_t)i * 6 + k] * h->Rc[(size_t)j * 6 + k];
+                K[(size_t)i * N + j] += a * s;
+            }
+    }
     // Cholesky (lower), in place
     for (int j = 0; j < N; j++) {
         double d = K[(size_t)j * N + j];
@@ -539,8 +624,17 @@ void solve_coarsest(Hier* h) {
     std::vector<double> b(N);
     for (size_t v = 0; v < L.nn(); v++) if (h->lastrow[v] >= 0) for (int c = 0; c < 3; c++) b[h->lastrow[v] * 3 + c] = L.F[c][v];
     const std::vector<double>& K = h->Lfac;
+    // b <- P b, x <- P x with P = I - Rc Rc^T (no supports)
+    auto project = [&](std::vector<double>& v) {
+        if (h->Rc.empty()) return;
+        double w[6] = {0, 0, 0, 0, 0, 0};
+        for (int i = 0; i < N; i++) for (int k = 0; k < 6; k++) w[k] += h->Rc[(size_t)i * 6 + k] * v[i];
+        for (int i = 0; i < N; i++) for (int k = 0; k < 6; k++) v[i] -= h->Rc[(size_t)i * 6 + k] * w[k];
+    };
+    project(b);
     for (int i = 0; i < N; i++) { double s = b[i]; for (int k = 0; k < i; k++) s -= K[(size_t)i * N + k] * b[k]; b[i] = s / K[(size_t)i * N + i]; }
     for (int i = N - 1; i >= 0; i--) { double s = b[i]; for (int k = i + 1; k < N; k++) s -= K[(size_t)k * N + i] * b[k]; b[i] = s / K[(size_t)i * N + i]; }
+    project(b);
     for (size_t v = 0; v < L.nn(); v++) for (int c = 0; c < 3; c++) L.U[c][v] = h->lastrow[v] >= 0 ? b[h->lastrow[v] * 3 + c] : 0.0;
 }
 
@@ -598,6 +692,44 @@ void force_project(Hier* h, std::vector<double>* f) {
             for (int c = 0; c < 3; c++) f[c][v] = d * h->nrm[c][v];
         }
     }
+    // no supports (nsff, nscf; projection.cpp:322-335): the load must also be
+    // balanced, f -= Rd Rd^T f with Rd the admissible rigid motions of the
+    // load nodes
+    if (!has_support(h)) {
+        double w[6] = {0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < h->nrd; k++)
+            for (int c = 0; c < 3; c++)
+                for (size_t v = 0; v < L.nn(); v++) w[k] += h->Rd[k][c][v] * f[c][v];
+        for (int k = 0; k < h->nrd; k++)
+            for (int c = 0; c < 3; c++)
+                for (size_t v = 0; v < L.nn(); v++) f[c][v] -= w[k] * h->Rd[k][c][v];
+    }
+}
+
+// displacementProject (projection.cpp:374-394): u -= (u . R_k) R_k for the
+// orthonormal rigid motions R_k of the finest nodes, one after the other
+static void displacement_project(Hier* h, std::vector<double>* u) {
+    Level& L = h->lv[0];
+    const size_t nn = L.nn();
+    if (h->Rf[0][0].empty()) {
+        std::vector<double> pos;
+        std::vector<size_t> ids;
+        for (int z = 0; z < L.n[2]; z++) for (int y = 0; y < L.n[1]; y++) for (int x = 0; x < L.n[0]; x++)
+            if (L.vact[L.vid(x, y, z)]) { pos.push_back(x); pos.push_back(y); pos.push_back(z); ids.push_back(L.vid(x, y, z)); }
+        std::vector<double> R;
+        rigid_columns(pos, R);
+        if (orthonormalize(R, 3 * ids.size()) != 6) throw std::runtime_error("fewer than 6 rigid motions");
+        for (int k = 0; k < 6; k++)
+            for (int c = 0; c < 3; c++) {
+                h->Rf[k][c].assign(nn, 0.0);
+                for (size_t i = 0; i < ids.size(); i++) h->Rf[k][c][ids[i]] = R[(3 * i + c) * 6 + k];
+            }
+    }
+    for (int k = 0; k < 6; k++) {
+        double w = 0;
+        for (int c = 0; c < 3; c++) for (size_t v = 0; v < nn; v++) w += u[c][v] * h->Rf[k][c][v];
+        for (int c = 0; c < 3; c++) for (size_t v = 0; v < nn; v++) u[c][v] -= w * h->Rf[k][c][v];
+    }
 }
 
 // modifiedPM, with-support branch (optimization.cpp:123-261) from the force
@@ -622,6 +754,8 @@ double modified_pm(Hier* h, int max_itn, int* steps) {
     while (itn++ < max_itn && (fch > 1e-4 || rel > 1e-2)) {
         rel = vcycle(h, 1, 1, false);
         nv++;
+        // no supports: remove the rigid motion (optimization.cpp:189-194)
+        if (!has_support(h)) displacement_project(h, L.U);
         if (rel > 1e4) throw std::runtime_error("modified power method diverged");
         for (int c = 0; c < 3; c++) L.F[c] = L.U[c];
         force_project(h, L.F);
@@ -907,6 +1041,28 @@ void or_set_load_nodes(void* hp, const uint8_t* load, const double* normals) {
             const double a = normals[v], b = normals[nn + v], d = normals[2 * nn + v];
             const double l = std::sqrt(a * a + b * b + d * d);
             if (h->load[v] && l > 0) { h->nrm[0][v] = a / l; h->nrm[1][v] = b / l; h->nrm[2][v] = d / l; }
+        }
+    // admissible rigid motions of the load nodes (projection.cpp:208-257):
+    // with normals only their normal part, then orthonormalised
+    std::vector<double> pos;
+    std::vector<size_t> ids;
+    for (int z = 0; z < L.n[2]; z++) for (int y = 0; y < L.n[1]; y++) for (int x = 0; x < L.n[0]; x++)
+        if (h->load[L.vid(x, y, z)]) { pos.push_back(x); pos.push_back(y); pos.push_back(z); ids.push_back(L.vid(x, y, z)); }
+    std::vector<double> R;
+    rigid_columns(pos, R);
+    if (h->normals)
+        for (size_t i = 0; i < ids.size(); i++)
+            for (int k = 0; k < 6; k++) {
+                double d = 0;
+                for (int c = 0; c < 3; c++) d += R[(3 * i + c) * 6 + k] * h->nrm[c][ids[i]];
+                for (int c = 0; c < 3; c++) R[(3 * i + c) * 6 + k] = d * h->nrm[c][ids[i]];
+            }
+    h->nrd = ids.empty() ? 0 : orthonormalize(R, 3 * ids.size());
+    for (int k = 0; k < 6; k++)
+        for (int c = 0; c < 3; c++) {
+            h->Rd[k][c].assign(nn, 0.0);
+            if (k < h->nrd)
+                for (size_t i = 0; i < ids.size(); i++) h->Rd[k][c][ids[i]] = R[(3 * i + c) * 6 + k];
         }
 }
 void or_force_project(void* hp) { Hier* h = (Hier*)hp; force_project(h, h->lv[0].F); }

@@ -115,6 +115,12 @@ struct Hier {
     // worst-case load search: load-node slots, unit normals (constrained
     // direction), projected force and the previous step's force, all [3][nload]
     DBuf load_slots, load_nrm, pm_fs, pm_old;
+    // no-support modes: the admissible rigid motions of the load nodes
+    // (compact [k][c * nload + i], nrd of them) and the finest nodes' rigid
+    // motions (6 vectors, built on first use)
+    DBuf load_rd;
+    int nrd = 0;
+    std::vector<V3<double>> rigid_f;
     int nload = 0;
     bool load_normals = false;
     int vprec = 64;
@@ -766,6 +772,59 @@ static void gemm(Hier& H, rocblas_operation ta, rocblas_operation tb, int M, int
         throw std::runtime_error("rocblas_dgemm failed");
 }
 
+// whether any finest node is a support (the with-support modes); without
+// supports the stiffness has the rigid motions as its null space
+static bool has_supports(const Hier& H) {
+    for (uint8_t f : H.fixed_box) if (f) return true;
+    return false;
+}
+
+// rigid motions at points p (3 per point) as 6 columns of 3 n dofs (dof
+// 3 i + c, column-major, ld = 3 n): e_x, e_y, e_z and the rotations [phat(p)]
+// of projection.cpp:46-112; then modified Gram-Schmidt in that order, a
+// column dropped when it falls below 1e-6 of its norm (projection.cpp:232-257).
+// Returns the number of columns kept, compacted to the front.
+static int rigid_basis(const std::vector<double>& p, std::vector<double>& R) {
+    const size_t n = p.size() / 3, m = 3 * n;
+    R.assign(m * 6, 0.0);
+    for (size_t i = 0; i < n; i++) {
+        const double x = p[3 * i], y = p[3 * i + 1], z = p[3 * i + 2];
+        const double ph[3][3] = {{0, -z, y}, {z, 0, -x}, {-y, x, 0}};
+        for (int c = 0; c < 3; c++) {
+            R[c * m + 3 * i + c] = 1.0;
+            for (int k = 0; k < 3; k++) R[(3 + k) * m + 3 * i + c] = ph[c][k];
+        }
+    }
+    bool keep[6];
+    for (int i = 0; i < 6; i++) {
+        double* ci = R.data() + i * m;
+        double old = 0;
+        for (size_t r = 0; r < m; r++) old += ci[r] * ci[r];
+        old = std::sqrt(old);
+        for (int j = 0; j < i; j++) {
+            if (!keep[j]) continue;
+            const double* cj = R.data() + j * m;
+            double w = 0;
+            for (size_t r = 0; r < m; r++) w += ci[r] * cj[r];
+            for (size_t r = 0; r < m; r++) ci[r] -= w * cj[r];
+        }
+        double res = 0;
+        for (size_t r = 0; r < m; r++) res += ci[r] * ci[r];
+        res = std::sqrt(res);
+        keep[i] = !(res < 1e-6 * old || old < 1e-6);
+        if (keep[i])
+            for (size_t r = 0; r < m; r++) ci[r] /= res;
+    }
+    int k = 0;
+    for (int i = 0; i < 6; i++) {
+        if (!keep[i]) continue;
+        if (k != i) std::copy(R.begin() + i * m, R.begin() + (i + 1) * m, R.begin() + k * m);
+        k++;
+    }
+    R.resize((size_t)k * m);
+    return k;
+}
+
 // dense exact solve of level li: the matrix assembled on the device from its
 // FP64 stencils, Cholesky and inverse there too (rocSOLVER, FP64); Ainv
 // n3 x n3, rowslot[r] = slot of the level's r-th active node
@@ -774,6 +833,7 @@ static void factor_level(Hier& H, int li, DBuf& Ainv, DBuf& rowslot, DBuf& fbuf,
     const size_t N = (size_t)L.g.N;
     const int nx = L.g.n[0], ny = L.g.n[1], nz = L.g.n[2];
     std::vector<int> slots, rowof(N, -1);
+    std::vector<double> pos;
     for (int z = 0; z < nz; z++)
         for (int y = 0; y < ny; y++)
             for (int x = 0; x < nx; x++)
@@ -781,6 +841,7 @@ static void factor_level(Hier& H, int li, DBuf& Ainv, DBuf& rowslot, DBuf& fbuf,
                     const int s = oct_p(L.g, x, y, z);
                     rowof[s] = (int)slots.size();
                     slots.push_back(s);
+                    pos.push_back(x); pos.push_back(y); pos.push_back(z);
                 }
     const int nr = (int)slots.size(), n = 3 * nr;
     std::vector<int> nbcol((size_t)27 * nr, -1);
@@ -802,10 +863,32 @@ static void factor_level(Hier& H, int li, DBuf& Ainv, DBuf& rowslot, DBuf& fbuf,
     RT_HIP(hipMemcpyAsync(nbc.p, nbcol.data(), sizeof(int) * nbcol.size(), hipMemcpyHostToDevice, H.s));
     Ainv.ensure(sizeof(double) * std::max<size_t>((size_t)n * n, 1));
     launch_dense_from_stencil(L.st64.as<double>(), N, rowslot.as<int>(), nbc.as<int>(), nr, Ainv.as<double>(), H.s);
+    ensure_blas(H);
+    // No supports (the no-support modes): the level's matrix has the rigid
+    // motions R of its nodes as null space. The reference's coarsest solve is
+    // BDCSVD's minimum-norm solution (Grid.cpp:1981-2043); the same is
+    // P (A + a R R^T)^-1 P b with P = I - R R^T and a the mean diagonal, so
+    // A + a R R^T is factored and the stored inverse is P (A + a R R^T)^-1 P.
+    const bool singular = !has_supports(H);
+    DBuf Rd_;
+    if (singular) {
+        std::vector<double> R;
+        if (rigid_basis(pos, R) != 6) throw std::domain_error("level has fewer than 6 rigid motions");
+        Rd_.alloc(sizeof(double) * R.size());
+        RT_HIP(hipMemcpyAsync(Rd_.p, R.data(), sizeof(double) * R.size(), hipMemcpyHostToDevice, H.s));
+        std::vector<double> diag(n);
+        RT_HIP(hipMemcpy2DAsync(diag.data(), sizeof(double), Ainv.p, sizeof(double) * (size_t)(n + 1), sizeof(double),
+                                n, hipMemcpyDeviceToHost, H.s));
+        RT_HIP(hipStreamSynchronize(H.s));
+        double a = 0;
+        for (double d : diag) a += d;
+        a /= n;
+        gemm(H, rocblas_operation_none, rocblas_operation_transpose, n, n, 6, a, Rd_.as<double>(), n,
+             Rd_.as<double>(), n, 1.0, Ainv.as<double>(), n);
+    }
     // Cholesky and inverse (rocSOLVER, FP64), then mirror the triangle the
     // inverse was written to so the GEMV can read whole rows
     H.dinfo.ensure(sizeof(int));
-    ensure_blas(H);
     auto info_ok = [&](const char* what) {
         int info = 0;
         RT_HIP(hipMemcpyAsync(&info, H.dinfo.p, sizeof(int), hipMemcpyDeviceToHost, H.s));
@@ -822,6 +905,24 @@ static void factor_level(Hier& H, int li, DBuf& Ainv, DBuf& rowslot, DBuf& fbuf,
         throw std::runtime_error("rocsolver_dpotri failed");
     info_ok("inverse");
     launch_mirror_upper(Ainv.as<double>(), n, H.s);
+    if (singular) {
+        // M <- P M P = M - G R^T - R G^T + R (R^T G) R^T, G = M R
+        const double* R = Rd_.as<double>();
+        DBuf G, T, S;
+        G.alloc(sizeof(double) * (size_t)n * 6);
+        T.alloc(sizeof(double) * (size_t)n * 6);
+        S.alloc(sizeof(double) * 36);
+        double* M = Ainv.as<double>();
+        gemm(H, rocblas_operation_none, rocblas_operation_none, n, 6, n, 1.0, M, n, R, n, 0.0, G.as<double>(), n);
+        gemm(H, rocblas_operation_transpose, rocblas_operation_none, 6, 6, n, 1.0, R, n, G.as<double>(), n, 0.0,
+             S.as<double>(), 6);
+        gemm(H, rocblas_operation_none, rocblas_operation_none, n, 6, 6, 1.0, R, n, S.as<double>(), 6, 0.0,
+             T.as<double>(), n);
+        gemm(H, rocblas_operation_none, rocblas_operation_transpose, n, n, 6, -1.0, G.as<double>(), n, R, n, 1.0, M, n);
+        gemm(H, rocblas_operation_none, rocblas_operation_transpose, n, n, 6, -1.0, R, n, G.as<double>(), n, 1.0, M, n);
+        gemm(H, rocblas_operation_none, rocblas_operation_transpose, n, n, 6, 1.0, T.as<double>(), n, R, n, 1.0, M, n);
+        RT_HIP(hipStreamSynchronize(H.s));
+    }
     n3_out = n;
     fbuf.ensure(sizeof(double) * (size_t)std::max(n, 1));
     RT_HIP(hipStreamSynchronize(H.s));
@@ -2184,6 +2285,62 @@ int rt_compliance(rt_hier* h, double* c) {
 // ---------------------------------------------------------------------------
 // worst-case load search (modified power method)
 // ---------------------------------------------------------------------------
+// rigid motions of the load nodes at p in the compact layout [k][c * nl + i]:
+// with normals (nrm[c][i]) each node's 3-vector reduced to its normal part,
+// then modified Gram-Schmidt with dropping as rigid_basis
+static void rigid_basis_load(const std::vector<double>& p, const std::vector<double>* nrm, std::vector<double>& out,
+                             int& kept) {
+    std::vector<double> R;
+    const size_t nl = p.size() / 3, m = 3 * nl;
+    // interleaved basis first (rigid_basis orthonormalises), so build the raw
+    // columns here in the compact layout and orthonormalise the same way
+    R.assign(m * 6, 0.0);
+    for (size_t i = 0; i < nl; i++) {
+        const double x = p[3 * i], y = p[3 * i + 1], z = p[3 * i + 2];
+        const double ph[3][3] = {{0, -z, y}, {z, 0, -x}, {-y, x, 0}};
+        double col[6][3];
+        for (int c = 0; c < 3; c++) {
+            for (int k = 0; k < 6; k++) col[k][c] = 0;
+            col[c][c] = 1.0;
+            for (int k = 0; k < 3; k++) col[3 + k][c] = ph[c][k];
+        }
+        if (nrm)
+            for (int k = 0; k < 6; k++) {
+                const double d = col[k][0] * nrm[0][i] + col[k][1] * nrm[1][i] + col[k][2] * nrm[2][i];
+                for (int c = 0; c < 3; c++) col[k][c] = d * nrm[c][i];
+            }
+        for (int k = 0; k < 6; k++)
+            for (int c = 0; c < 3; c++) R[k * m + c * nl + i] = col[k][c];
+    }
+    bool keep[6];
+    for (int i = 0; i < 6; i++) {
+        double* ci = R.data() + i * m;
+        double old = 0;
+        for (size_t r = 0; r < m; r++) old += ci[r] * ci[r];
+        old = std::sqrt(old);
+        for (int j = 0; j < i; j++) {
+            if (!keep[j]) continue;
+            const double* cj = R.data() + j * m;
+            double w = 0;
+            for (size_t r = 0; r < m; r++) w += ci[r] * cj[r];
+            for (size_t r = 0; r < m; r++) ci[r] -= w * cj[r];
+        }
+        double res = 0;
+        for (size_t r = 0; r < m; r++) res += ci[r] * ci[r];
+        res = std::sqrt(res);
+        keep[i] = !(res < 1e-6 * old || old < 1e-6);
+        if (keep[i])
+            for (size_t r = 0; r < m; r++) ci[r] /= res;
+    }
+    out.clear();
+    kept = 0;
+    for (int i = 0; i < 6; i++)
+        if (keep[i]) {
+            out.insert(out.end(), R.begin() + i * m, R.begin() + (i + 1) * m);
+            kept++;
+        }
+}
+
 // setLoadNodes (projection.cpp:129-269), with-support modes: load nodes are
 // existing, non-support nodes; normals (if given) are normalised per node
 static void set_load_nodes(Hier& H, const uint8_t* load_box, const double* nrm_box) {
@@ -2218,6 +2375,22 @@ static void set_load_nodes(Hier& H, const uint8_t* load_box, const double* nrm_b
     }
     H.pm_fs.alloc(sizeof(double) * 3 * nl);
     H.pm_old.alloc(sizeof(double) * 3 * nl);
+    // the admissible rigid motions of the load nodes for the no-support
+    // modes (projection.cpp:208-257): with normals only their normal part
+    // (Rd -= Nd^T Nd Rd), orthonormalised; stored compact [k][c * nl + i]
+    std::vector<double> pos;
+    pos.reserve(3 * nl);
+    for (size_t v = 0; v < nb; v++) {
+        if (!load_box[v] || !L0.vact[v] || H.fixed_box[v]) continue;
+        pos.push_back((double)(v % nx)); pos.push_back((double)(v / nx % ny)); pos.push_back((double)(v / ((size_t)nx * ny)));
+    }
+    std::vector<double> R;
+    rigid_basis_load(pos, H.load_normals ? nrm : nullptr, R, H.nrd);
+    H.load_rd.release();
+    if (H.nrd) {
+        H.load_rd.alloc(sizeof(double) * R.size());
+        RT_HIP(hipMemcpy(H.load_rd.p, R.data(), sizeof(double) * R.size(), hipMemcpyHostToDevice));
+    }
 }
 
 static const int kPmParts = 256;
@@ -2237,10 +2410,26 @@ static std::pair<double, double> pm_commit(Hier& H, double scale) {
     return {std::sqrt(d), std::sqrt(o)};
 }
 
+// no-support modes (projection.cpp:322-335): the gathered load fs also loses
+// its rigid part, fs -= Rd Rd^T fs; returns |fs|^2 (n2 unchanged with supports)
+static double pm_balance(Hier& H, double n2) {
+    if (has_supports(H) || !H.nrd) return n2;
+    const int m = 3 * H.nload;
+    double* w = H.scal.as<double>() + 8;   // the weights Rd^T fs, on the device
+    const double one = 1.0, zero = 0.0, mone = -1.0;
+    if (rocblas_dgemv(H.blas, rocblas_operation_transpose, m, H.nrd, &one, H.load_rd.as<double>(), m,
+                      H.pm_fs.as<double>(), 1, &zero, w, 1) != rocblas_status_success ||
+        rocblas_dgemv(H.blas, rocblas_operation_none, m, H.nrd, &mone, H.load_rd.as<double>(), m, w, 1, &one,
+                      H.pm_fs.as<double>(), 1) != rocblas_status_success)
+        throw std::runtime_error("rocblas_dgemv failed");
+    return dot<double, double>(H, H.pm_fs.as<double>(), H.pm_fs.as<double>(), (size_t)m);
+}
+
 // forceProject (projection.cu:165-231) of the finest force, in place
 static void force_project(Hier& H, bool scale_to_unit) {
     if (!H.nload) throw std::logic_error("no load nodes set");
-    const double n2 = pm_gather(H, H.f64);
+    ensure_blas(H);
+    const double n2 = pm_balance(H, pm_gather(H, H.f64));
     launch_fill<double>(H.f64.data(), 0.0, (size_t)3 * H.lv[0].g.N, H.s);
     double sc = 1.0;
     if (scale_to_unit) {
@@ -2248,6 +2437,43 @@ static void force_project(Hier& H, bool scale_to_unit) {
         sc = 1.0 / std::sqrt(n2);
     }
     pm_commit(H, sc);
+}
+
+// displacementProject (projection.cpp:374-394): x64 -= (x64 . R_k) R_k for
+// the orthonormal rigid motions R_k of the finest nodes, one after the other
+static void displacement_project(Hier& H) {
+    Level& L0 = H.lv[0];
+    const size_t n = (size_t)3 * L0.g.N;
+    if (H.rigid_f.empty()) {
+        const int nx = L0.g.n[0], ny = L0.g.n[1];
+        const size_t nb = L0.nn_box();
+        std::vector<double> pos;
+        std::vector<size_t> ids;
+        for (size_t v = 0; v < nb; v++)
+            if (L0.vact[v]) {
+                pos.push_back((double)(v % nx)); pos.push_back((double)(v / nx % ny));
+                pos.push_back((double)(v / ((size_t)nx * ny)));
+                ids.push_back(v);
+            }
+        std::vector<double> R;
+        if (rigid_basis(pos, R) != 6) throw std::domain_error("fewer than 6 rigid motions");
+        const size_t m = 3 * ids.size();
+        std::vector<double> box(3 * nb);
+        H.rigid_f.resize(6);
+        for (int k = 0; k < 6; k++) {
+            std::fill(box.begin(), box.end(), 0.0);
+            for (size_t i = 0; i < ids.size(); i++)
+                for (int c = 0; c < 3; c++) box[c * nb + ids[i]] = R[k * m + 3 * i + c];
+            H.rigid_f[k].alloc(L0.g.N);
+            upload_vec<double>(H, L0.g, box.data(), H.rigid_f[k]);
+        }
+    }
+    for (int k = 0; k < 6; k++) {
+        const double w = dot<double, double>(H, H.x64.data(), H.rigid_f[k].data(), n);
+        const double* hv[2] = {H.x64.data(), H.rigid_f[k].data()};
+        const double c[2] = {1.0, -w};
+        launch_lincomb(H.x64.data(), hv, c, 2, n, H.s);
+    }
 }
 
 // modifiedPM (optimization.cpp:123-261), with-support branch, from the current
@@ -2261,11 +2487,14 @@ static double modified_pm(Hier& H, int max_itn, int* steps) {
     launch_fill<double>(H.x64.data(), 0.0, (size_t)3 * H.lv[0].g.N, H.s);
     double fch = 1, rel = 1;
     int itn = 0, nv = 0;
+    const bool free_body = !has_supports(H);
     while (itn++ < max_itn && (fch > 1e-4 || rel > 1e-2)) {
         rel = vcycle_on_solution<T>(H, 1, 1);
         nv++;
+        // no supports: remove the rigid motion (optimization.cpp:189-194)
+        if (free_body) displacement_project(H);
         if (!(rel <= 1e4)) throw std::domain_error("modified power method diverged");
-        const double n2 = pm_gather(H, H.x64);
+        const double n2 = pm_balance(H, pm_gather(H, H.x64));
         if (!(n2 > 0)) throw std::domain_error("projected displacement is zero");
         const auto dn = pm_commit(H, 1.0 / std::sqrt(n2));
         fch = dn.first / dn.second;
@@ -3012,6 +3241,8 @@ int rt_block_lanczos(rt_hier* h, int k, int max_blocks, double tol, double solve
     RT_TRY({
         need(worst_compliance);
         Hier& H = H_(h);
+        if (!has_supports(H))
+            throw std::logic_error("block Lanczos needs supports; the no-support modes use rt_modified_pm");
         int blocks = 0, solves = 0, iters = 0;
         *worst_compliance = block_lanczos(H, k, max_blocks, tol, solve_tol, mode, pre, post, seed, &blocks, &solves,
                                           &iters);

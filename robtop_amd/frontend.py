@@ -318,8 +318,9 @@ class MeshProblem:
 
     def __init__(self, V, F, config, gridreso, youngs=1e6, poisson=0.4, power=3.0, shell_width=0, mode="wsff"):
         from .grid import HierarchyGrid, voxelize_mesh
-        if mode not in ("wsff", "wscf"):
-            raise ValueError("work modes with supports only: wsff (free load directions) or wscf (normal loads)")
+        if mode not in ("wsff", "wscf", "nsff", "nscf"):
+            raise ValueError("work modes: wsff / wscf (with supports), nsff / nscf (without); ff = free load "
+                             "directions, cf = normal loads")
         self.V, self.F, self.cfg, self.mode = V, F, config, mode
         t0 = time.perf_counter()
         bits, R, box = voxelize_mesh(V, F, gridreso)
@@ -335,7 +336,8 @@ class MeshProblem:
         self.node_pos = self.box[0] + idx.transpose(2, 1, 0, 3).reshape(-1, 3) * self.h   # x fastest
         surf = g0.surface_nodes()
         fixed = np.zeros_like(surf)
-        fixed[surf] = config.fixed.contains(self.node_pos[surf])
+        if mode.startswith("ws"):        # the no-support modes leave the body free
+            fixed[surf] = config.fixed.contains(self.node_pos[surf])
         load = np.zeros_like(surf)
         rest = surf & ~fixed
         load[rest] = config.load.contains(self.node_pos[rest])
@@ -347,9 +349,9 @@ class MeshProblem:
         self.hg.set_support(fixed)
         self.hg.set_force(f)
         self.normals = None
-        self.mesh = MeshDistance(V, F) if (mode == "wscf" or shell_width > 0) else None
+        self.mesh = MeshDistance(V, F) if (mode.endswith("cf") or shell_width > 0) else None
         if load.any():
-            if mode == "wscf":
+            if mode.endswith("cf"):
                 nrm = np.zeros((3, surf.size))
                 nrm[:, load] = self.mesh.outward_normals(self.node_pos[load]).T
                 self.normals = nrm
@@ -402,10 +404,12 @@ def main(argv=None):
     ap.add_argument("--max_itn", type=int, default=100)
     ap.add_argument("--min_density", type=float, default=1e-3)
     ap.add_argument("--shell_width", type=float, default=0)
-    ap.add_argument("--workmode", default="wscf", choices=["wsff", "wscf"])
+    ap.add_argument("--workmode", default="wscf", choices=["wsff", "wscf", "nsff", "nscf"])
     ap.add_argument("--method", default="lanczos", choices=["pm", "lanczos"],
                     help="worst-load search: the reference's modified power method or block Lanczos")
     a = ap.parse_args(argv)
+    if a.workmode.startswith("ns") and a.method == "lanczos":
+        a.method = "pm"                  # block Lanczos solves need supports
     from .optimization import Parameter, worst_case_topopt
     cfg = BoundaryConfig.from_file(a.jsonfile)
     mesh = a.meshfile or os.path.join(os.path.dirname(os.path.abspath(a.jsonfile)), cfg.model)

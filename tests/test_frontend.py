@@ -198,14 +198,18 @@ def test_mesh_problem_supports_loads_and_shell(tmp_path):
 
 
 @pytest.mark.gpu
-def test_command_line_driver(tmp_path):
+@pytest.mark.parametrize("mode", ["wscf", "nsff"])
+def test_command_line_driver(tmp_path, mode):
+    """The front end end to end: with supports and normal loads (block
+    Lanczos), and as a free body with balanced free loads (the modified power
+    method, rigid motion removed)."""
     V, F = box_mesh((0, 0, 0), (24, 12, 24))
     write_obj(tmp_path / "cube24x24x12.obj", V, F)
     with open(tmp_path / "config1.json", "w") as fp:
         json.dump(cube_config(), fp)
-    print("[frontend] python -m robtop_amd.frontend, 2 iterations", flush=True)
+    print(f"[frontend] python -m robtop_amd.frontend, {mode}, 2 iterations", flush=True)
     p = subprocess.run([sys.executable, "-m", "robtop_amd.frontend", "--jsonfile", str(tmp_path / "config1.json"),
-                        "--gridreso", "47", "--max_itn", "2", "--workmode", "wscf", "--outdir", str(tmp_path / "out")],
+                        "--gridreso", "47", "--max_itn", "2", "--workmode", mode, "--outdir", str(tmp_path / "out")],
                        capture_output=True, text=True, timeout=300, cwd=ROOT)
     assert p.returncode == 0, p.stderr[-2000:]
     hist = json.load(open(tmp_path / "out" / "history.json"))

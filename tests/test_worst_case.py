@@ -267,3 +267,150 @@ def test_gpu_torus_lanczos_matches_pm():
     cp, _ = g.modified_pm(500)
     assert abs(cl - cp) / cl < 2e-3, (cl, cp, st)
     g.close()
+
+
+# --------------------------------------------------------------------------
+# no-support modes (nsff, nscf): balanced loads and rigid-motion removal
+# (projection.cpp:46-112, :208-257, :322-335, :374-394; optimization.cpp:189-194)
+# --------------------------------------------------------------------------
+def _ns_problem():
+    a, H, fixed, rho, rhop, load, nrm, f0 = _problem()
+    H.set_fixed(np.zeros_like(fixed))              # a free body
+    return a, H, rho, rhop, load, nrm, f0
+
+
+def _rigid(lv0, mask):
+    """orthonormal rigid motions of the nodes in mask (interleaved dofs of the
+    whole node box), by QR"""
+    n = lv0["n"]
+    nn = lv0["nn"]
+    idx = np.nonzero(mask)[0]
+    x, y, z = idx % n[0], idx // n[0] % n[1], idx // (n[0] * n[1])
+    R = np.zeros((3 * nn, 6))
+    for c in range(3):
+        R[3 * idx + c, c] = 1.0
+    # p x e_k columns: rotation k maps p to e_k x p
+    R[3 * idx + 1, 3], R[3 * idx + 2, 3] = -z, y        # e_x x p = (0, -z, y)
+    R[3 * idx + 0, 4], R[3 * idx + 2, 4] = z, -x        # e_y x p = (z, 0, -x)
+    R[3 * idx + 0, 5], R[3 * idx + 1, 5] = -y, x        # e_z x p = (-y, x, 0)
+    q, _ = np.linalg.qr(R)
+    return q
+
+
+def _admissible(lv0, load, normals):
+    """orthonormal basis (interleaved dofs) of the balanced admissible loads:
+    f on the load nodes, along the normals if given, with no net force or
+    moment (R^T f = 0 for every rigid motion R of the whole body)"""
+    nn = lv0["nn"]
+    cols = []
+    for v in np.nonzero(load)[0]:
+        dirs = [np.eye(3)[k] for k in range(3)] if normals is None else [normals[:, v] / np.linalg.norm(normals[:, v])]
+        for d in dirs:
+            x = np.zeros(3 * nn)
+            x[3 * v:3 * v + 3] = d
+            cols.append(x)
+    B = np.array(cols).T
+    Rf = _rigid(lv0, lv0["vact"].astype(bool))
+    _, s, vt = np.linalg.svd(Rf.T @ B)
+    Z = vt[int(np.sum(s > 1e-10 * s[0])):].T
+    return B @ Z
+
+
+@pytest.mark.parametrize("mode", ["nsff", "nscf"])
+def test_ns_force_project_is_the_balanced_projector(mode):
+    a, H, rho, rhop, load, nrm, f0 = _ns_problem()
+    normals = nrm if mode == "nscf" else None
+    lv0 = H.levels[0]
+    H.set_load_nodes(load.astype(np.uint8), normals)
+    H.set_vec(0, 1, f0)
+    H.force_project()
+    got = H.get_vec(0, 1).T.reshape(-1)
+    S = _admissible(lv0, load, normals)
+    ref = S @ (S.T @ f0.T.reshape(-1))
+    assert rel_err(got, ref) < 1e-10
+    # no net force or moment remains
+    Rf = _rigid(lv0, lv0["vact"].astype(bool))
+    assert np.abs(Rf.T @ got).max() < 1e-10 * np.abs(got).max()
+
+
+def _ns_dense_top(a, H, rho, rhop, load, normals):
+    """largest eigenpair of the compliance on the balanced admissible loads,
+    S^T K^+ S, with K^+ b the minimum-norm solution: a statically determinate
+    3-2-1 support pins the rigid motion for a sparse solve, then the rigid part
+    is projected out"""
+    lv0 = H.levels[0]
+    nn = lv0["nn"]
+    fixed = np.zeros(nn, bool)
+    P = bf.Problem(a, bf.from_flat(rho, lv0["e"]), bf.from_flat(fixed, lv0["n"]), rhop=bf.from_flat(rhop, lv0["e"]))
+    K = P.K0.tocsr()
+    act_nodes = np.nonzero(lv0["vact"])[0]
+    n = lv0["n"]
+    xs, ys, zs = act_nodes % n[0], act_nodes // n[0] % n[1], act_nodes // (n[0] * n[1])
+    A = act_nodes[0]
+    B_ = act_nodes[np.argmax(xs - xs[0] - 1e-3 * np.abs(ys - ys[0]) - 1e-3 * np.abs(zs - zs[0]))]
+    C = act_nodes[np.argmax(ys - ys[0] - 1e-3 * np.abs(zs - zs[0]))]
+    pinned = {3 * A, 3 * A + 1, 3 * A + 2, 3 * B_ + 1, 3 * B_ + 2, 3 * C + 2}
+    act = np.array([d for d in np.nonzero(np.repeat(lv0["vact"].astype(bool), 3))[0] if d not in pinned])
+    lu = spla.splu(K[act][:, act].tocsc())
+    Rf = _rigid(lv0, lv0["vact"].astype(bool))
+    S = _admissible(lv0, load, normals)
+    X = np.zeros_like(S)
+    X[act] = lu.solve(S[act])
+    X -= Rf @ (Rf.T @ X)
+    w, V = np.linalg.eigh(S.T @ X)
+    return w[-1], S @ V[:, -1]
+
+
+@pytest.mark.parametrize("mode", ["nsff", "nscf"])
+def test_ns_modified_pm_finds_top_eigenpair(mode):
+    a, H, rho, rhop, load, nrm, f0 = _ns_problem()
+    normals = nrm if mode == "nscf" else None
+    H.update_stencil()
+    H.set_load_nodes(load.astype(np.uint8), normals)
+    H.set_vec(0, 1, f0)
+    c, steps = H.modified_pm(500)
+    assert 1 < steps < 500
+    lam, ftop = _ns_dense_top(a, H, rho, rhop, load, normals)
+    assert abs(c - lam) / lam < 5e-4
+    f = H.get_vec(0, 1).T.reshape(-1)
+    assert abs(abs(f @ ftop) - 1) < 1e-3
+    u = H.get_vec(0, 0).T.reshape(-1)
+    Rf = _rigid(H.levels[0], H.levels[0]["vact"].astype(bool))
+    assert np.abs(Rf.T @ u).max() < 1e-10 * np.abs(u).max()     # no rigid motion in the displacement
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["nsff", "nscf"])
+def test_gpu_ns_modified_pm_matches_oracle(mode):
+    """No supports: the balanced-load projection, the singular coarsest solve
+    (minimum norm) and the rigid-motion removal after each V-cycle, step by
+    step against the oracle (FP64 V-cycle)."""
+    import robtop_amd as rt
+    a, H, rho, rhop, load, nrm, f0 = _ns_problem()
+    normals = nrm if mode == "nscf" else None
+    print(f"[pm] holes3 {mode}, no supports", flush=True)
+    g = rt.HierarchyGrid.genFromMesh(bits_of(a), 32)
+    g.set_material(1e6, 0.4, 3.0)
+    g.set_density(rho)
+    H.set_rhop(g.get_rhop())
+    g.set_precision(64)
+    g.update_stencil()
+    nl = g.set_load_nodes(load.astype(np.uint8), normals)
+    assert nl == int(load.sum())
+    g.set_force(f0)
+    g.force_project()
+    H.set_load_nodes(load.astype(np.uint8), normals)
+    H.set_vec(0, 1, f0)
+    H.force_project()
+    assert rel_err(g.get_force(), H.get_vec(0, 1)) < 1e-12
+    g.set_force(f0)
+    cg, sg = g.modified_pm(500)
+    H.update_stencil()
+    H.set_vec(0, 1, f0)
+    co, so = H.modified_pm(500)
+    print(f"[pm] {mode}: GPU {sg} cycles c {cg:.12e}, oracle {so} cycles c {co:.12e}", flush=True)
+    assert sg == so
+    assert abs(cg - co) / abs(co) < 1e-10
+    assert rel_err(g.get_force(), H.get_vec(0, 1)) < 1e-10
+    assert rel_err(g.get_displacement(), H.get_vec(0, 0)) < 1e-10
+    g.close()
