@@ -1020,11 +1020,13 @@ void launch_pcg_update_x_r(T* x, T* r, const T* p, const T* q, double alpha, siz
     RT_HIP(hipGetLastError());
 }
 
-// reliable-update PCG step: x (FP64) += a p, r (FP32) -= a q; partials of r.r
+// reliable-update PCG step: r (FP32) -= a q with partials of r.r, and, when
+// DX, the FP32 solution increment dx += a p in the same pass.
 // The FP32 streaming kernels of the reliable-update PCG take four values per
 // lane per step (16 B loads) when n is a multiple of 4 (it is: n = 3 * 8 cs),
 // over kStreamBlocks workgroups, so enough bytes are in flight to approach
 // the HBM rate.
+template <bool DX>
 __global__ void __launch_bounds__(256) k_pcg_ru_dxr(float* __restrict__ r, const float* __restrict__ q,
                                                     float* __restrict__ dx, const float* __restrict__ p, float a,
                                                     size_t n4, double* __restrict__ partials,
@@ -1037,12 +1039,15 @@ __global__ void __launch_bounds__(256) k_pcg_ru_dxr(float* __restrict__ r, const
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
         if (chunk && !chunk[i >> 10]) continue;
         float4 ri = r4[i];
-        const float4 qi = q4[i], pi = p4[i];
-        float4 di = d4[i];
+        const float4 qi = q4[i];
         ri.x -= a * qi.x; ri.y -= a * qi.y; ri.z -= a * qi.z; ri.w -= a * qi.w;
-        di.x += a * pi.x; di.y += a * pi.y; di.z += a * pi.z; di.w += a * pi.w;
         r4[i] = ri;
-        d4[i] = di;
+        if constexpr (DX) {
+            const float4 pi = p4[i];
+            float4 di = d4[i];
+            di.x += a * pi.x; di.y += a * pi.y; di.z += a * pi.z; di.w += a * pi.w;
+            d4[i] = di;
+        }
         s += ((double)ri.x * ri.x + (double)ri.y * ri.y) + ((double)ri.z * ri.z + (double)ri.w * ri.w);
     }
     s = blk_sum(s);
@@ -1051,20 +1056,59 @@ __global__ void __launch_bounds__(256) k_pcg_ru_dxr(float* __restrict__ r, const
 void launch_pcg_ru_dxr(float* r, const float* q, float* dx, const float* p, double alpha, size_t n, double* partials,
                        int nparts, hipStream_t s, const uint8_t* chunk) {
     if (n % 4) throw std::logic_error("launch_pcg_ru_dxr: n must be a multiple of 4");
-    hipLaunchKernelGGL(k_pcg_ru_dxr, dim3(nparts), dim3(256), 0, s, r, q, dx, p, (float)alpha, n / 4, partials, chunk);
+    if (dx) {
+        if (!p) throw std::logic_error("launch_pcg_ru_dxr: dx without p");
+        hipLaunchKernelGGL(k_pcg_ru_dxr<true>, dim3(nparts), dim3(256), 0, s, r, q, dx, p, (float)alpha, n / 4,
+                           partials, chunk);
+    } else {
+        hipLaunchKernelGGL(k_pcg_ru_dxr<false>, dim3(nparts), dim3(256), 0, s, r, q, dx, p, (float)alpha, n / 4,
+                           partials, chunk);
+    }
     RT_HIP(hipGetLastError());
 }
 
-// x += dx (FP64), dx = 0
-__global__ void k_pcg_flush_x(double* __restrict__ x, float* __restrict__ dx, size_t n) {
+// x += dx (FP64), dx = 0. With p, the increment a p still pending on dx is
+// added first, in FP32 as the streaming kernels add it: x += (dx + a p).
+__global__ void k_pcg_flush_x(double* __restrict__ x, float* __restrict__ dx, const float* __restrict__ p, float a,
+                              size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        x[i] += (double)dx[i];
+        float d = dx[i];
+        if (p) d += a * p[i];
+        x[i] += (double)d;
         dx[i] = 0.f;
     }
 }
-void launch_pcg_flush_x(double* x, float* dx, size_t n, hipStream_t s) {
+void launch_pcg_flush_x(double* x, float* dx, size_t n, hipStream_t s, const float* p, double alpha) {
     unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
-    hipLaunchKernelGGL(k_pcg_flush_x, dim3(nb), dim3(256), 0, s, x, dx, n);
+    hipLaunchKernelGGL(k_pcg_flush_x, dim3(nb), dim3(256), 0, s, x, dx, p, (float)alpha, n);
+    RT_HIP(hipGetLastError());
+}
+
+// dx += a p, then p = z + b p: the direction update with the previous step's
+// solution increment, deferred from k_pcg_ru_dxr so that p is read once for
+// both (same FP32 operations, so the same bits as updating dx with r)
+__global__ void __launch_bounds__(256) k_pcg_p4_dx(float* __restrict__ p, const float* __restrict__ z, float b,
+                                                   float* __restrict__ dx, float a, size_t n4,
+                                                   const uint8_t* __restrict__ chunk) {
+    float4* p4 = reinterpret_cast<float4*>(p);
+    float4* d4 = reinterpret_cast<float4*>(dx);
+    const float4* z4 = reinterpret_cast<const float4*>(z);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
+        if (chunk && !chunk[i >> 10]) continue;
+        const float4 zi = z4[i];
+        float4 pi = p4[i];
+        float4 di = d4[i];
+        di.x += a * pi.x; di.y += a * pi.y; di.z += a * pi.z; di.w += a * pi.w;
+        pi.x = zi.x + b * pi.x; pi.y = zi.y + b * pi.y; pi.z = zi.z + b * pi.z; pi.w = zi.w + b * pi.w;
+        d4[i] = di;
+        p4[i] = pi;
+    }
+}
+void launch_pcg_p_dx(float* p, const float* z, double beta, float* dx, double alpha, size_t n, hipStream_t s,
+                     const uint8_t* chunk) {
+    if (n % 4) throw std::logic_error("launch_pcg_p_dx: n must be a multiple of 4");
+    hipLaunchKernelGGL(k_pcg_p4_dx, dim3(kStreamBlocks), dim3(256), 0, s, p, z, (float)beta, dx, (float)alpha, n / 4,
+                       chunk);
     RT_HIP(hipGetLastError());
 }
 

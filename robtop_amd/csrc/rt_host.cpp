@@ -1326,8 +1326,9 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
     bool have_true = false;   // rel_true is the true residual of the returned x64
     if (nb > 0 && rel > tol) {
         // The solution increment since the last residual replacement is kept
-        // in FP32 (dx, updated with r) and flushed into x64 at each
-        // replacement and at exit.
+        // in FP32 (dx) and flushed into x64 at each replacement and at exit.
+        // Each step's alpha p is added to dx by the next direction update,
+        // which reads p anyway, or by the flush (apend: the step pending).
         const int ns = kStreamBlocks;
         double best_true = rel_true;
         H.cg_alpha.clear();
@@ -1337,18 +1338,21 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
         double rz = precond_rz(H, pc, post, n);
         launch_convert<float, float>(pc.U.data(), H.p32.data(), n, H.s);
         bool flushed = false;
+        double apend = 0.0;
         for (it = 1; it <= maxit; it++) {
             const double pq = apply_dot<float>(H, H.p32, H.q32);
             if (!(pq > 0)) throw std::domain_error("PCG breakdown: p.Ap <= 0");
             const double alpha = rz / pq;
             H.cg_alpha.push_back(alpha);
-            launch_pcg_ru_dxr(pc.F.data(), H.q32.data(), H.dx32.data(), H.p32.data(), alpha, n,
-                              H.partials.as<double>(), ns, H.s, vec_chunks(H));
+            launch_pcg_ru_dxr(pc.F.data(), H.q32.data(), nullptr, nullptr, alpha, n, H.partials.as<double>(), ns,
+                              H.s, vec_chunks(H));
+            apend = alpha;
             flushed = false;
             const double rn = std::sqrt(sum_partials_host(H, ns));
             rel = rn / nb;
             if (rel <= tol || rn < RT_REPLACE_DROP * rmax) {
-                launch_pcg_flush_x(H.x64.data(), H.dx32.data(), n, H.s);
+                launch_pcg_flush_x(H.x64.data(), H.dx32.data(), n, H.s, H.p32.data(), apend);
+                apend = 0.0;
                 flushed = true;
                 rel_true = true_rel_residual(H);      // r64 = f - K x64
                 if (rel_true <= tol) { rel = rel_true; have_true = true; break; }
@@ -1364,9 +1368,13 @@ static void pcg_reliable(Hier& H, double tol, int maxit, int pre, int post, rt_s
             const double beta = rz_new / rz;
             H.cg_beta.push_back(beta);
             rz = rz_new;
-            launch_pcg_update_p<float, float>(H.p32.data(), pc.U.data(), beta, n, H.s, vec_chunks(H));
+            if (apend != 0.0)
+                launch_pcg_p_dx(H.p32.data(), pc.U.data(), beta, H.dx32.data(), apend, n, H.s, vec_chunks(H));
+            else
+                launch_pcg_update_p<float, float>(H.p32.data(), pc.U.data(), beta, n, H.s, vec_chunks(H));
+            apend = 0.0;
         }
-        if (!flushed) launch_pcg_flush_x(H.x64.data(), H.dx32.data(), n, H.s);
+        if (!flushed) launch_pcg_flush_x(H.x64.data(), H.dx32.data(), n, H.s, H.p32.data(), apend);
         if (it > maxit) it = maxit;
     }
     if (diverged) {
@@ -2855,6 +2863,9 @@ static void pcg_batch_reliable(Hier& H, int nb, double tol, int maxit, int pre, 
     double nf[kBatchMax], rmax[kBatchMax], best[kBatchMax], rz[kBatchMax], pq[kBatchMax], rn[kBatchMax],
         rt[kBatchMax];
     bool act[kBatchMax], bad[kBatchMax], dirty[kBatchMax];
+    // apend[b]: the step whose alpha p is not yet on dx (added by the next
+    // direction update or by the flush, as in pcg_reliable)
+    double apend[kBatchMax] = {};
     for (int b = 0; b < nb; b++) {
         launch_fill<double>(H.bx.at(b), 0.0, n, H.s);
         launch_fill<float>(H.bdx32.at(b), 0.f, n, H.s);
@@ -2906,8 +2917,9 @@ static void pcg_batch_reliable(Hier& H, int nb, double tol, int maxit, int pre, 
         for (int b = 0; b < nb; b++) {
             if (!act[b]) continue;
             if (!(pq[b] > 0)) throw std::domain_error("PCG breakdown: p.Ap <= 0");
-            launch_pcg_ru_dxr(L0.bF.at(b), H.bq32.at(b), H.bdx32.at(b), H.bp32.at(b), rz[b] / pq[b], n,
-                              part + (size_t)b * npa, ns, H.s, vec_chunks(H));
+            apend[b] = rz[b] / pq[b];
+            launch_pcg_ru_dxr(L0.bF.at(b), H.bq32.at(b), nullptr, nullptr, apend[b], n, part + (size_t)b * npa, ns,
+                              H.s, vec_chunks(H));
             dirty[b] = true;
         }
         batch_sums(H, nb, ns, npa, rn);
@@ -2918,7 +2930,8 @@ static void pcg_batch_reliable(Hier& H, int nb, double tol, int maxit, int pre, 
             iters[b] = it;
             rn[b] = std::sqrt(rn[b]);
             if (rn[b] / nf[b] <= tol || rn[b] < RT_REPLACE_DROP * rmax[b] || it == maxit) {
-                launch_pcg_flush_x(H.bx.at(b), H.bdx32.at(b), n, H.s);
+                launch_pcg_flush_x(H.bx.at(b), H.bdx32.at(b), n, H.s, H.bp32.at(b), apend[b]);
+                apend[b] = 0.0;
                 dirty[b] = false;
                 rep[b] = true;
                 nrep++;
@@ -2948,12 +2961,17 @@ static void pcg_batch_reliable(Hier& H, int nb, double tol, int maxit, int pre, 
         batch_sums(H, nb, ns, npa, rzn);
         for (int b = 0; b < nb; b++) {
             if (!act[b]) continue;
-            launch_pcg_update_p<float, float>(H.bp32.at(b), L0.bU.at(b), rzn[b] / rz[b], n, H.s, vec_chunks(H));
+            if (apend[b] != 0.0)
+                launch_pcg_p_dx(H.bp32.at(b), L0.bU.at(b), rzn[b] / rz[b], H.bdx32.at(b), apend[b], n, H.s,
+                                vec_chunks(H));
+            else
+                launch_pcg_update_p<float, float>(H.bp32.at(b), L0.bU.at(b), rzn[b] / rz[b], n, H.s, vec_chunks(H));
+            apend[b] = 0.0;
             rz[b] = rzn[b];
         }
     }
     for (int b = 0; b < nb; b++)
-        if (dirty[b]) launch_pcg_flush_x(H.bx.at(b), H.bdx32.at(b), n, H.s);
+        if (dirty[b]) launch_pcg_flush_x(H.bx.at(b), H.bdx32.at(b), n, H.s, H.bp32.at(b), apend[b]);
     // a vector whose FP32 recurrence failed: all of them again with FP64
     // Krylov vectors (pcg_batch zero-starts every vector; the others' results
     // are kept and restored)

@@ -205,11 +205,15 @@ void launch_sum_partials(const double* partials, int nparts, double* out, hipStr
 template <typename T, typename TZ>
 void launch_pcg_update_x_r(T* x, T* r, const T* p, const T* q, double alpha, size_t n, double* partials, int nparts,
                            TZ* rz_out, hipStream_t s);
-// reliable-update PCG (FP32 vectors): r -= alpha q, dx += alpha p, partials of r.r
+// reliable-update PCG (FP32 vectors): r -= alpha q with partials of r.r, and
+// dx += alpha p unless dx is null
 void launch_pcg_ru_dxr(float* r, const float* q, float* dx, const float* p, double alpha, size_t n, double* partials,
                        int nparts, hipStream_t s, const uint8_t* chunk = nullptr);
-// x64 += dx, dx = 0
-void launch_pcg_flush_x(double* x, float* dx, size_t n, hipStream_t s);
+// x64 += dx (+ alpha p, a pending increment, when p is given), dx = 0
+void launch_pcg_flush_x(double* x, float* dx, size_t n, hipStream_t s, const float* p = nullptr, double alpha = 0.0);
+// dx += alpha p, then p = z + beta p (FP32, n a multiple of 4)
+void launch_pcg_p_dx(float* p, const float* z, double beta, float* dx, double alpha, size_t n, hipStream_t s,
+                     const uint8_t* chunk = nullptr);
 // streaming kernels' grid size (partials written by the reducing ones)
 #ifndef RT_STREAM_BLOCKS
 #define RT_STREAM_BLOCKS 2048
