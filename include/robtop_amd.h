@@ -209,7 +209,8 @@ int rt_set_preconditioner(rt_hier* h, int coarse_sweep_mult, int gamma);
  * solves) stops one level above the coarsest and solves that level exactly
  * (FP64 dense inverse) when it has at most max_dofs unknowns (default 6000:
  * at 511 x 256 x 256 this is the 4131-unknown level of 16 x 8 x 8 elements).
- * 0 keeps the whole hierarchy. rt_vcycle always visits every level. */
+ * 0 keeps the whole hierarchy; at most 30000 (a 7.2 GB inverse), else
+ * RT_EARG. rt_vcycle always visits every level. */
 int rt_set_coarse_direct_dofs(rt_hier* h, int max_dofs);
 /* Sweep multiplier of one coarse level (1 .. levels-1) in the preconditioner's
  * cycle, overriding rt_set_preconditioner's coarse_sweep_mult there (0 puts

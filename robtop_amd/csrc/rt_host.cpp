@@ -2053,7 +2053,7 @@ int rt_set_level_sweeps(rt_hier* h, int level, int mult) {
 
 int rt_set_coarse_direct_dofs(rt_hier* h, int max_dofs) {
     RT_TRY({
-        if (max_dofs < 0) throw std::invalid_argument("max_dofs must be >= 0");
+        if (max_dofs < 0 || max_dofs > 30000) throw std::invalid_argument("max_dofs must be 0..30000");
         Hier& H = H_(h);
         H.pc_direct_dofs = max_dofs;
         H.drop_graph();

@@ -25,6 +25,7 @@ def _cantilever(size=(64, 32, 32), seed=0):
 
 
 def test_early_exact_solve_engages_by_level_size():
+    import robtop_amd as rt
     hg, f, _ = _cantilever()
     n = len(hg)
     _, nodes = hg.num_active(n - 2)
@@ -35,6 +36,10 @@ def test_early_exact_solve_engages_by_level_size():
     hg.set_coarse_direct_dofs(3 * nodes - 1)
     assert hg.preconditioner_levels() == n
     hg.set_coarse_direct_dofs(0)
+    assert hg.preconditioner_levels() == n
+    for bad in (-1, 30001):
+        with pytest.raises(rt.RobtopError):
+            hg.set_coarse_direct_dofs(bad)
     assert hg.preconditioner_levels() == n
     hg.close()
 
