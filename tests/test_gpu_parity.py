@@ -243,6 +243,42 @@ def test_coarse_gs_residual_coarsest(pair, bits):
 
 
 @pytest.mark.parametrize("bits", [64, 32])
+def test_coarse_gs_rows_of_two_waves(bits):
+    """A coarse level whose rows hold 65 nodes of a colour (the x4 level of a
+    512 x 8 x 8-element bar is 129 x 3 x 3 nodes, as at 511 x 256 x 256): rows
+    longer than a wave, so the one-colour sliced smoother runs (the colour-pair
+    launch needs rows of at most 64), forward and reverse sweeps, against the
+    oracle."""
+    from robtop_amd.workloads import box_bits, cube_side
+    R = cube_side(512, 8, 8)
+    bb = box_bits(R, 512, 8, 8)
+    print(f"[pair] bar 512x8x8 bits={bits}", flush=True)
+    o = oracle.Hierarchy(R, bb)
+    g = rt.HierarchyGrid.genFromMesh(bb, R)
+    o.set_material(1e6, 0.4, 3.0)
+    g.set_material(1e6, 0.4, 3.0)
+    fixed, f, rho = problem_inputs(o.levels[0], 3)
+    o.set_fixed(fixed)
+    g.set_support(fixed)
+    o.set_rho(rho)
+    g.set_density(rho)
+    o.set_rhop(g.get_rhop())
+    assert int(o.levels[1]["n"][0]) == 129
+    g.update_stencil(); o.update_stencil()
+    g.set_precision(bits)
+    lv = o.levels[1]
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal((3, lv["nn"])) * lv["vact"][None]
+    f1 = rng.standard_normal((3, lv["nn"])) * lv["vact"][None]
+    g[1].setDisplacement(u); g[1].setForce(f1)
+    o.set_vec(1, 0, u); o.set_vec(1, 1, f1)
+    for rev in (False, True):
+        g[1].gs_relax(2, reverse=rev); o.gs_relax(1, 2, rev)
+        assert rel_err(g[1].getDisplacement(), o.get_vec(1, 0)) < TOL[bits] * 10, rev
+    g.close()
+
+
+@pytest.mark.parametrize("bits", [64, 32])
 def test_vcycle(pair, bits):
     g, o = pair.g, pair.o
     g.update_stencil(); o.update_stencil()
