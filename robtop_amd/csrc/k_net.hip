@@ -13,6 +13,7 @@
 #include "net_internal.h"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace rt {
 
@@ -134,6 +135,136 @@ __global__ void __launch_bounds__(256) k_conv(Dims D, int flip, const uint16_t* 
     }
 }
 
+// The same convolution with the input staged in LDS: a workgroup computes a
+// 16 x 16 tile of one plane (wave w: rows 4w .. 4w + 3). For each of the three
+// input planes the taps read (dz = -1, 0, 1, the taps' outer order) it stages
+// the plane's 18 x 18 positions around the tile (zeros outside the box) for KS
+// input channels at a time, and the waves read their A fragments from LDS
+// instead of gathering them through L1 once per tap. With one slice
+// (CIN <= 32) the products run in the order of k_conv<..., 4>, so the results
+// are the same bits; with CIN = 64 each plane's nine taps run per 32-channel
+// slice.
+constexpr int kLdsTile = 16;
+constexpr int kLdsHX = kLdsTile + 2, kLdsHP = kLdsHX * kLdsHX;
+constexpr int kConvLdsMinTiles = 256;
+
+template <int CIN, int COUT, int EPI>
+__global__ void __launch_bounds__(256) k_conv_lds(Dims D, int flip, const uint16_t* __restrict__ X,
+                                                  const uint16_t* __restrict__ W, const float* __restrict__ bias,
+                                                  const uint16_t* __restrict__ aux, void* __restrict__ Yv) {
+    constexpr int KS = CIN >= 32 ? 32 : 16;   // channels per slice: one MFMA K step
+    constexpr int KL = CIN >= 32 ? 8 : 4;     // channels per lane and product
+    constexpr int CH = KS / 8;                // 16-byte chunks per staged position
+    __shared__ __attribute__((aligned(16))) uint16_t tile[kLdsHP * KS];
+    const int nx = D.n[0], ny = D.n[1], nz = D.n[2];
+    const int tx = (nx + kLdsTile - 1) / kLdsTile, ty = (ny + kLdsTile - 1) / kLdsTile;
+    const int bt = blockIdx.x;
+    const int x0 = (bt % tx) * kLdsTile, y0 = ((bt / tx) % ty) * kLdsTile, z = bt / (tx * ty);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m = lane & 15, kq = lane >> 4;
+    f4 acc[4][COUT / 16];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int nb = 0; nb < COUT / 16; nb++) acc[r][nb] = f4{0.f, 0.f, 0.f, 0.f};
+    const int sg = flip ? -1 : 1;
+    for (int pl = 0; pl < 3; pl++) {
+        const int gz = z + sg * (pl - 1);
+        const bool zin = gz >= 0 && gz < nz;
+        for (int sl = 0; sl < CIN / KS; sl++) {
+            if (pl | sl) __syncthreads();   // every wave has read the previous stage
+            for (int i = threadIdx.x; i < kLdsHP * CH; i += 256) {
+                const int pos = i / CH, c = i % CH;
+                const int gx = x0 - 1 + pos % kLdsHX, gy = y0 - 1 + pos / kLdsHX;
+                uint4 v = {0u, 0u, 0u, 0u};
+                if (zin && gx >= 0 && gx < nx && gy >= 0 && gy < ny)
+                    v = *(const uint4*)(X + ((size_t)gx + (size_t)nx * ((size_t)gy + (size_t)ny * gz)) * CIN +
+                                        sl * KS + c * 8);
+                *(uint4*)(tile + pos * KS + c * 8) = v;
+            }
+            __syncthreads();
+            for (int t = 9 * pl; t < 9 * pl + 9; t++) {
+                const int dx = sg * (t % 3 - 1), dy = sg * ((t / 3) % 3 - 1);
+                const int hb = (4 * w + dy + 1) * kLdsHX + m + dx + 1;   // row 0 of the wave
+                const uint16_t* wp = W + ((size_t)t * COUT + m) * CIN + sl * KS + kq * KL;
+                if constexpr (CIN >= 32) {
+                    b8 a[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) a[r] = *(const b8*)(tile + (hb + r * kLdsHX) * KS + kq * KL);
+#pragma unroll
+                    for (int nb = 0; nb < COUT / 16; nb++) {
+                        const b8 b = *(const b8*)(wp + (size_t)nb * 16 * CIN);
+#pragma unroll
+                        for (int r = 0; r < 4; r++)
+                            acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], b, acc[r][nb], 0, 0, 0);
+                    }
+                } else {
+                    s4 a[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) a[r] = *(const s4*)(tile + (hb + r * kLdsHX) * KS + kq * KL);
+#pragma unroll
+                    for (int nb = 0; nb < COUT / 16; nb++) {
+                        const s4 b = *(const s4*)(wp + (size_t)nb * 16 * CIN);
+#pragma unroll
+                        for (int r = 0; r < 4; r++)
+                            acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[r], b, acc[r][nb], 0, 0, 0);
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int y = y0 + 4 * w + r;
+        if (y >= ny) break;
+        const size_t prow = (size_t)nx * ((size_t)y + (size_t)ny * z);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int xo = x0 + 4 * kq + j;
+            if (xo >= nx) continue;
+            const size_t po = prow + xo;
+#pragma unroll
+            for (int nb = 0; nb < COUT / 16; nb++) {
+                const int co = nb * 16 + m;
+                float v = acc[r][nb][j];
+                if (EPI == EPI_RELU || EPI == EPI_RELU_SKIP || EPI == EPI_F32) v += bias[co];
+                if (EPI == EPI_RELU_SKIP) v += bf2f(aux[po * COUT + co]);
+                if (EPI == EPI_RELU || EPI == EPI_RELU_SKIP) v = fmaxf(v, 0.f);
+                if (EPI == EPI_MASK) v = bf2f(aux[po * COUT + co]) > 0.f ? v : 0.f;
+                if (EPI == EPI_F32 || EPI == EPI_RAW_F32)
+                    ((float*)Yv)[po * COUT + co] = v;
+                else
+                    ((uint16_t*)Yv)[po * COUT + co] = f2bf(v);
+            }
+        }
+    }
+}
+
+template <int CIN, int COUT>
+static void conv_epi_lds(int epi, Dims D, int flip, const uint16_t* X, const uint16_t* W, const float* b,
+                         const uint16_t* aux, void* Y, hipStream_t s) {
+    const size_t tiles = (size_t)((D.n[0] + kLdsTile - 1) / kLdsTile) * ((D.n[1] + kLdsTile - 1) / kLdsTile) * D.n[2];
+    const dim3 grid((unsigned)tiles), blk(256);
+    switch (epi) {
+        case EPI_RELU: hipLaunchKernelGGL((k_conv_lds<CIN, COUT, EPI_RELU>), grid, blk, 0, s, D, flip, X, W, b, aux, Y); break;
+        case EPI_RELU_SKIP: hipLaunchKernelGGL((k_conv_lds<CIN, COUT, EPI_RELU_SKIP>), grid, blk, 0, s, D, flip, X, W, b, aux, Y); break;
+        case EPI_F32: hipLaunchKernelGGL((k_conv_lds<CIN, COUT, EPI_F32>), grid, blk, 0, s, D, flip, X, W, b, aux, Y); break;
+        case EPI_MASK: hipLaunchKernelGGL((k_conv_lds<CIN, COUT, EPI_MASK>), grid, blk, 0, s, D, flip, X, W, b, aux, Y); break;
+        case EPI_RAW_F32: hipLaunchKernelGGL((k_conv_lds<CIN, COUT, EPI_RAW_F32>), grid, blk, 0, s, D, flip, X, W, b, aux, Y); break;
+        default: throw std::invalid_argument("conv: unknown epilogue");
+    }
+}
+
+// the LDS-staged convolution on the boxes large enough for it (on by default;
+// RT_CONV_LDS=0 in the environment keeps k_conv, for comparison)
+static bool conv_lds_on() {
+    static const bool on = [] {
+        const char* e = std::getenv("RT_CONV_LDS");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 template <int CIN, int COUT, int MT>
 static void conv_epi_mt(int epi, Dims D, int flip, const uint16_t* X, const uint16_t* W, const float* b,
                         const uint16_t* aux, void* Y, hipStream_t s) {
@@ -152,7 +283,9 @@ template <int CIN, int COUT>
 static void conv_epi(int epi, Dims D, int flip, const uint16_t* X, const uint16_t* W, const float* b,
                      const uint16_t* aux, void* Y, hipStream_t s) {
     const size_t waves4 = (size_t)((D.n[0] + 15) / 16) * ((D.n[1] + 3) / 4) * D.n[2];
-    if (waves4 >= (size_t)kConvMinWaves) conv_epi_mt<CIN, COUT, 4>(epi, D, flip, X, W, b, aux, Y, s);
+    const size_t tiles = (size_t)((D.n[0] + kLdsTile - 1) / kLdsTile) * ((D.n[1] + kLdsTile - 1) / kLdsTile) * D.n[2];
+    if (conv_lds_on() && tiles >= (size_t)kConvLdsMinTiles) conv_epi_lds<CIN, COUT>(epi, D, flip, X, W, b, aux, Y, s);
+    else if (waves4 >= (size_t)kConvMinWaves) conv_epi_mt<CIN, COUT, 4>(epi, D, flip, X, W, b, aux, Y, s);
     else conv_epi_mt<CIN, COUT, 1>(epi, D, flip, X, W, b, aux, Y, s);
 }
 

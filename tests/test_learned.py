@@ -27,9 +27,11 @@ def rel(a, b):
     return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
 
 
-# (32, 32, 64) is large enough for the four-row-tile convolution kernels at the
-# full resolution; the others exercise the one-row kernels and ragged rows
-@pytest.mark.parametrize("shape", [(7, 9, 21), (2, 3, 4), (5, 8, 33), (32, 32, 64)])
+# (32, 32, 64) and (21, 37, 70) are large enough for the LDS-staged
+# convolution at the full resolution (the second with ragged 16 x 16 tiles) and
+# the four-row-tile kernel at the half resolution; the others exercise the
+# one-row kernels and ragged rows
+@pytest.mark.parametrize("shape", [(7, 9, 21), (2, 3, 4), (5, 8, 33), (32, 32, 64), (21, 37, 70)])
 def test_gpu_net_forward_matches_oracle(shape):
     print(f"[learned] forward {shape}", flush=True)
     rng = np.random.default_rng(sum(shape))
