@@ -435,14 +435,17 @@ __global__ void k_pool(int C, Dims F, const uint16_t* __restrict__ X, uint16_t* 
         Y[i] = f2bf(v / (float)children(F, qx, qy, qz));
     }
 }
+// nearest upsampling, eight channels (16 bytes) per thread: C is a multiple of 16
 __global__ void k_upsample(int C, Dims F, const uint16_t* __restrict__ X, uint16_t* __restrict__ Y) {
     const Dims Hd = half(F);
-    const size_t n = (size_t)F.n[0] * F.n[1] * F.n[2] * C;
+    const int c8 = C >> 3;
+    const size_t n = (size_t)F.n[0] * F.n[1] * F.n[2] * c8;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int c = (int)(i % C);
+        const int c = (int)(i % c8) * 8;
+        const size_t p = i / c8;
         int x, y, z;
-        decode(i / C, F, x, y, z);
-        Y[i] = X[lin(Hd, x >> 1, y >> 1, z >> 1) * C + c];
+        decode(p, F, x, y, z);
+        *(uint4*)(Y + p * C + c) = *(const uint4*)(X + lin(Hd, x >> 1, y >> 1, z >> 1) * C + c);
     }
 }
 __global__ void k_upsample_t(int C, Dims F, const float* __restrict__ Hf, const uint16_t* __restrict__ act,
@@ -483,7 +486,8 @@ void launch_pool(int c, const int dims[3], const uint16_t* X, uint16_t* Y, hipSt
     RT_HIP(hipGetLastError());
 }
 void launch_upsample(int c, const int dims[3], const uint16_t* X, uint16_t* Y, hipStream_t s) {
-    hipLaunchKernelGGL(k_upsample, dim3(nblocks(npos(dims) * c)), dim3(256), 0, s, c, mk(dims), X, Y);
+    if (c % 16) throw std::invalid_argument("upsample: channels must be a multiple of 16");
+    hipLaunchKernelGGL(k_upsample, dim3(nblocks(npos(dims) * (c / 8))), dim3(256), 0, s, c, mk(dims), X, Y);
     RT_HIP(hipGetLastError());
 }
 void launch_upsample_t(int c, const int dims[3], const float* H, const uint16_t* act, uint16_t* G, hipStream_t s) {
