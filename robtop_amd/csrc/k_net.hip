@@ -420,19 +420,34 @@ __device__ __forceinline__ int children(const Dims& F, int qx, int qy, int qz) {
     return (2 * qx + 1 < F.n[0] ? 2 : 1) * (2 * qy + 1 < F.n[1] ? 2 : 1) * (2 * qz + 1 < F.n[2] ? 2 : 1);
 }
 
+// 2x2x2 average pool, eight channels (16 bytes) per thread: C is a multiple of 16
 __global__ void k_pool(int C, Dims F, const uint16_t* __restrict__ X, uint16_t* __restrict__ Y) {
     const Dims Hd = half(F);
-    const size_t n = (size_t)Hd.n[0] * Hd.n[1] * Hd.n[2] * C;
+    const int c8 = C >> 3;
+    const size_t n = (size_t)Hd.n[0] * Hd.n[1] * Hd.n[2] * c8;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int c = (int)(i % C);
+        const int c = (int)(i % c8) * 8;
+        const size_t q = i / c8;
         int qx, qy, qz;
-        decode(i / C, Hd, qx, qy, qz);
-        float v = 0.f;
+        decode(q, Hd, qx, qy, qz);
+        float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         for (int k = 0; k < 8; k++) {
             const int x = 2 * qx + (k & 1), y = 2 * qy + ((k >> 1) & 1), z = 2 * qz + (k >> 2);
-            if (x < F.n[0] && y < F.n[1] && z < F.n[2]) v += bf2f(X[lin(F, x, y, z) * C + c]);
+            if (x < F.n[0] && y < F.n[1] && z < F.n[2]) {
+                const uint4 w = *(const uint4*)(X + lin(F, x, y, z) * C + c);
+                const uint32_t u[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    v[2 * j] += bf2f((uint16_t)(u[j] & 0xffffu));
+                    v[2 * j + 1] += bf2f((uint16_t)(u[j] >> 16));
+                }
+            }
         }
-        Y[i] = f2bf(v / (float)children(F, qx, qy, qz));
+        const float nc = (float)children(F, qx, qy, qz);
+        uint32_t o[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) o[j] = (uint32_t)f2bf(v[2 * j] / nc) | ((uint32_t)f2bf(v[2 * j + 1] / nc) << 16);
+        *(uint4*)(Y + q * C + c) = uint4{o[0], o[1], o[2], o[3]};
     }
 }
 // nearest upsampling, eight channels (16 bytes) per thread: C is a multiple of 16
@@ -482,7 +497,8 @@ __global__ void k_pool_t(int C, Dims F, const float* __restrict__ Hf, const uint
 static size_t half_pos(const int d[3]) { return (size_t)((d[0] + 1) / 2) * ((d[1] + 1) / 2) * ((d[2] + 1) / 2); }
 
 void launch_pool(int c, const int dims[3], const uint16_t* X, uint16_t* Y, hipStream_t s) {
-    hipLaunchKernelGGL(k_pool, dim3(nblocks(half_pos(dims) * c)), dim3(256), 0, s, c, mk(dims), X, Y);
+    if (c % 16) throw std::invalid_argument("pool: channels must be a multiple of 16");
+    hipLaunchKernelGGL(k_pool, dim3(nblocks(half_pos(dims) * (c / 8))), dim3(256), 0, s, c, mk(dims), X, Y);
     RT_HIP(hipGetLastError());
 }
 void launch_upsample(int c, const int dims[3], const uint16_t* X, uint16_t* Y, hipStream_t s) {
