@@ -168,21 +168,38 @@ __global__ void __launch_bounds__(256) k_conv_lds(Dims D, int flip, const uint16
 #pragma unroll
         for (int nb = 0; nb < COUT / 16; nb++) acc[r][nb] = f4{0.f, 0.f, 0.f, 0.f};
     const int sg = flip ? -1 : 1;
-    for (int pl = 0; pl < 3; pl++) {
-        const int gz = z + sg * (pl - 1);
+    // stage st = (plane pl, slice sl), pl-major; the next stage's global loads
+    // are issued into registers before this stage's products and written to
+    // LDS after them, so their latency overlaps the matrix-core work
+    constexpr int NSL = CIN / KS, NST = 3 * NSL;
+    constexpr int NPT = (kLdsHP * CH + 255) / 256;   // staged chunks per thread
+    uint4 pre[NPT];
+    auto fetch = [&](int st) {
+        const int gz = z + sg * (st / NSL - 1), sl = st % NSL;
         const bool zin = gz >= 0 && gz < nz;
-        for (int sl = 0; sl < CIN / KS; sl++) {
-            if (pl | sl) __syncthreads();   // every wave has read the previous stage
-            for (int i = threadIdx.x; i < kLdsHP * CH; i += 256) {
-                const int pos = i / CH, c = i % CH;
-                const int gx = x0 - 1 + pos % kLdsHX, gy = y0 - 1 + pos / kLdsHX;
-                uint4 v = {0u, 0u, 0u, 0u};
-                if (zin && gx >= 0 && gx < nx && gy >= 0 && gy < ny)
-                    v = *(const uint4*)(X + ((size_t)gx + (size_t)nx * ((size_t)gy + (size_t)ny * gz)) * CIN +
-                                        sl * KS + c * 8);
-                *(uint4*)(tile + pos * KS + c * 8) = v;
+#pragma unroll
+        for (int j = 0; j < NPT; j++) {
+            const int i = threadIdx.x + 256 * j;
+            const int pos = i / CH, c = i % CH;
+            const int gx = x0 - 1 + pos % kLdsHX, gy = y0 - 1 + pos / kLdsHX;
+            pre[j] = uint4{0u, 0u, 0u, 0u};
+            if (i < kLdsHP * CH && zin && gx >= 0 && gx < nx && gy >= 0 && gy < ny)
+                pre[j] = *(const uint4*)(X + ((size_t)gx + (size_t)nx * ((size_t)gy + (size_t)ny * gz)) * CIN +
+                                         sl * KS + c * 8);
+        }
+    };
+    fetch(0);
+    for (int st = 0; st < NST; st++) {
+        const int pl = st / NSL, sl = st % NSL;
+        {
+            if (st) __syncthreads();   // every wave has read the previous stage
+#pragma unroll
+            for (int j = 0; j < NPT; j++) {
+                const int i = threadIdx.x + 256 * j;
+                if (i < kLdsHP * CH) *(uint4*)(tile + (i / CH) * KS + (i % CH) * 8) = pre[j];
             }
             __syncthreads();
+            if (st + 1 < NST) fetch(st + 1);
             for (int t = 9 * pl; t < 9 * pl + 9; t++) {
                 const int dx = sg * (t % 3 - 1), dy = sg * ((t / 3) % 3 - 1);
                 const int hb = (4 * w + dy + 1) * kLdsHX + m + dx + 1;   // row 0 of the wave
